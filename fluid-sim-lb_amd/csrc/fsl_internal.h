@@ -1,0 +1,188 @@
+// Internal declarations shared by the HIP translation units of libfsl.
+//
+// Storage model. The domain [0,n)^3 plus every voxel the reference can touch
+// ([-1, n] on each axis) is tiled by 8^3 leaves; leaf (a,b,c) holds voxels
+// [8a-8, 8a) x [8b-8, 8b) x [8c-8, 8c), so there are L = n/8 + 2 leaves per
+// axis. Every field is a dense array over all leaves in leaf-major order
+// (512 voxels per leaf, voxel offset (x<<6)|(y<<3)|z, the OpenVDB LeafNode
+// order). Work is proportional to the active-leaf list, not to the box:
+// kernels run one workgroup per listed leaf, and leaves off the list hold
+// zeros, which is the OpenVDB background value, so a read anywhere returns
+// what the reference's tree would return.
+#ifndef FSL_INTERNAL_H_
+#define FSL_INTERNAL_H_
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "fsl.h"
+
+namespace fsl {
+
+constexpr int kLeaf = 512;           // voxels per leaf
+constexpr uint32_t kDead = 0xFFFFFFFFu;  // particle key of a deleted particle
+constexpr int kAir = 0, kFluid = 1, kSolid = 2;
+
+struct Dom {
+  int n[3];   // cells per axis
+  int L[3];   // leaves per axis, n/8 + 2
+  int T[3];   // tiles (reference partitions) per axis
+};
+
+__host__ __device__ inline int64_t vox_of(const Dom &d, int i, int j, int k) {
+  const int a = i + 8, b = j + 8, c = k + 8;
+  const int64_t leaf = (int64_t(a >> 3) * d.L[1] + (b >> 3)) * d.L[2] + (c >> 3);
+  return (leaf << 9) | ((a & 7) << 6) | ((b & 7) << 3) | (c & 7);
+}
+__host__ __device__ inline bool in_store(const Dom &d, int i, int j, int k) {
+  return i >= -8 && j >= -8 && k >= -8 && i < d.n[0] + 8 && j < d.n[1] + 8 && k < d.n[2] + 8;
+}
+__host__ __device__ inline bool in_bbox(const Dom &d, int i, int j, int k) {
+  return i >= 0 && j >= 0 && k >= 0 && i < d.n[0] && j < d.n[1] && k < d.n[2];
+}
+__host__ __device__ inline void leaf_origin(const Dom &d, int64_t leaf, int o[3]) {
+  const int64_t l12 = int64_t(d.L[1]) * d.L[2];
+  o[0] = int(leaf / l12) * 8 - 8;
+  o[1] = int((leaf / d.L[2]) % d.L[1]) * 8 - 8;
+  o[2] = int(leaf % d.L[2]) * 8 - 8;
+}
+__host__ __device__ inline int64_t leaf_count(const Dom &d) {
+  return int64_t(d.L[0]) * d.L[1] * d.L[2];
+}
+// Reference partition rank of a cell (MetaGrid::ComputeRank, metagrid.cc:84-87).
+__host__ __device__ inline int tile_of(const Dom &d, int i, int j, int k) {
+  int c[3] = {i, j, k};
+  int t[3];
+  for (int a = 0; a < 3; ++a) {
+    int w = d.n[a] / d.T[a];
+    int x = c[a] < 0 ? 0 : (c[a] >= d.n[a] ? d.n[a] - 1 : c[a]);
+    t[a] = x / w;
+  }
+  return t[0] + d.T[0] * (t[1] + d.T[1] * t[2]);
+}
+
+// Scalars of one PCG solve, resident on the device.
+struct PcgState {
+  double rz, rz_old, rr, pAp, alpha, beta;
+  double bnorm2, thr, rr_true, b_unused;
+  int it, done, status, max_iter;
+  int stop_mode, precision, refresh_due, pad;
+  unsigned cnt_a, cnt_b, cnt_r, cnt_x;
+};
+
+// Device-side error flags of the grid stages.
+struct DevFlags {
+  unsigned zero_diag;
+  unsigned vel_update;
+  unsigned cfl_bits;
+  unsigned pad;
+  unsigned long long n_rows;
+  unsigned long long tile_rows[64];
+};
+
+struct Particles {
+  float *a[9];  // px py pz vx vy vz gx gy gz
+};
+
+}  // namespace fsl
+
+struct fsl_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  fsl::Dom dom{};
+  int64_t nleaves = 0, nvox = 0;
+  float gravity = 9.8f, force[3] = {0, 9.8f, 0}, flip = 0.95f;
+  // Grids.
+  float *v[3] = {}, *u[3] = {};
+  uint8_t *vflag = nullptr;  // bit0: active in grid_velocity_, bit1: in grid_velocity_update_
+  uint8_t *marker = nullptr, *solid = nullptr, *lab = nullptr;
+  float *div = nullptr, *pres = nullptr;
+  // PCG vectors (sized for doubles; FP32 modes use the first half).
+  void *x = nullptr, *r = nullptr, *p[2] = {}, *Ap = nullptr;
+  double *x64 = nullptr;
+  void *partials = nullptr;
+  fsl::PcgState *pcg = nullptr;
+  fsl::PcgState *pcg_host = nullptr;  // pinned mirror
+  int pcg_precision = FSL_PCG_FP32, pcg_stop = FSL_STOP_ABS_RR, pcg_refresh = 32;
+  int pcg_pbuf = 0;            // which p buffer holds the current direction
+  bool pcg_ready = false;
+  // Leaves.
+  uint32_t *lhas = nullptr;   // leaf holds particles
+  uint32_t *lact = nullptr;   // leaf active (26-dilation of lhas)
+  uint32_t *lfluid = nullptr; // leaf holds a FLUID cell
+  uint32_t *lscan = nullptr;  // scan scratch over leaves (nleaves + 1)
+  int32_t *act_list = nullptr, *fluid_list = nullptr;
+  int64_t n_act = 0, n_fluid = 0;
+  bool bc_applied = false, grid_ready = false, outside_deleted = false;
+  // Particles.
+  int64_t cap = 0, np = 0;
+  fsl::Particles P{}, Q{};
+  uint32_t *key = nullptr, *perm = nullptr;
+  uint32_t *vcount = nullptr, *vstart = nullptr;  // per voxel; vstart has nvox + 1
+  uint32_t *scan_tmp = nullptr;
+  int64_t scan_tmp_len = 0;
+  // Flags and staging.
+  fsl::DevFlags *flags = nullptr, *flags_host = nullptr;
+  uint32_t *host_u32 = nullptr;  // pinned scratch for small readbacks
+  std::string err;
+};
+
+namespace fsl {
+
+// scan.hip
+int64_t scan_tmp_elems(int64_t n);
+hipError_t exclusive_scan(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *tmp,
+                          hipStream_t s);
+
+// particles.hip
+hipError_t launch_seed(fsl_ctx *c, const uint8_t *mask_dev, const uint32_t *offs, int ppc,
+                       uint32_t seed, const float vel[3]);
+hipError_t launch_advect(fsl_ctx *c, float dt, bool drop_outside);
+hipError_t launch_sort_tail(fsl_ctx *c, uint32_t n_alive);
+hipError_t launch_mask_count(fsl_ctx *c, const uint8_t *mask_dev, int64_t n, uint32_t *cnt);
+hipError_t launch_leaf_flags(fsl_ctx *c);
+hipError_t launch_compact(fsl_ctx *c, const uint32_t *flag, const uint32_t *pos, int32_t *list);
+hipError_t launch_p2g(fsl_ctx *c);
+hipError_t launch_g2p(fsl_ctx *c);
+hipError_t launch_cell_counts_box(fsl_ctx *c, int32_t *out_dev);
+
+// grid.hip
+hipError_t launch_clear_leaves(fsl_ctx *c, const int32_t *list, int64_t n);
+hipError_t launch_bc(fsl_ctx *c);
+hipError_t launch_save(fsl_ctx *c);
+hipError_t launch_gravity(fsl_ctx *c, float dt);
+hipError_t launch_divergence(fsl_ctx *c);
+hipError_t launch_pressure_update(fsl_ctx *c);
+hipError_t launch_velocity_update(fsl_ctx *c);
+hipError_t launch_extrapolate(fsl_ctx *c);
+hipError_t launch_cfl(fsl_ctx *c);
+hipError_t launch_box_vec(fsl_ctx *c, int which, float *out3, uint8_t *act, bool to_box);
+hipError_t launch_box_scalar(fsl_ctx *c, float *grid, float *box, bool to_box);
+hipError_t launch_box_u8(fsl_ctx *c, uint8_t *grid, int32_t *box, bool to_box);
+hipError_t launch_all_leaves(fsl_ctx *c);
+
+// pcg.hip
+int pcg_prepare(fsl_ctx *c, const fsl_solve_params *prm);
+int pcg_run(fsl_ctx *c, const fsl_solve_params *prm, fsl_solve_stats *st);
+int pcg_iterate_once(fsl_ctx *c);
+int pcg_finish(fsl_ctx *c, fsl_solve_stats *st);
+int pcg_true_residual(fsl_ctx *c, double *rr_true);
+
+// adjoint.hip
+int project_backward(fsl_ctx *c, const fsl_solve_params *prm, const float *grad_out,
+                     float *grad_in, fsl_solve_stats *st);
+
+}  // namespace fsl
+
+#define FSL_HIP(expr)                                                        \
+  do {                                                                       \
+    hipError_t e_ = (expr);                                                  \
+    if (e_ != hipSuccess) {                                                  \
+      ctx->err = std::string(#expr) + ": " + hipGetErrorString(e_);          \
+      return FSL_ERR_HIP;                                                    \
+    }                                                                        \
+  } while (0)
+
+#endif  // FSL_INTERNAL_H_

@@ -1,0 +1,386 @@
+// Grid-stage kernels: clear, boundary conditions, save, gravity, divergence
+// and solver labels, pressure-gradient update, velocity update, boundary
+// extrapolation, CFL, and box-format conversion. One workgroup of 512
+// threads per active leaf, one thread per voxel.
+#include "device_util.h"
+#include "fsl_internal.h"
+
+namespace fsl {
+
+namespace {
+
+constexpr int kThreads = 256;
+inline unsigned blocks_for(int64_t n, int t) { return unsigned((n + t - 1) / t); }
+
+// Voxel handled by this thread of a per-leaf kernel; false past the list.
+__device__ inline bool leaf_voxel(const Dom &d, const int32_t *list, int64_t nlist, int64_t &v,
+                                  int c[3]) {
+  const int64_t w = xcd_work(nlist);
+  if (w >= nlist) return false;
+  const int64_t leaf = list[w];
+  int o[3];
+  leaf_origin(d, leaf, o);
+  const int t = threadIdx.x;
+  c[0] = o[0] + (t >> 6); c[1] = o[1] + ((t >> 3) & 7); c[2] = o[2] + (t & 7);
+  v = (leaf << 9) | t;
+  return true;
+}
+
+__global__ __launch_bounds__(512) void k_clear(const int32_t *list, int64_t nlist, float *v0,
+                                               float *v1, float *v2, float *u0, float *u1,
+                                               float *u2, uint8_t *vflag, uint8_t *marker,
+                                               uint8_t *lab, float *div, float *pres) {
+  const int64_t w = xcd_work(nlist);
+  if (w >= nlist) return;
+  const int64_t v = (int64_t(list[w]) << 9) | threadIdx.x;
+  v0[v] = 0; v1[v] = 0; v2[v] = 0; u0[v] = 0; u1[v] = 0; u2[v] = 0;
+  vflag[v] = 0; marker[v] = 0; lab[v] = 0; div[v] = 0; pres[v] = 0;
+}
+
+__device__ inline uint8_t solid_at(const Dom &d, const uint8_t *solid, int i, int j, int k) {
+  return in_store(d, i, j, k) ? solid[vox_of(d, i, j, k)] : uint8_t(0);
+}
+
+// ApplyBoundaryConditions (flip_simulation.cc:656-713): a solid voxel is
+// marked SOLID; the normal component on a face between a solid and a
+// non-solid cell is zeroed when the face is active. Face d of voxel c lies
+// between c - e_d and c, so it is zeroed iff solid(c) != solid(c - e_d).
+__global__ __launch_bounds__(512) void k_bc(Dom d, const int32_t *list, int64_t nlist,
+                                            const uint8_t *solid, uint8_t *marker,
+                                            const uint8_t *vflag, float *vx, float *vy,
+                                            float *vz, int save, float *ux, float *uy, float *uz,
+                                            uint8_t *vflag_out) {
+  int64_t v;
+  int c[3];
+  if (!leaf_voxel(d, list, nlist, v, c)) return;
+  const uint8_t s = solid[v];
+  if (s) marker[v] = kSolid;
+  const uint8_t f = vflag[v];
+  float a = vx[v], b = vy[v], e = vz[v];
+  if (f & 1) {
+    if (s != solid_at(d, solid, c[0] - 1, c[1], c[2])) a = 0.0f;
+    if (s != solid_at(d, solid, c[0], c[1] - 1, c[2])) b = 0.0f;
+    if (s != solid_at(d, solid, c[0], c[1], c[2] - 1)) e = 0.0f;
+    vx[v] = a; vy[v] = b; vz[v] = e;
+  }
+  if (save) {  // SaveGridVelocities (flip_simulation.cc:490-496)
+    ux[v] = a; uy[v] = b; uz[v] = e;
+    vflag_out[v] = (f & 1) ? uint8_t(3) : uint8_t(0);
+  }
+}
+
+__global__ __launch_bounds__(512) void k_save(Dom d, const int32_t *list, int64_t nlist,
+                                              const float *vx, const float *vy, const float *vz,
+                                              float *ux, float *uy, float *uz, uint8_t *vflag) {
+  int64_t v;
+  int c[3];
+  if (!leaf_voxel(d, list, nlist, v, c)) return;
+  ux[v] = vx[v]; uy[v] = vy[v]; uz[v] = vz[v];
+  vflag[v] = (vflag[v] & 1) ? uint8_t(3) : uint8_t(0);
+}
+
+// AddGravity (flip_simulation.cc:505-515) on active faces.
+__global__ __launch_bounds__(512) void k_gravity(Dom d, const int32_t *list, int64_t nlist,
+                                                 const uint8_t *vflag, float *vx, float *vy,
+                                                 float *vz, float ax, float ay, float az) {
+#pragma clang fp contract(off)
+  int64_t v;
+  int c[3];
+  if (!leaf_voxel(d, list, nlist, v, c)) return;
+  if (!(vflag[v] & 1)) return;
+  vx[v] = vx[v] + ax;
+  vy[v] = vy[v] - ay;
+  vz[v] = vz[v] + az;
+}
+
+// ComputeDivergence (flip_simulation.cc:715-736) fused with the solver
+// labels: lab = marker | diag << 2, diag = count of FLUID or AIR neighbours
+// (CreateLaplacianMatrix, solver.cc:214-231). Also flags leaves holding FLUID
+// cells and counts rows per tile (solver.cc:80, metagrid.cc:84-87).
+__global__ __launch_bounds__(512) void k_divergence(Dom d, const int32_t *list, int64_t nlist,
+                                                    const uint8_t *marker, const float *vx,
+                                                    const float *vy, const float *vz,
+                                                    float *div, uint8_t *lab, uint32_t *lfluid,
+                                                    DevFlags *flags) {
+#pragma clang fp contract(off)
+  __shared__ unsigned s_cnt[16];
+  int64_t v;
+  int c[3];
+  if (!leaf_voxel(d, list, nlist, v, c)) return;
+  const uint8_t m = marker[v];
+  const bool fluid = m == kFluid && in_bbox(d, c[0], c[1], c[2]);
+  float dv = 0.0f;
+  uint8_t l = m;
+  if (fluid) {
+    const int64_t xp = vox_of(d, c[0] + 1, c[1], c[2]), yp = vox_of(d, c[0], c[1] + 1, c[2]),
+                  zp = vox_of(d, c[0], c[1], c[2] + 1);
+    dv = vx[xp] - vx[v];
+    dv = dv + (vy[yp] - vy[v]);
+    dv = dv + (vz[zp] - vz[v]);
+    int diag = 0;
+    const int off[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const uint8_t nm = marker[vox_of(d, c[0] + off[q][0], c[1] + off[q][1], c[2] + off[q][2])];
+      diag += (nm == kFluid || nm == kAir) ? 1 : 0;
+    }
+    if (diag == 0) atomicOr(&flags->zero_diag, 1u);
+    l = uint8_t(kFluid | (diag << 2));
+  }
+  // A FLUID marker outside the partition box is not a row of this system
+  // (in the reference it is a ghost column); label 3 keeps it out of the
+  // solve while still counting as FLUID in its neighbours' diagonals.
+  if (m == kFluid && !fluid) l = 3;
+  div[v] = dv;
+  lab[v] = l;
+  const unsigned cnt = unsigned(block_sum<unsigned>(fluid ? 1u : 0u, s_cnt));
+  if (threadIdx.x == 0) {
+    lfluid[v >> 9] = cnt ? 1u : 0u;
+    if (cnt) {
+      atomicAdd(&flags->n_rows, (unsigned long long)cnt);
+      atomicAdd(&flags->tile_rows[tile_of(d, c[0], c[1], c[2]) & 63], (unsigned long long)cnt);
+    }
+  }
+}
+
+// UpdateVelocityFromPressure (flip_simulation.cc:738-778) over faces
+// [0, n] of active leaves; AIR pressure is 0 (the cleared grid).
+__global__ __launch_bounds__(512) void k_pressure_update(Dom d, const int32_t *list,
+                                                         int64_t nlist, const uint8_t *marker,
+                                                         const float *pres, float *vx, float *vy,
+                                                         float *vz, uint8_t *vflag) {
+  int64_t v;
+  int c[3];
+  if (!leaf_voxel(d, list, nlist, v, c)) return;
+  if (c[0] < 0 || c[1] < 0 || c[2] < 0 || c[0] > d.n[0] || c[1] > d.n[1] || c[2] > d.n[2]) return;
+  const uint8_t m = marker[v];
+  if (m == kSolid) return;
+  const float pc = pres[v];
+  bool upd = false;
+  float *g[3] = {vx, vy, vz};
+  const int off[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int64_t w = vox_of(d, c[0] - off[a][0], c[1] - off[a][1], c[2] - off[a][2]);
+    const uint8_t mw = marker[w];
+    if ((mw == kFluid || m == kFluid) && mw != kSolid) {
+      g[a][v] = g[a][v] + (pc - pres[w]);
+      upd = true;
+    }
+  }
+  if (upd) vflag[v] = vflag[v] | 1;
+}
+
+// GetVelocityUpdate (flip_simulation.cc:780-801): dv = v - saved on faces
+// [0, n] active in both; elsewhere the saved value stays.
+__global__ __launch_bounds__(512) void k_velocity_update(Dom d, const int32_t *list,
+                                                         int64_t nlist, const uint8_t *vflag,
+                                                         const float *vx, const float *vy,
+                                                         const float *vz, float *ux, float *uy,
+                                                         float *uz, DevFlags *flags) {
+  int64_t v;
+  int c[3];
+  if (!leaf_voxel(d, list, nlist, v, c)) return;
+  if (c[0] < 0 || c[1] < 0 || c[2] < 0 || c[0] > d.n[0] || c[1] > d.n[1] || c[2] > d.n[2]) return;
+  const uint8_t f = vflag[v];
+  if ((f & 1) && !(f & 2)) atomicOr(&flags->vel_update, 1u);
+  if ((f & 3) == 3) {
+    ux[v] = vx[v] - ux[v];
+    uy[v] = vy[v] - uy[v];
+    uz[v] = vz[v] - uz[v];
+  }
+}
+
+// VectorGrid::ExtrapolateToBoundary (vector_grid.cc:302-381) for one axis
+// (lower and upper layers), on v and dv. The reference runs x, y, z in
+// sequence and later axes read values earlier axes wrote, so each axis is
+// its own launch. Destinations outside active leaves only ever receive zeros
+// and are skipped.
+__global__ void k_extrapolate(Dom d, int axis, const uint32_t *lact, float *vx, float *vy,
+                              float *vz, float *ux, float *uy, float *uz, uint8_t *vflag) {
+  const int a = axis == 0 ? 1 : 0, b = axis == 2 ? 1 : 2;
+  const int na = d.n[a] + 1, nb = d.n[b] + 1;
+  const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q >= 2 * int64_t(na) * nb) return;
+  const int side = int(q / (int64_t(na) * nb));
+  const int ia = int((q / nb) % na), ib = int(q % nb);
+  int out[3], in[3];
+  out[axis] = side ? d.n[axis] : -1;
+  in[axis] = side ? d.n[axis] - 1 : 0;
+  out[a] = in[a] = ia;
+  out[b] = in[b] = ib;
+  const int64_t vo = vox_of(d, out[0], out[1], out[2]);
+  if (!lact[vo >> 9]) return;
+  const int64_t vi = vox_of(d, in[0], in[1], in[2]);
+  float *gv[3] = {vx, vy, vz};
+  float *gu[3] = {ux, uy, uz};
+  gv[a][vo] = gv[a][vi]; gv[b][vo] = gv[b][vi];
+  gu[a][vo] = gu[a][vi]; gu[b][vo] = gu[b][vi];
+  vflag[vo] = 3;
+}
+
+// CFL (flip_simulation.cc:459-488): max |v| over active faces inside the
+// partition box, skipping samples past the last face of each component.
+__global__ __launch_bounds__(512) void k_cfl(Dom d, const int32_t *list, int64_t nlist,
+                                             const uint8_t *vflag, const float *vx,
+                                             const float *vy, const float *vz, unsigned *bits) {
+  __shared__ float s_m[8];
+  int64_t v;
+  int c[3];
+  if (!leaf_voxel(d, list, nlist, v, c)) return;
+  float m = 0.0f;
+  if (in_bbox(d, c[0], c[1], c[2]) && (vflag[v] & 1)) {
+    const int e0 = d.n[0] - 1, e1 = d.n[1] - 1, e2 = d.n[2] - 1;
+    if (c[1] != e1 && c[2] != e2) m = fmaxf(m, fabsf(vx[v]));
+    if (c[0] != e0 && c[2] != e2) m = fmaxf(m, fabsf(vy[v]));
+    if (c[0] != e0 && c[1] != e1) m = fmaxf(m, fabsf(vz[v]));
+  }
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float t = 0.0f;
+    for (int q = 0; q < 8; ++q) t = fmaxf(t, s_m[q]);
+    atomicMax(bits, __float_as_uint(t));  // non-negative floats order as uints
+  }
+}
+
+// Box format <-> leaf storage.
+__device__ inline bool box_coord(const Dom &d, int64_t q, int &i, int &j, int &k) {
+  const int bx = d.n[0] + 2, by = d.n[1] + 2, bz = d.n[2] + 2;
+  if (q >= int64_t(bx) * by * bz) return false;
+  i = int(q / (int64_t(by) * bz)) - 1; j = int((q / bz) % by) - 1; k = int(q % bz) - 1;
+  return true;
+}
+__global__ void k_box_vec(Dom d, float *g0, float *g1, float *g2, uint8_t *vflag, int bit,
+                          float *box, uint8_t *act, int to_box) {
+  const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  int i, j, k;
+  if (!box_coord(d, q, i, j, k)) return;
+  const int64_t v = vox_of(d, i, j, k);
+  if (to_box) {
+    box[3 * q] = g0[v]; box[3 * q + 1] = g1[v]; box[3 * q + 2] = g2[v];
+    if (act) act[q] = (vflag[v] >> bit) & 1;
+  } else {
+    g0[v] = box[3 * q]; g1[v] = box[3 * q + 1]; g2[v] = box[3 * q + 2];
+    const uint8_t on = act[q] ? uint8_t(1u << bit) : uint8_t(0);
+    vflag[v] = uint8_t((vflag[v] & ~(1u << bit)) | on);
+  }
+}
+__global__ void k_box_scalar(Dom d, float *g, float *box, int to_box) {
+  const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  int i, j, k;
+  if (!box_coord(d, q, i, j, k)) return;
+  const int64_t v = vox_of(d, i, j, k);
+  if (to_box) box[q] = g[v];
+  else g[v] = box[q];
+}
+__global__ void k_box_u8(Dom d, uint8_t *g, int32_t *box, int to_box) {
+  const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  int i, j, k;
+  if (!box_coord(d, q, i, j, k)) return;
+  const int64_t v = vox_of(d, i, j, k);
+  if (to_box) box[q] = g[v];
+  else g[v] = uint8_t(box[q]);
+}
+
+__global__ void k_all_leaves(int64_t n, uint32_t *lact, uint32_t *lhas, int32_t *list) {
+  const int64_t l = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (l >= n) return;
+  lact[l] = 1u;
+  lhas[l] = 0u;
+  list[l] = int32_t(l);
+}
+
+}  // namespace
+
+hipError_t launch_clear_leaves(fsl_ctx *c, const int32_t *list, int64_t n) {
+  if (n == 0) return hipSuccess;
+  k_clear<<<xcd_grid(n), 512, 0, c->stream>>>(list, n, c->v[0], c->v[1], c->v[2], c->u[0],
+                                               c->u[1], c->u[2], c->vflag, c->marker, c->lab,
+                                               c->div, c->pres);
+  return hipGetLastError();
+}
+hipError_t launch_bc(fsl_ctx *c) {
+  if (c->n_act == 0) return hipSuccess;
+  k_bc<<<xcd_grid(c->n_act), 512, 0, c->stream>>>(c->dom, c->act_list, c->n_act, c->solid,
+                                                   c->marker, c->vflag, c->v[0], c->v[1], c->v[2],
+                                                   0, nullptr, nullptr, nullptr, nullptr);
+  return hipGetLastError();
+}
+hipError_t launch_save(fsl_ctx *c) {
+  if (c->n_act == 0) return hipSuccess;
+  k_save<<<xcd_grid(c->n_act), 512, 0, c->stream>>>(c->dom, c->act_list, c->n_act, c->v[0],
+                                                     c->v[1], c->v[2], c->u[0], c->u[1], c->u[2],
+                                                     c->vflag);
+  return hipGetLastError();
+}
+hipError_t launch_gravity(fsl_ctx *c, float dt) {
+  if (c->n_act == 0) return hipSuccess;
+  const float ax = dt * c->force[0], ay = dt * c->force[1], az = dt * c->force[2];
+  k_gravity<<<xcd_grid(c->n_act), 512, 0, c->stream>>>(c->dom, c->act_list, c->n_act, c->vflag,
+                                                        c->v[0], c->v[1], c->v[2], ax, ay, az);
+  return hipGetLastError();
+}
+hipError_t launch_divergence(fsl_ctx *c) {
+  if (c->n_act == 0) return hipSuccess;
+  k_divergence<<<xcd_grid(c->n_act), 512, 0, c->stream>>>(c->dom, c->act_list, c->n_act,
+                                                           c->marker, c->v[0], c->v[1], c->v[2],
+                                                           c->div, c->lab, c->lfluid, c->flags);
+  return hipGetLastError();
+}
+hipError_t launch_pressure_update(fsl_ctx *c) {
+  if (c->n_act == 0) return hipSuccess;
+  k_pressure_update<<<xcd_grid(c->n_act), 512, 0, c->stream>>>(
+      c->dom, c->act_list, c->n_act, c->marker, c->pres, c->v[0], c->v[1], c->v[2], c->vflag);
+  return hipGetLastError();
+}
+hipError_t launch_velocity_update(fsl_ctx *c) {
+  if (c->n_act == 0) return hipSuccess;
+  k_velocity_update<<<xcd_grid(c->n_act), 512, 0, c->stream>>>(
+      c->dom, c->act_list, c->n_act, c->vflag, c->v[0], c->v[1], c->v[2], c->u[0], c->u[1],
+      c->u[2], c->flags);
+  return hipGetLastError();
+}
+hipError_t launch_extrapolate(fsl_ctx *c) {
+  for (int axis = 0; axis < 3; ++axis) {
+    const int a = axis == 0 ? 1 : 0, b = axis == 2 ? 1 : 2;
+    const int64_t n = 2 * int64_t(c->dom.n[a] + 1) * (c->dom.n[b] + 1);
+    k_extrapolate<<<blocks_for(n, kThreads), kThreads, 0, c->stream>>>(
+        c->dom, axis, c->lact, c->v[0], c->v[1], c->v[2], c->u[0], c->u[1], c->u[2], c->vflag);
+  }
+  return hipGetLastError();
+}
+hipError_t launch_cfl(fsl_ctx *c) {
+  if (c->n_act == 0) return hipSuccess;
+  k_cfl<<<xcd_grid(c->n_act), 512, 0, c->stream>>>(c->dom, c->act_list, c->n_act, c->vflag,
+                                                    c->v[0], c->v[1], c->v[2], &c->flags->cfl_bits);
+  return hipGetLastError();
+}
+hipError_t launch_box_vec(fsl_ctx *c, int which, float *out3, uint8_t *act, bool to_box) {
+  float **g = which == FSL_GRID_VELOCITY ? c->v : c->u;
+  const int64_t nb = int64_t(c->dom.n[0] + 2) * (c->dom.n[1] + 2) * (c->dom.n[2] + 2);
+  k_box_vec<<<blocks_for(nb, kThreads), kThreads, 0, c->stream>>>(
+      c->dom, g[0], g[1], g[2], c->vflag, which == FSL_GRID_VELOCITY ? 0 : 1, out3, act,
+      to_box ? 1 : 0);
+  return hipGetLastError();
+}
+hipError_t launch_box_scalar(fsl_ctx *c, float *grid, float *box, bool to_box) {
+  const int64_t nb = int64_t(c->dom.n[0] + 2) * (c->dom.n[1] + 2) * (c->dom.n[2] + 2);
+  k_box_scalar<<<blocks_for(nb, kThreads), kThreads, 0, c->stream>>>(c->dom, grid, box,
+                                                                      to_box ? 1 : 0);
+  return hipGetLastError();
+}
+hipError_t launch_box_u8(fsl_ctx *c, uint8_t *grid, int32_t *box, bool to_box) {
+  const int64_t nb = int64_t(c->dom.n[0] + 2) * (c->dom.n[1] + 2) * (c->dom.n[2] + 2);
+  k_box_u8<<<blocks_for(nb, kThreads), kThreads, 0, c->stream>>>(c->dom, grid, box,
+                                                                  to_box ? 1 : 0);
+  return hipGetLastError();
+}
+hipError_t launch_all_leaves(fsl_ctx *c) {
+  k_all_leaves<<<blocks_for(c->nleaves, kThreads), kThreads, 0, c->stream>>>(
+      c->nleaves, c->lact, c->lhas, c->act_list);
+  return hipGetLastError();
+}
+
+}  // namespace fsl

@@ -1,0 +1,517 @@
+// Particle-side kernels: seeding, advection + counting sort by cell, leaf
+// activation, particle-to-grid (P2G) and grid-to-particle (G2P).
+//
+// Particles live in nine SoA float arrays sorted by cell (voxel key in the
+// leaf-major layout), so a leaf's particles are one contiguous range and a
+// cell's particles are a contiguous sub-range [vstart[v], vstart[v+1]).
+// Within a cell, particles keep their previous order (the sort is stable),
+// as the reference's StepInGrid re-adds particles in traversal order
+// (particles.cc:489-544); results are therefore deterministic run to run.
+#include "device_util.h"
+#include "fsl_internal.h"
+
+namespace fsl {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+// ---------------------------------------------------------------- seeding
+__device__ inline void philox10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+  for (int r = 0; r < 10; ++r) {
+    if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+    const uint32_t hi0 = __umulhi(M0, c[0]), lo0 = M0 * c[0];
+    const uint32_t hi1 = __umulhi(M1, c[2]), lo1 = M1 * c[2];
+    const uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+  }
+}
+__device__ inline float jitter(int i, uint32_t x) {
+  const float u = float(x >> 8) * (1.0f / 16777216.0f);
+  const float p = __fadd_rn(float(i), u);
+  const float top = __uint_as_float(__float_as_uint(float(i + 1)) - 1u);
+  return fminf(p, top);
+}
+
+// One thread per fluid cell (mask over [0,n)^3, x-major); writes `ppc`
+// particles at offs[cell] * ppc.
+__global__ void k_seed(Dom d, const uint8_t *mask, const uint32_t *offs, int ppc, uint32_t seed,
+                       float v0, float v1, float v2, Particles P) {
+  const int64_t cell = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t ncell = int64_t(d.n[0]) * d.n[1] * d.n[2];
+  if (cell >= ncell || !mask[cell]) return;
+  const int i = int(cell / (int64_t(d.n[1]) * d.n[2]));
+  const int j = int((cell / d.n[2]) % d.n[1]);
+  const int k = int(cell % d.n[2]);
+  const int64_t base = int64_t(offs[cell]) * ppc;
+  for (int s = 0; s < ppc; ++s) {
+    uint32_t c[4] = {uint32_t(cell), uint32_t(s), 0u, 0u};
+    philox10(c, seed, 0u);
+    const int64_t q = base + s;
+    P.a[0][q] = jitter(i, c[0]);
+    P.a[1][q] = jitter(j, c[1]);
+    P.a[2][q] = jitter(k, c[2]);
+    P.a[3][q] = v0; P.a[4][q] = v1; P.a[5][q] = v2;
+    P.a[6][q] = v0; P.a[7][q] = v1; P.a[8][q] = v2;
+  }
+}
+
+__global__ void k_mask_to_count(const uint8_t *mask, int64_t n, uint32_t *cnt) {
+  const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q < n) cnt[q] = mask[q] ? 1u : 0u;
+}
+
+// ------------------------------------------------------ advection and sort
+// Particles::StepInGrid (particles.cc:504-543) + DeleteParticlesInSolids
+// (particles.cc:202-219). Particles outside the partition box were deleted by
+// DeleteOutsideParticles after the previous P2G; they are dropped here.
+__global__ void k_advect(Dom d, int64_t np, Particles P, float dt, const uint8_t *solid,
+                         uint32_t *key, uint32_t *vcount, int drop_outside) {
+#pragma clang fp contract(off)
+  const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q >= np) return;
+  float p[3] = {P.a[0][q], P.a[1][q], P.a[2][q]};
+  if (drop_outside) {
+    const int a = int(floorf(p[0])), b = int(floorf(p[1])), c = int(floorf(p[2]));
+    if (!in_bbox(d, a, b, c)) { key[q] = kDead; return; }
+  }
+  int id[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float m = P.a[6 + a][q] * dt;
+    p[a] = p[a] + m;
+    id[a] = int(floorf(p[a]));
+    if (id[a] < 0) { id[a] = 0; p[a] = 0.0f; }
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+    if (id[a] >= d.n[a]) { id[a] = d.n[a]; p[a] = float(d.n[a]); }
+  P.a[0][q] = p[0]; P.a[1][q] = p[1]; P.a[2][q] = p[2];
+  const int64_t v = vox_of(d, id[0], id[1], id[2]);
+  if (solid[v]) { key[q] = kDead; return; }
+  key[q] = uint32_t(v);
+  atomicAdd(&vcount[v], 1u);
+}
+
+// Counting-sort scatter. vcount returns to zero, ready for the next step.
+__global__ void k_scatter(int64_t np, const uint32_t *key, const uint32_t *vstart,
+                          uint32_t *vcount, uint32_t *perm) {
+  const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q >= np) return;
+  const uint32_t k = key[q];
+  if (k == kDead) return;
+  const uint32_t slot = vstart[k] + atomicSub(&vcount[k], 1u) - 1u;
+  perm[slot] = uint32_t(q);
+}
+
+// Restore the previous order inside each cell (stable sort): the thread at a
+// cell's first slot insertion-sorts the cell's old indices.
+__global__ void k_cell_sort(uint32_t n, const uint32_t *key, const uint32_t *vstart,
+                            uint32_t *perm) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const uint32_t k = key[perm[s]];
+  const uint32_t b = vstart[k], e = vstart[k + 1];
+  if (s != b || e - b < 2) return;
+  for (uint32_t x = b + 1; x < e; ++x) {
+    const uint32_t val = perm[x];
+    uint32_t y = x;
+    while (y > b && perm[y - 1] > val) { perm[y] = perm[y - 1]; --y; }
+    perm[y] = val;
+  }
+}
+
+__global__ void k_gather(uint32_t n, const uint32_t *perm, Particles in, Particles out) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const uint32_t q = perm[s];
+#pragma unroll
+  for (int a = 0; a < 9; ++a) out.a[a][s] = in.a[a][q];
+}
+
+// ------------------------------------------------------------- leaf lists
+__global__ void k_leaf_has(int64_t nleaves, const uint32_t *vstart, uint32_t *lhas) {
+  const int64_t l = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (l >= nleaves) return;
+  lhas[l] = vstart[(l + 1) * kLeaf] > vstart[l * kLeaf] ? 1u : 0u;
+}
+// A leaf is active when it or one of its 26 neighbours holds particles:
+// everything the reference writes within a step lies within two cells of a
+// particle's cell, hence in such a leaf.
+__global__ void k_leaf_dilate(Dom d, const uint32_t *lhas, uint32_t *lact) {
+  const int64_t l = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (l >= leaf_count(d)) return;
+  const int a = int(l / (int64_t(d.L[1]) * d.L[2])), b = int((l / d.L[2]) % d.L[1]),
+            c = int(l % d.L[2]);
+  uint32_t act = 0;
+  for (int x = a - 1; x <= a + 1; ++x)
+    for (int y = b - 1; y <= b + 1; ++y)
+      for (int z = c - 1; z <= c + 1; ++z) {
+        if (x < 0 || y < 0 || z < 0 || x >= d.L[0] || y >= d.L[1] || z >= d.L[2]) continue;
+        act |= lhas[(int64_t(x) * d.L[1] + y) * d.L[2] + z];
+      }
+  lact[l] = act;
+}
+
+__global__ void k_compact(int64_t n, const uint32_t *flag, const uint32_t *pos, int32_t *list) {
+  const int64_t l = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (l < n && flag[l]) list[pos[l]] = int32_t(l);
+}
+
+// ------------------------------------------------------------------ P2G
+// TransferParticlesToGrid (flip_simulation.cc:808-923) restated as a gather:
+// one workgroup per active leaf owns the leaf's 512 face samples of each
+// component. It visits the 10^3 cells of the leaf and its one-cell ring (the
+// only cells whose particles reach the leaf's samples, flip_simulation.cc:
+// 936-1009); each thread sums its cell's particles in registers over the 18
+// samples a cell can reach per component, then the 18 offsets are committed
+// to LDS one phase at a time. Within a phase every cell targets a different
+// sample, so no atomics are needed and the order is fixed.
+template <int D>
+struct Axes;  // the two axes other than D
+template <> struct Axes<0> { static constexpr int a = 1, b = 2; };
+template <> struct Axes<1> { static constexpr int a = 0, b = 2; };
+template <> struct Axes<2> { static constexpr int a = 0, b = 1; };
+
+// Weight of corner (cD, ca, cb) for component D, or 0 when out of range.
+template <int D>
+__device__ inline float corner(const float wt[2][2][2], int cD, int ca, int cb) {
+  if (ca < 0 || ca > 1 || cb < 0 || cb > 1) return 0.0f;
+  int c[3];
+  c[D] = cD; c[Axes<D>::a] = ca; c[Axes<D>::b] = cb;
+  return wt[c[0]][c[1]][c[2]];
+}
+
+// Accumulate one particle into the 18 sample slots (slot = sd*9 + sa*3 + sb;
+// sample offset from the cell: +sd along D, sa-1 along a, sb-1 along b).
+// Weights follow the reference's factored form (flip_simulation.cc:977-990).
+template <int D>
+__device__ inline void p2g_particle(const float pos[3], const int c[3], float vel, float avw[18],
+                                    float aw[18], uint32_t &cls) {
+#pragma clang fp contract(off)
+  float lower[3] = {float(c[0]), float(c[1]), float(c[2])};
+  int h[3] = {0, 0, 0};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (a == D) continue;
+    lower[a] = lower[a] + 0.5f;
+    if (pos[a] < lower[a]) lower[a] = lower[a] - 1.0f;
+    else h[a] = 1;
+  }
+  const float x = pos[0] - lower[0], y = pos[1] - lower[1], z = pos[2] - lower[2];
+  const float xy = x * y, yz = y * z, xz = z * x, xyz = xy * z;
+  float wt[2][2][2];
+  wt[1][1][1] = xyz;
+  wt[1][1][0] = xy - xyz;
+  wt[1][0][1] = xz - xyz;
+  wt[1][0][0] = (x - xz) - wt[1][1][0];
+  wt[0][1][1] = yz - xyz;
+  wt[0][1][0] = (y - yz) - wt[1][1][0];
+  wt[0][0][1] = (z - yz) - wt[1][0][1];
+  wt[0][0][0] = ((1.0f - x) - y + xy) - wt[0][0][1];
+  const int ha = h[Axes<D>::a], hb = h[Axes<D>::b];
+  cls |= 1u << (ha * 2 + hb);
+#pragma unroll
+  for (int sd = 0; sd < 2; ++sd)
+#pragma unroll
+    for (int sa = 0; sa < 3; ++sa)
+#pragma unroll
+      for (int sb = 0; sb < 3; ++sb) {
+        const float w00 = corner<D>(wt, sd, sa, sb), w01 = corner<D>(wt, sd, sa, sb - 1),
+                    w10 = corner<D>(wt, sd, sa - 1, sb), w11 = corner<D>(wt, sd, sa - 1, sb - 1);
+        const float w = ha ? (hb ? w11 : w10) : (hb ? w01 : w00);
+        const int s = sd * 9 + sa * 3 + sb;
+        const float m = w * vel;
+        avw[s] = avw[s] + m;
+        aw[s] = aw[s] + w;
+      }
+}
+
+// Bitmask of particle classes (ha, hb) whose stencil reaches slot (sa, sb).
+__host__ __device__ constexpr uint32_t slot_classes(int sa, int sb) {
+  uint32_t m = 0;
+  for (int ha = 0; ha < 2; ++ha)
+    for (int hb = 0; hb < 2; ++hb)
+      if (sa - ha >= 0 && sa - ha <= 1 && sb - hb >= 0 && sb - hb <= 1) m |= 1u << (ha * 2 + hb);
+  return m;
+}
+
+template <int D>
+__device__ inline void p2g_component(const Dom &d, const Particles &P, const int rc[2][3],
+                                     const uint32_t beg[2], const uint32_t end[2], bool has[2],
+                                     float *s_vw, float *s_w, uint8_t *s_touch) {
+  float avw[2][18], aw[2][18];
+  uint32_t cls[2] = {0u, 0u};
+#pragma unroll
+  for (int s = 0; s < 2; ++s)
+#pragma unroll
+    for (int q = 0; q < 18; ++q) { avw[s][q] = 0.0f; aw[s][q] = 0.0f; }
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    if (!has[s]) continue;
+    const int c[3] = {rc[s][0], rc[s][1], rc[s][2]};
+    for (uint32_t q = beg[s]; q < end[s]; ++q) {
+      const float pos[3] = {P.a[0][q], P.a[1][q], P.a[2][q]};
+      p2g_particle<D>(pos, c, P.a[3 + D][q], avw[s], aw[s], cls[s]);
+    }
+  }
+  // rc holds region coordinates in the ring-extended leaf: own samples are
+  // region coordinates 1..8; region coordinate r <-> local voxel r - 1.
+  int rr[2][3];
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const int idx = int(threadIdx.x) + s * 512;
+    rr[s][0] = idx / 100; rr[s][1] = (idx / 10) % 10; rr[s][2] = idx % 10;
+  }
+#pragma unroll
+  for (int sd = 0; sd < 2; ++sd)
+#pragma unroll
+    for (int sa = 0; sa < 3; ++sa)
+#pragma unroll
+      for (int sb = 0; sb < 3; ++sb) {
+        const int slot = sd * 9 + sa * 3 + sb;
+        constexpr int A = Axes<D>::a, B = Axes<D>::b;
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+          if (!has[s]) continue;
+          int t[3] = {rr[s][0], rr[s][1], rr[s][2]};
+          t[D] += sd; t[A] += sa - 1; t[B] += sb - 1;
+          if (t[0] < 1 || t[1] < 1 || t[2] < 1 || t[0] > 8 || t[1] > 8 || t[2] > 8) continue;
+          const int own = ((t[0] - 1) << 6) | ((t[1] - 1) << 3) | (t[2] - 1);
+          s_vw[own] += avw[s][slot];
+          s_w[own] += aw[s][slot];
+          if (cls[s] & slot_classes(sa, sb)) s_touch[own] = 1;
+        }
+        __syncthreads();
+      }
+}
+
+__global__ __launch_bounds__(512) void k_p2g(Dom d, const int32_t *list, int64_t nlist,
+                                             const uint32_t *vstart, Particles P, float *vx,
+                                             float *vy, float *vz, uint8_t *vflag,
+                                             uint8_t *marker) {
+  __shared__ float s_vw[3][512];
+  __shared__ float s_w[3][512];
+  __shared__ uint8_t s_touch[512];
+  const int64_t w = xcd_work(nlist);
+  if (w >= nlist) return;
+  const int64_t leaf = list[w];
+  int o[3];
+  leaf_origin(d, leaf, o);
+  const int t = threadIdx.x;
+  for (int a = 0; a < 3; ++a) { s_vw[a][t] = 0.0f; s_w[a][t] = 0.0f; }
+  s_touch[t] = 0;
+  int rc[2][3];
+  uint32_t beg[2] = {0u, 0u}, end[2] = {0u, 0u};
+  bool has[2] = {false, false};
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const int idx = t + s * 512;
+    if (idx >= 1000) continue;
+    rc[s][0] = o[0] - 1 + idx / 100;
+    rc[s][1] = o[1] - 1 + (idx / 10) % 10;
+    rc[s][2] = o[2] - 1 + idx % 10;
+    if (rc[s][0] < 0 || rc[s][1] < 0 || rc[s][2] < 0 || rc[s][0] > d.n[0] ||
+        rc[s][1] > d.n[1] || rc[s][2] > d.n[2])
+      continue;
+    const int64_t v = vox_of(d, rc[s][0], rc[s][1], rc[s][2]);
+    beg[s] = vstart[v];
+    end[s] = vstart[v + 1];
+    has[s] = end[s] > beg[s];
+  }
+  __syncthreads();
+  p2g_component<0>(d, P, rc, beg, end, has, s_vw[0], s_w[0], s_touch);
+  p2g_component<1>(d, P, rc, beg, end, has, s_vw[1], s_w[1], s_touch);
+  p2g_component<2>(d, P, rc, beg, end, has, s_vw[2], s_w[2], s_touch);
+  // Normalise (flip_simulation.cc:900-922) and mark FLUID
+  // (flip_simulation.cc:852-857).
+  const int i = o[0] + (t >> 6), j = o[1] + ((t >> 3) & 7), k = o[2] + (t & 7);
+  const int64_t v = (leaf << 9) | t;
+  float out[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float ws = s_w[a][t];
+    out[a] = double(ws) < 1e-5 ? 0.0f : s_vw[a][t] / ws;
+  }
+  vx[v] = out[0]; vy[v] = out[1]; vz[v] = out[2];
+  vflag[v] = s_touch[t] ? 1 : 0;
+  uint8_t m = kAir;
+  if (in_bbox(d, i, j, k) && vstart[v + 1] > vstart[v]) m = kFluid;
+  marker[v] = m;
+}
+
+// ------------------------------------------------------------------ G2P
+// UpdateParticlesFromGrid (flip_simulation.cc:1024-1189): per leaf with
+// particles, the 10^3 x 3 samples of v and dv are staged in LDS, then each
+// particle is interpolated with the reference's lerp order (k, j, i).
+template <int D>
+__device__ inline float interp(const float pos[3], const int c[3], const int lc[3],
+                               const float *buf) {
+#pragma clang fp contract(off)
+  float lower[3] = {float(c[0]), float(c[1]), float(c[2])};
+  int off[3] = {0, 0, 0};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (a == D) continue;
+    lower[a] = lower[a] + 0.5f;
+    if (pos[a] < lower[a]) { lower[a] = lower[a] - 1.0f; off[a] = -1; }
+  }
+  const int i = lc[0] + off[0], j = lc[1] + off[1], k = lc[2] + off[2];
+  const float w0 = pos[0] - lower[0], w1 = pos[1] - lower[1], w2 = pos[2] - lower[2];
+  auto at = [&](int a, int b, int cc) { return buf[(a * 10 + b) * 10 + cc]; };
+  const float v000 = at(i, j, k), v001 = at(i, j, k + 1), v010 = at(i, j + 1, k),
+              v011 = at(i, j + 1, k + 1), v100 = at(i + 1, j, k), v101 = at(i + 1, j, k + 1),
+              v110 = at(i + 1, j + 1, k), v111 = at(i + 1, j + 1, k + 1);
+  float tt;
+  tt = w2 * (v001 - v000); const float v00 = v000 + tt;
+  tt = w2 * (v011 - v010); const float v01 = v010 + tt;
+  tt = w2 * (v101 - v100); const float v10 = v100 + tt;
+  tt = w2 * (v111 - v110); const float v11 = v110 + tt;
+  tt = w1 * (v01 - v00); const float v0 = v00 + tt;
+  tt = w1 * (v11 - v10); const float v1 = v10 + tt;
+  tt = w0 * (v1 - v0);
+  return v0 + tt;
+}
+
+__global__ __launch_bounds__(256) void k_g2p(Dom d, const int32_t *list, int64_t nlist,
+                                             const uint32_t *lhas, const uint32_t *vstart,
+                                             Particles P, const float *vx, const float *vy,
+                                             const float *vz, const float *ux, const float *uy,
+                                             const float *uz, float flip) {
+#pragma clang fp contract(off)
+  __shared__ float s_v[3][1000];
+  __shared__ float s_u[3][1000];
+  const int64_t w = xcd_work(nlist);
+  if (w >= nlist) return;
+  const int64_t leaf = list[w];
+  if (!lhas[leaf]) return;
+  int o[3];
+  leaf_origin(d, leaf, o);
+  for (int idx = threadIdx.x; idx < 1000; idx += blockDim.x) {
+    const int i = o[0] - 1 + idx / 100, j = o[1] - 1 + (idx / 10) % 10, k = o[2] - 1 + idx % 10;
+    float a[6] = {0, 0, 0, 0, 0, 0};
+    if (in_store(d, i, j, k)) {
+      const int64_t v = vox_of(d, i, j, k);
+      a[0] = vx[v]; a[1] = vy[v]; a[2] = vz[v]; a[3] = ux[v]; a[4] = uy[v]; a[5] = uz[v];
+    }
+    s_v[0][idx] = a[0]; s_v[1][idx] = a[1]; s_v[2][idx] = a[2];
+    s_u[0][idx] = a[3]; s_u[1][idx] = a[4]; s_u[2][idx] = a[5];
+  }
+  __syncthreads();
+  const uint32_t b = vstart[leaf << 9], e = vstart[(leaf + 1) << 9];
+  for (uint32_t q = b + threadIdx.x; q < e; q += blockDim.x) {
+    const float pos[3] = {P.a[0][q], P.a[1][q], P.a[2][q]};
+    const int c[3] = {int(floorf(pos[0])), int(floorf(pos[1])), int(floorf(pos[2]))};
+    if (!in_bbox(d, c[0], c[1], c[2])) continue;  // removed by DeleteOutsideParticles
+    const int lc[3] = {c[0] - o[0] + 1, c[1] - o[1] + 1, c[2] - o[2] + 1};
+    const float gv[3] = {interp<0>(pos, c, lc, s_v[0]), interp<1>(pos, c, lc, s_v[1]),
+                         interp<2>(pos, c, lc, s_v[2])};
+    const float dv[3] = {interp<0>(pos, c, lc, s_u[0]), interp<1>(pos, c, lc, s_u[1]),
+                         interp<2>(pos, c, lc, s_u[2])};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const float s1 = dv[a] + P.a[3 + a][q];
+      const float s2 = s1 - gv[a];
+      const float s3 = flip * s2;
+      P.a[3 + a][q] = gv[a] + s3;
+      P.a[6 + a][q] = gv[a];
+    }
+  }
+}
+
+__global__ void k_cell_counts_box(Dom d, const uint32_t *vstart, int32_t *out) {
+  const int bx = d.n[0] + 2, by = d.n[1] + 2, bz = d.n[2] + 2;
+  const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q >= int64_t(bx) * by * bz) return;
+  const int i = int(q / (int64_t(by) * bz)) - 1, j = int((q / bz) % by) - 1, k = int(q % bz) - 1;
+  const int64_t v = vox_of(d, i, j, k);
+  out[q] = int32_t(vstart[v + 1] - vstart[v]);
+}
+
+inline unsigned blocks_for(int64_t n, int t) { return unsigned((n + t - 1) / t); }
+
+}  // namespace
+
+hipError_t launch_seed(fsl_ctx *c, const uint8_t *mask_dev, const uint32_t *offs, int ppc,
+                       uint32_t seed, const float vel[3]) {
+  const int64_t ncell = int64_t(c->dom.n[0]) * c->dom.n[1] * c->dom.n[2];
+  k_seed<<<blocks_for(ncell, kThreads), kThreads, 0, c->stream>>>(
+      c->dom, mask_dev, offs, ppc, seed, vel[0], vel[1], vel[2], c->P);
+  return hipGetLastError();
+}
+
+hipError_t launch_mask_count(fsl_ctx *c, const uint8_t *mask_dev, int64_t n, uint32_t *cnt) {
+  k_mask_to_count<<<blocks_for(n, kThreads), kThreads, 0, c->stream>>>(mask_dev, n, cnt);
+  return hipGetLastError();
+}
+
+hipError_t launch_advect(fsl_ctx *c, float dt, bool drop_outside) {
+  if (c->np == 0) return hipSuccess;
+  k_advect<<<blocks_for(c->np, kThreads), kThreads, 0, c->stream>>>(
+      c->dom, c->np, c->P, dt, c->solid, c->key, c->vcount, drop_outside ? 1 : 0);
+  return hipGetLastError();
+}
+
+// After the advection kernel and the scan of vcount into vstart: scatter,
+// restore in-cell order, gather into the other buffer and swap.
+hipError_t launch_sort_tail(fsl_ctx *c, uint32_t n_alive) {
+  if (c->np > 0) {
+    k_scatter<<<blocks_for(c->np, kThreads), kThreads, 0, c->stream>>>(c->np, c->key, c->vstart,
+                                                                        c->vcount, c->perm);
+  }
+  if (n_alive > 0) {
+    k_cell_sort<<<blocks_for(n_alive, kThreads), kThreads, 0, c->stream>>>(n_alive, c->key,
+                                                                            c->vstart, c->perm);
+    k_gather<<<blocks_for(n_alive, kThreads), kThreads, 0, c->stream>>>(n_alive, c->perm, c->P,
+                                                                         c->Q);
+  }
+  Particles t = c->P;
+  c->P = c->Q;
+  c->Q = t;
+  c->np = n_alive;
+  return hipGetLastError();
+}
+
+// Active leaves: holds particles or neighbours a leaf that does.
+hipError_t launch_leaf_flags(fsl_ctx *c) {
+  const int64_t nl = c->nleaves;
+  k_leaf_has<<<blocks_for(nl, kThreads), kThreads, 0, c->stream>>>(nl, c->vstart, c->lhas);
+  k_leaf_dilate<<<blocks_for(nl, kThreads), kThreads, 0, c->stream>>>(c->dom, c->lhas, c->lact);
+  hipError_t e = exclusive_scan(c->lact, c->lscan, nl, c->scan_tmp, c->stream);
+  if (e != hipSuccess) return e;
+  k_compact<<<blocks_for(nl, kThreads), kThreads, 0, c->stream>>>(nl, c->lact, c->lscan,
+                                                                   c->act_list);
+  return hipGetLastError();
+}
+
+hipError_t launch_compact(fsl_ctx *c, const uint32_t *flag, const uint32_t *pos, int32_t *list) {
+  k_compact<<<blocks_for(c->nleaves, kThreads), kThreads, 0, c->stream>>>(c->nleaves, flag, pos,
+                                                                           list);
+  return hipGetLastError();
+}
+
+hipError_t launch_p2g(fsl_ctx *c) {
+  if (c->n_act == 0) return hipSuccess;
+  k_p2g<<<xcd_grid(c->n_act), 512, 0, c->stream>>>(c->dom, c->act_list, c->n_act, c->vstart, c->P,
+                                                     c->v[0], c->v[1], c->v[2], c->vflag,
+                                                     c->marker);
+  return hipGetLastError();
+}
+
+hipError_t launch_g2p(fsl_ctx *c) {
+  if (c->n_act == 0) return hipSuccess;
+  k_g2p<<<xcd_grid(c->n_act), 256, 0, c->stream>>>(c->dom, c->act_list, c->n_act, c->lhas,
+                                                     c->vstart, c->P, c->v[0], c->v[1], c->v[2],
+                                                     c->u[0], c->u[1], c->u[2], c->flip);
+  return hipGetLastError();
+}
+
+hipError_t launch_cell_counts_box(fsl_ctx *c, int32_t *out_dev) {
+  const int64_t nb = int64_t(c->dom.n[0] + 2) * (c->dom.n[1] + 2) * (c->dom.n[2] + 2);
+  k_cell_counts_box<<<blocks_for(nb, kThreads), kThreads, 0, c->stream>>>(c->dom, c->vstart,
+                                                                           out_dev);
+  return hipGetLastError();
+}
+
+}  // namespace fsl

@@ -1,0 +1,337 @@
+"""Python binding of libfsl (the MI355X FLIP projection core) over its C ABI.
+
+The binding is a thin ctypes layer: every call goes to the HIP library
+built in tree (fluid-sim-lb_amd/libfsl.so); there is no CPU fallback. If the
+library is missing, loading raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import scenes  # noqa: F401
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(HERE), "libfsl.so")
+
+OK = 0
+ERR = {
+    -1: "FSL_ERR_ARG", -2: "FSL_ERR_HIP", -3: "FSL_ERR_STATE", -4: "FSL_ERR_CAPACITY",
+    -10: "FSL_ERR_ZERO_DIAGONAL", -11: "FSL_ERR_NAN", -12: "FSL_ERR_ALPHA",
+    -13: "FSL_ERR_BETA", -14: "FSL_ERR_VELOCITY_UPDATE",
+}
+PCG_FP32, PCG_FP64, PCG_MIXED = 0, 1, 2
+STOP_ABS_RR, STOP_REL = 0, 1
+GRID_VELOCITY, GRID_VELOCITY_UPDATE = 0, 1
+GRID_DIVERGENCE, GRID_PRESSURE = 0, 1
+GRID_MARKER, GRID_SOLID, GRID_ROW_INDEX, GRID_CELL_COUNT = 0, 1, 2, 3
+
+
+class SolveParams(C.Structure):
+    _fields_ = [("precision", C.c_int), ("stop_mode", C.c_int), ("threshold", C.c_double),
+                ("max_iterations", C.c_int), ("refresh_every", C.c_int), ("batch", C.c_int)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("n_rows", C.c_int), ("n_cols", C.c_int),
+                ("n_fluid_leaves", C.c_int), ("rr", C.c_double),
+                ("true_rel_residual", C.c_double), ("bnorm", C.c_double),
+                ("seconds", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+# Exported symbols and their argument types (the declarations of include/fsl.h).
+_P = C.c_void_p
+_I = C.c_int
+_F = C.c_float
+_D = C.c_double
+_I64 = C.c_int64
+_FP = C.POINTER(C.c_float)
+_DP = C.POINTER(C.c_double)
+_IP = C.POINTER(C.c_int)
+SIGNATURES = {
+    "fsl_create": [C.POINTER(_P), _IP, _IP, _I, _I64],
+    "fsl_destroy": [_P],
+    "fsl_last_error": [_P],
+    "fsl_set_params": [_P, _F, _FP, _F],
+    "fsl_set_box_boundary": [_P, _I],
+    "fsl_add_solid_box": [_P, _IP, _IP],
+    "fsl_set_particles": [_P, _I64, _P, _P, _P],
+    "fsl_seed_particles": [_P, _P, _I, C.c_uint32, _FP],
+    "fsl_num_particles": [_P, C.POINTER(_I64)],
+    "fsl_get_particles": [_P, _P, _P, _P],
+    "fsl_cfl": [_P, _FP],
+    "fsl_move_particles": [_P, _F],
+    "fsl_clear": [_P],
+    "fsl_p2g": [_P],
+    "fsl_apply_bc": [_P],
+    "fsl_save_velocities": [_P],
+    "fsl_delete_outside": [_P],
+    "fsl_add_gravity": [_P, _F],
+    "fsl_divergence": [_P],
+    "fsl_pcg_solve": [_P, C.POINTER(SolveParams), C.POINTER(SolveStats)],
+    "fsl_pressure_update": [_P],
+    "fsl_velocity_update": [_P],
+    "fsl_extrapolate": [_P],
+    "fsl_g2p": [_P],
+    "fsl_step": [_P, _F, C.POINTER(SolveParams), C.POINTER(SolveStats)],
+    "fsl_compute_dt": [_P, _I, _F, _FP, _FP],
+    "fsl_get_vec": [_P, _I, _P, _P],
+    "fsl_set_vec": [_P, _I, _P, _P],
+    "fsl_get_scalar": [_P, _I, _P],
+    "fsl_set_scalar": [_P, _I, _P],
+    "fsl_get_int": [_P, _I, _P],
+    "fsl_set_marker": [_P, _P],
+    "fsl_get_active_leaves": [_P, _I, _P, _I64, C.POINTER(_I64)],
+    "fsl_tile_rows": [_P, C.POINTER(_I64)],
+    "fsl_pcg_prepare": [_P, C.POINTER(SolveParams), _DP, _DP],
+    "fsl_pcg_iterate": [_P, _DP, _DP, _DP, _DP, _DP],
+    "fsl_pcg_finish": [_P, C.POINTER(SolveStats)],
+    "fsl_project_forward": [_P, C.POINTER(SolveParams), C.POINTER(SolveStats)],
+    "fsl_project_backward": [_P, C.POINTER(SolveParams), _P, _P, C.POINTER(SolveStats)],
+}
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load libfsl.so (raises OSError when it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise OSError(f"libfsl.so not built: {path} (run fluid-sim-lb_amd/build.py)")
+    lib = C.CDLL(path)
+    for name, args in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = C.c_char_p if name == "fsl_last_error" else C.c_int
+    _lib = lib
+    return lib
+
+
+class FslError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERR.get(code, code)}: {msg}")
+        self.code = code
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def solve_params(precision=PCG_FP32, stop_mode=STOP_ABS_RR, threshold=None, dims=None,
+                 max_iterations=1500, refresh_every=32, batch=16) -> SolveParams:
+    """Solver parameters; by default the reference's rule r.r < N^3 * 0.5e-8
+    (flip_app.cc:326-328, flip_app.h:52-53)."""
+    if threshold is None:
+        vol = int(np.prod(dims)) if dims is not None else 64 ** 3
+        threshold = float(np.float32(np.float32(vol) * np.float32(0.5e-8)))
+    return SolveParams(precision, stop_mode, threshold, max_iterations, refresh_every, batch)
+
+
+class Context:
+    """One simulation domain on one GPU (an fsl_ctx)."""
+
+    def __init__(self, dims, tiles=(1, 1, 1), device=0, capacity=0):
+        self.lib = load()
+        self.dims = tuple(int(d) for d in dims)
+        self.box = tuple(d + 2 for d in self.dims)
+        h = C.c_void_p()
+        dv = (C.c_int * 3)(*self.dims)
+        tv = (C.c_int * 3)(*tiles)
+        rc = self.lib.fsl_create(C.byref(h), dv, tv, int(device), int(capacity))
+        if rc != OK:
+            raise FslError(rc, "fsl_create failed")
+        self.h = h
+        self.tiles = tuple(tiles)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.fsl_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != OK:
+            msg = self.lib.fsl_last_error(self.h)
+            raise FslError(rc, msg.decode() if msg else "")
+        return rc
+
+    def call(self, name, *args):
+        return self._check(getattr(self.lib, name)(self.h, *args))
+
+    # -- set-up
+    def set_params(self, gravity_scaled, force, flip):
+        f = (C.c_float * 3)(*force)
+        self.call("fsl_set_params", float(gravity_scaled), f, float(flip))
+
+    def set_box_boundary(self, tank=False):
+        self.call("fsl_set_box_boundary", int(tank))
+
+    def add_solid_box(self, lo, hi):
+        self.call("fsl_add_solid_box", (C.c_int * 3)(*lo), (C.c_int * 3)(*hi))
+
+    def set_particles(self, pos, vel, gv=None):
+        pos = np.ascontiguousarray(pos, dtype=np.float32)
+        vel = np.ascontiguousarray(vel, dtype=np.float32)
+        gv = None if gv is None else np.ascontiguousarray(gv, dtype=np.float32)
+        self.call("fsl_set_particles", int(pos.shape[0]), _ptr(pos), _ptr(vel), _ptr(gv))
+
+    def seed_particles(self, mask, ppc, seed, velocity=(0.0, 0.0, 0.0)):
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        assert m.shape == self.dims
+        self.call("fsl_seed_particles", _ptr(m), int(ppc), int(seed), (C.c_float * 3)(*velocity))
+
+    def num_particles(self) -> int:
+        n = C.c_int64()
+        self.call("fsl_num_particles", C.byref(n))
+        return n.value
+
+    def get_particles(self):
+        n = self.num_particles()
+        pos = np.empty((n, 3), np.float32)
+        vel = np.empty((n, 3), np.float32)
+        gv = np.empty((n, 3), np.float32)
+        self.call("fsl_get_particles", _ptr(pos), _ptr(vel), _ptr(gv))
+        return pos, vel, gv
+
+    # -- stages
+    def cfl(self) -> float:
+        x = C.c_float()
+        self.call("fsl_cfl", C.byref(x))
+        return x.value
+
+    def compute_dt(self, cfl, frame_step, step):
+        s = C.c_float(step)
+        dt = C.c_float()
+        self.call("fsl_compute_dt", int(cfl), float(frame_step), C.byref(s), C.byref(dt))
+        return dt.value, s.value
+
+    def move_particles(self, dt):
+        self.call("fsl_move_particles", float(dt))
+
+    def clear(self):
+        self.call("fsl_clear")
+
+    def p2g(self):
+        self.call("fsl_p2g")
+
+    def apply_bc(self):
+        self.call("fsl_apply_bc")
+
+    def save_velocities(self):
+        self.call("fsl_save_velocities")
+
+    def delete_outside(self):
+        self.call("fsl_delete_outside")
+
+    def add_gravity(self, dt):
+        self.call("fsl_add_gravity", float(dt))
+
+    def divergence(self):
+        self.call("fsl_divergence")
+
+    def solve(self, params: SolveParams) -> dict:
+        st = SolveStats()
+        self.call("fsl_pcg_solve", C.byref(params), C.byref(st))
+        return st.as_dict()
+
+    def pressure_update(self):
+        self.call("fsl_pressure_update")
+
+    def velocity_update(self):
+        self.call("fsl_velocity_update")
+
+    def extrapolate(self):
+        self.call("fsl_extrapolate")
+
+    def g2p(self):
+        self.call("fsl_g2p")
+
+    def step(self, dt, params: SolveParams) -> dict:
+        st = SolveStats()
+        self.call("fsl_step", float(dt), C.byref(params), C.byref(st))
+        return st.as_dict()
+
+    def project_forward(self, params: SolveParams) -> dict:
+        st = SolveStats()
+        self.call("fsl_project_forward", C.byref(params), C.byref(st))
+        return st.as_dict()
+
+    def project_backward(self, params: SolveParams, grad_out: np.ndarray):
+        g = np.ascontiguousarray(grad_out, dtype=np.float32)
+        assert g.shape == self.box + (3,)
+        out = np.empty_like(g)
+        st = SolveStats()
+        self.call("fsl_project_backward", C.byref(params), _ptr(g), _ptr(out), C.byref(st))
+        return out, st.as_dict()
+
+    # -- PCG stage API
+    def pcg_prepare(self, params: SolveParams):
+        rr, rz = C.c_double(), C.c_double()
+        self.call("fsl_pcg_prepare", C.byref(params), C.byref(rr), C.byref(rz))
+        return rr.value, rz.value
+
+    def pcg_iterate(self):
+        v = [C.c_double() for _ in range(5)]
+        self.call("fsl_pcg_iterate", *[C.byref(x) for x in v])
+        return dict(zip(("pAp", "alpha", "rr", "rz", "beta"), (x.value for x in v)))
+
+    def pcg_finish(self) -> dict:
+        st = SolveStats()
+        self.call("fsl_pcg_finish", C.byref(st))
+        return st.as_dict()
+
+    # -- grids (box format over [-1, n])
+    def get_vec(self, which=GRID_VELOCITY):
+        out = np.empty(self.box + (3,), np.float32)
+        act = np.empty(self.box, np.uint8)
+        self.call("fsl_get_vec", int(which), _ptr(out), _ptr(act))
+        return out, act
+
+    def set_vec(self, which, values, active):
+        v = np.ascontiguousarray(values, dtype=np.float32)
+        a = np.ascontiguousarray(active, dtype=np.uint8)
+        assert v.shape == self.box + (3,) and a.shape == self.box
+        self.call("fsl_set_vec", int(which), _ptr(v), _ptr(a))
+
+    def get_scalar(self, which):
+        out = np.empty(self.box, np.float32)
+        self.call("fsl_get_scalar", int(which), _ptr(out))
+        return out
+
+    def set_scalar(self, which, values):
+        v = np.ascontiguousarray(values, dtype=np.float32)
+        self.call("fsl_set_scalar", int(which), _ptr(v))
+
+    def get_int(self, which):
+        out = np.empty(self.box, np.int32)
+        self.call("fsl_get_int", int(which), _ptr(out))
+        return out
+
+    def set_marker(self, marker):
+        m = np.ascontiguousarray(marker, dtype=np.int32)
+        self.call("fsl_set_marker", _ptr(m))
+
+    def active_leaves(self, fluid=False):
+        n = C.c_int64()
+        self.call("fsl_get_active_leaves", int(fluid), None, 0, C.byref(n))
+        out = np.empty((n.value, 3), np.int32)
+        self.call("fsl_get_active_leaves", int(fluid), _ptr(out), n.value, C.byref(n))
+        return out
+
+    def tile_rows(self):
+        nt = int(np.prod(self.tiles))
+        out = np.zeros(max(nt, 1), np.int64)
+        self.call("fsl_tile_rows", out.ctypes.data_as(C.POINTER(C.c_int64)))
+        return out[:nt]

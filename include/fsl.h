@@ -1,0 +1,191 @@
+/*
+ * fsl.h - C ABI of the MI355X FLIP projection core.
+ *
+ * One context holds one simulation domain on one GPU: sparse grids stored as
+ * 8^3 leaf blocks with a device list of active leaves, particles in SoA
+ * arrays sorted by cell, and the Jacobi-PCG pressure solver. The reference's
+ * Canary partitions become tiles of the one domain (per-tile statistics are
+ * reported; the physics is the one-partition physics).
+ *
+ * Conventions
+ *  - Every call returns FSL_OK (0) or a negative FSL_ERR_* code and never
+ *    aborts; fsl_last_error() returns a message for the last failure.
+ *  - Host arrays are copied, never aliased. Calls on one context are not
+ *    thread-safe; contexts are independent. One HIP stream per context.
+ *  - "Box" arrays are dense over the index box [-1, n] on every axis, x-major:
+ *    index(i,j,k) = ((i+1)*(ny+2) + (j+1))*(nz+2) + (k+1). Vector grids store
+ *    3 floats per voxel; component d of voxel (i,j,k) is the velocity on the
+ *    lower d-face of cell (i,j,k) (vector_grid.cc:20-22).
+ *  - Particle arrays are float[n][3].
+ *
+ * Each entry point names the reference symbol it replaces (file:line in the
+ * reference repository schinmayee/fluid-sim-lb).
+ */
+#ifndef FSL_H_
+#define FSL_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FSL_OK 0
+#define FSL_ERR_ARG (-1)            /* bad argument or size */
+#define FSL_ERR_HIP (-2)            /* HIP runtime failure */
+#define FSL_ERR_STATE (-3)          /* call out of order */
+#define FSL_ERR_CAPACITY (-4)       /* particle capacity exceeded */
+#define FSL_ERR_ZERO_DIAGONAL (-10) /* FLUID cell whose 6 neighbours are SOLID */
+#define FSL_ERR_NAN (-11)           /* NaN residual (solver_driver.cc:327) */
+#define FSL_ERR_ALPHA (-12)         /* alpha > 1e5 (solver.cc:333) */
+#define FSL_ERR_BETA (-13)          /* beta > 1e5 (solver.cc:361) */
+#define FSL_ERR_VELOCITY_UPDATE (-14) /* face active in v, not saved (flip_simulation.cc:788-792) */
+
+/* PCG arithmetic. */
+#define FSL_PCG_FP32 0  /* reference precision: FP32 vectors and dots, ordered stencil, no FMA */
+#define FSL_PCG_FP64 1  /* FP64 vectors, stencil and dots */
+#define FSL_PCG_MIXED 2 /* FP32 stencil/axpys, FP64 dots, FP64 true-residual refresh */
+
+/* PCG stopping rule. */
+#define FSL_STOP_ABS_RR 0 /* stop when r.r < threshold (solver_driver.cc:330) */
+#define FSL_STOP_REL 1    /* stop when ||r|| < threshold * ||b|| */
+
+/* Grid selectors. */
+#define FSL_GRID_VELOCITY 0        /* grid_velocity_ */
+#define FSL_GRID_VELOCITY_UPDATE 1 /* grid_velocity_update_ (saved v, then dv) */
+#define FSL_GRID_DIVERGENCE 0      /* grid_divergence_ */
+#define FSL_GRID_PRESSURE 1        /* grid_pressure_ */
+#define FSL_GRID_MARKER 0          /* grid_marker_: 0 AIR, 1 FLUID, 2 SOLID */
+#define FSL_GRID_SOLID 1           /* grid_solid_: 0/1 */
+#define FSL_GRID_ROW_INDEX 2       /* solver row index, -1 where not a row */
+#define FSL_GRID_CELL_COUNT 3      /* particles per cell */
+
+typedef struct fsl_ctx fsl_ctx;
+
+typedef struct fsl_solve_params {
+  int precision;      /* FSL_PCG_* */
+  int stop_mode;      /* FSL_STOP_* */
+  double threshold;   /* absolute r.r bound, or relative ||r||/||b|| bound */
+  int max_iterations; /* reference default 1500 (flip_app.h:52) */
+  int refresh_every;  /* MIXED: FP64 residual refresh period (0 = 32) */
+  int batch;          /* iterations launched between host checks (0 = 16) */
+} fsl_solve_params;
+
+typedef struct fsl_solve_stats {
+  int iterations;
+  int n_rows;          /* FLUID cells in the domain (solver.cc:80) */
+  int n_cols;          /* FLUID cells in the domain + 1 ring (solver.cc:81) */
+  int n_fluid_leaves;  /* 8^3 leaves holding at least one FLUID cell */
+  double rr;           /* final recursive r.r */
+  double true_rel_residual; /* ||b - A x|| / ||b||, FP64, after the solve */
+  double bnorm;        /* ||b|| */
+  double seconds;      /* wall time of the iteration loop on the stream */
+} fsl_solve_stats;
+
+/* ---- context ---- */
+/* FlipSimulation::InitializeMetadata + InitializeVariables
+ * (flip_simulation.cc:83-122); dims must be multiples of 8; tiles are the
+ * reference partitions (metagrid.cc:25-35). */
+int fsl_create(fsl_ctx **out, const int dims[3], const int tiles[3], int device,
+               int64_t particle_capacity);
+int fsl_destroy(fsl_ctx *ctx);
+const char *fsl_last_error(const fsl_ctx *ctx);
+/* Scaled gravity (flip_app_sim.cc:74), force = config force * scaled gravity
+ * (flip_app_sim.cc:87, SetForces flip_simulation.cc:248-253), flip factor. */
+int fsl_set_params(fsl_ctx *ctx, float gravity_scaled, const float force[3], float flip);
+/* SetBoxBoundary / SetTankBoundary (flip_simulation.cc:255-377). */
+int fsl_set_box_boundary(fsl_ctx *ctx, int tank);
+/* AddSolidCuboid (flip_simulation.cc:315-324); inclusive integer box. */
+int fsl_add_solid_box(fsl_ctx *ctx, const int lo[3], const int hi[3]);
+
+/* ---- particles ---- */
+/* Particles::AddParticleToData for a batch (particles.h); replaces all
+ * particles. gv may be NULL (then gv = vel). */
+int fsl_set_particles(fsl_ctx *ctx, int64_t n, const float *pos, const float *vel,
+                      const float *gv);
+/* InitializePhiAndParticles for a given fluid mask (flip_simulation.cc:138-188):
+ * ppc Philox-jittered particles per mask cell, seeded on the device; mask is
+ * uint8 over [0, n) x-major; velocity and grid velocity are `velocity`. */
+int fsl_seed_particles(fsl_ctx *ctx, const uint8_t *mask, int ppc, uint32_t seed,
+                       const float velocity[3]);
+/* Particles::size (particles.h). */
+int fsl_num_particles(fsl_ctx *ctx, int64_t *n);
+/* Particle data in device order (sorted by cell). Any pointer may be NULL. */
+int fsl_get_particles(fsl_ctx *ctx, float *pos, float *vel, float *gv);
+
+/* ---- FlipSimulation stages (flip_simulation.h:133-182) ---- */
+/* CFL (flip_simulation.cc:459-488): returns 1/max(sqrt(g/2), v_inf, 1e-2). */
+int fsl_cfl(fsl_ctx *ctx, float *inv_speed);
+/* MoveParticles + DeleteParticlesInSolids (flip_app_sim.cc:496-517,
+ * particles.cc:478-546, 202-219): advect, clamp, delete, sort by cell. */
+int fsl_move_particles(fsl_ctx *ctx, float dt);
+/* ClearGridData (flip_simulation.cc:379-389). */
+int fsl_clear(fsl_ctx *ctx);
+/* TransferParticlesToGrid (flip_simulation.cc:808-923), incl. FLUID marking. */
+int fsl_p2g(fsl_ctx *ctx);
+/* ApplyBoundaryConditions (flip_simulation.cc:656-713). */
+int fsl_apply_bc(fsl_ctx *ctx);
+/* SaveGridVelocities (flip_simulation.cc:490-496). */
+int fsl_save_velocities(fsl_ctx *ctx);
+/* DeleteOutsideParticles (particles.cc:273-293). */
+int fsl_delete_outside(fsl_ctx *ctx);
+/* AddGravity (flip_simulation.cc:505-515). */
+int fsl_add_gravity(fsl_ctx *ctx, float dt);
+/* ComputeDivergence (flip_simulation.cc:715-736). */
+int fsl_divergence(fsl_ctx *ctx);
+/* PCGDriver::Solve (solver_driver.cc:13-69): assemble, prepare, iterate,
+ * copy solution into the pressure grid. */
+int fsl_pcg_solve(fsl_ctx *ctx, const fsl_solve_params *params, fsl_solve_stats *stats);
+/* UpdateVelocityFromPressure (flip_simulation.cc:738-778). */
+int fsl_pressure_update(fsl_ctx *ctx);
+/* GetVelocityUpdate (flip_simulation.cc:780-801). */
+int fsl_velocity_update(fsl_ctx *ctx);
+/* VectorGrid::ExtrapolateToBoundary on v and dv (vector_grid.cc:302-381). */
+int fsl_extrapolate(fsl_ctx *ctx);
+/* UpdateParticlesFromGrid (flip_simulation.cc:1024-1189), incl. extrapolation. */
+int fsl_g2p(fsl_ctx *ctx);
+
+/* FlipDriver::RunOneStep (flip_app_sim.cc:211-254) with a given dt. */
+int fsl_step(fsl_ctx *ctx, float dt, const fsl_solve_params *params, fsl_solve_stats *stats);
+/* WhileAndComputeDt's dt (flip_app_sim.cc:371-410): cfl * CFL(), min with
+ * 1/frame_step, floor at frame_step/10, snap to the frame end; advances *step. */
+int fsl_compute_dt(fsl_ctx *ctx, int cfl, float frame_step, float *step, float *dt);
+
+/* ---- grid access in box format ---- */
+int fsl_get_vec(fsl_ctx *ctx, int which, float *out, uint8_t *active);
+int fsl_set_vec(fsl_ctx *ctx, int which, const float *in, const uint8_t *active);
+int fsl_get_scalar(fsl_ctx *ctx, int which, float *out);
+int fsl_set_scalar(fsl_ctx *ctx, int which, const float *in);
+int fsl_get_int(fsl_ctx *ctx, int which, int32_t *out);
+/* Sets the marker grid (labels); SOLID cells must also be solid. */
+int fsl_set_marker(fsl_ctx *ctx, const int32_t *in);
+/* Active 8^3 leaves (origins, int[3] each, in device list order). */
+int fsl_get_active_leaves(fsl_ctx *ctx, int which, int32_t *origins, int64_t cap, int64_t *count);
+/* Per-tile FLUID row counts (n_rows of each reference partition). */
+int fsl_tile_rows(fsl_ctx *ctx, int64_t *rows_per_tile);
+
+/* ---- PCG stage API (solver.cc:262-366, solver_driver.cc:318-343) ---- */
+/* Assembly + PreparePCGSolver: r = b - A x, z = M^-1 r, p = z, rz, rr. */
+int fsl_pcg_prepare(fsl_ctx *ctx, const fsl_solve_params *params, double *rr, double *rz);
+/* One While-loop body: ComputeLocal1 (Ap, pAp), ComputeLocal2 (alpha, x, r,
+ * z, rr, rz), ComputeLocal3 (beta; p = z + beta p is applied lazily by the
+ * next ComputeLocal1). Returns the scalars after the iteration. */
+int fsl_pcg_iterate(fsl_ctx *ctx, double *pAp, double *alpha, double *rr, double *rz,
+                    double *beta);
+/* CopySolution (solver.cc:93-99). */
+int fsl_pcg_finish(fsl_ctx *ctx, fsl_solve_stats *stats);
+
+/* ---- differentiable projection (no reference counterpart) ---- */
+/* Forward: u_post = P(u) with the current labels, i.e. divergence, PCG,
+ * pressure update, on the velocity grid in place. */
+int fsl_project_forward(fsl_ctx *ctx, const fsl_solve_params *params, fsl_solve_stats *stats);
+/* Backward: given g = dL/du_post (box format, 3 floats per voxel), writes
+ * dL/du = P^T g (box format). One adjoint PCG solve on the same operator. */
+int fsl_project_backward(fsl_ctx *ctx, const fsl_solve_params *params, const float *grad_out,
+                         float *grad_in, fsl_solve_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FSL_H_ */

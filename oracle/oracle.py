@@ -1,0 +1,213 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker. The oracle restates the
+reference's algorithm; see flip_oracle.cpp for what each function follows.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        lib = C.CDLL(LIB_PATH)
+        P, I, F, D, I64 = C.c_void_p, C.c_int, C.c_float, C.c_double, C.c_int64
+        sig = {
+            "or_create": ([I, I, I], P), "or_destroy": ([P], None),
+            "or_set_params": ([P, F, F, F, F, F], None), "or_set_box_boundary": ([P, I], None),
+            "or_add_solid_box": ([P, P, P], None),
+            "or_set_particles": ([P, I64, P, P, P], None), "or_num_particles": ([P], I64),
+            "or_get_particles": ([P, P, P, P, P], None), "or_cfl": ([P], F),
+            "or_move_particles": ([P, F], None), "or_clear": ([P], None), "or_p2g": ([P], None),
+            "or_apply_bc": ([P], None), "or_save_velocities": ([P], None),
+            "or_delete_outside": ([P], None), "or_add_gravity": ([P, F], None),
+            "or_divergence": ([P], None),
+            "or_solve": ([P, I, I, D, I, C.POINTER(I), C.POINTER(D)], I),
+            "or_solver_sizes": ([P, C.POINTER(I), C.POINTER(I)], I),
+            "or_residual_history": ([P, P, I], I), "or_solution": ([P, P, I], I),
+            "or_pressure_update": ([P], None), "or_velocity_update": ([P], I),
+            "or_extrapolate": ([P], None), "or_g2p": ([P], None),
+            "or_run_one_step": ([P, F, I, I, D, I, C.POINTER(I)], I),
+            "or_compute_dt": ([P, I, F, C.POINTER(F)], F),
+            "or_get_vec": ([P, I, P, P], None), "or_set_vec": ([P, I, P, P], None),
+            "or_get_scalar": ([P, I, P], None), "or_set_scalar": ([P, I, P], None),
+            "or_get_int": ([P, I, P], None), "or_set_marker": ([P, P], None),
+            "or_cell_counts": ([P, P], None),
+        }
+        for name, (args, res) in sig.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = lib
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Oracle:
+    """The reference's one-partition FLIP step on dense CPU grids."""
+
+    def __init__(self, dims):
+        self.lib = load()
+        self.dims = tuple(int(d) for d in dims)
+        self.box = tuple(d + 2 for d in self.dims)
+        self.h = self.lib.or_create(*self.dims)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.or_destroy(self.h)
+            self.h = None
+
+    def set_params(self, gravity_scaled, force, flip):
+        self.lib.or_set_params(self.h, gravity_scaled, force[0], force[1], force[2], flip)
+
+    def set_box_boundary(self, tank=False):
+        self.lib.or_set_box_boundary(self.h, int(tank))
+
+    def add_solid_box(self, lo, hi):
+        self.lib.or_add_solid_box(self.h, _ptr(np.array(lo, np.int32)), _ptr(np.array(hi, np.int32)))
+
+    def set_particles(self, pos, vel, gv=None):
+        pos = np.ascontiguousarray(pos, np.float32)
+        vel = np.ascontiguousarray(vel, np.float32)
+        gv = vel if gv is None else np.ascontiguousarray(gv, np.float32)
+        self.lib.or_set_particles(self.h, pos.shape[0], _ptr(pos), _ptr(vel), _ptr(gv))
+
+    def num_particles(self):
+        return int(self.lib.or_num_particles(self.h))
+
+    def get_particles(self):
+        n = self.num_particles()
+        pos, vel, gv = (np.empty((n, 3), np.float32) for _ in range(3))
+        cell = np.empty((n, 3), np.int32)
+        self.lib.or_get_particles(self.h, _ptr(pos), _ptr(vel), _ptr(gv), _ptr(cell))
+        return pos, vel, gv, cell
+
+    def cfl(self):
+        return float(self.lib.or_cfl(self.h))
+
+    def compute_dt(self, cfl, frame_step, step):
+        s = C.c_float(step)
+        dt = self.lib.or_compute_dt(self.h, int(cfl), float(frame_step), C.byref(s))
+        return float(dt), s.value
+
+    def move_particles(self, dt):
+        self.lib.or_move_particles(self.h, float(dt))
+
+    def clear(self):
+        self.lib.or_clear(self.h)
+
+    def p2g(self):
+        self.lib.or_p2g(self.h)
+
+    def apply_bc(self):
+        self.lib.or_apply_bc(self.h)
+
+    def save_velocities(self):
+        self.lib.or_save_velocities(self.h)
+
+    def delete_outside(self):
+        self.lib.or_delete_outside(self.h)
+
+    def add_gravity(self, dt):
+        self.lib.or_add_gravity(self.h, float(dt))
+
+    def divergence(self):
+        self.lib.or_divergence(self.h)
+
+    def solve(self, precision=0, stop_mode=0, threshold=None, max_iter=1500):
+        if threshold is None:
+            threshold = float(np.float32(np.float32(np.prod(self.dims)) * np.float32(0.5e-8)))
+        it, rr = C.c_int(), C.c_double()
+        rc = self.lib.or_solve(self.h, precision, stop_mode, threshold, max_iter,
+                               C.byref(it), C.byref(rr))
+        return rc, it.value, rr.value
+
+    def solver_sizes(self):
+        a, b = C.c_int(), C.c_int()
+        self.lib.or_solver_sizes(self.h, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def residual_history(self):
+        out = np.empty(4096, np.float64)
+        m = self.lib.or_residual_history(self.h, _ptr(out), out.size)
+        return out[: min(m, out.size)].copy()
+
+    def solution(self):
+        n, _ = self.solver_sizes()
+        out = np.empty(max(n, 1), np.float64)
+        m = self.lib.or_solution(self.h, _ptr(out), out.size)
+        return out[:m].copy()
+
+    def pressure_update(self):
+        self.lib.or_pressure_update(self.h)
+
+    def velocity_update(self):
+        return int(self.lib.or_velocity_update(self.h))
+
+    def extrapolate(self):
+        self.lib.or_extrapolate(self.h)
+
+    def g2p(self):
+        self.lib.or_g2p(self.h)
+
+    def run_one_step(self, dt, precision=0, stop_mode=0, threshold=None, max_iter=1500):
+        if threshold is None:
+            threshold = float(np.float32(np.float32(np.prod(self.dims)) * np.float32(0.5e-8)))
+        it = C.c_int()
+        rc = self.lib.or_run_one_step(self.h, float(dt), precision, stop_mode, threshold,
+                                      max_iter, C.byref(it))
+        return rc, it.value
+
+    def get_vec(self, which=0):
+        out = np.empty(self.box + (3,), np.float32)
+        act = np.empty(self.box, np.uint8)
+        self.lib.or_get_vec(self.h, which, _ptr(out), _ptr(act))
+        return out, act
+
+    def set_vec(self, which, values, active):
+        v = np.ascontiguousarray(values, np.float32)
+        a = np.ascontiguousarray(active, np.uint8)
+        self.lib.or_set_vec(self.h, which, _ptr(v), _ptr(a))
+
+    def get_scalar(self, which):
+        out = np.empty(self.box, np.float32)
+        self.lib.or_get_scalar(self.h, which, _ptr(out))
+        return out
+
+    def set_scalar(self, which, values):
+        self.lib.or_set_scalar(self.h, which, _ptr(np.ascontiguousarray(values, np.float32)))
+
+    def get_int(self, which):
+        out = np.empty(self.box, np.int32)
+        self.lib.or_get_int(self.h, which, _ptr(out))
+        return out
+
+    def set_marker(self, marker):
+        self.lib.or_set_marker(self.h, _ptr(np.ascontiguousarray(marker, np.int32)))
+
+    def cell_counts(self):
+        out = np.empty(self.box, np.int32)
+        self.lib.or_cell_counts(self.h, _ptr(out))
+        return out

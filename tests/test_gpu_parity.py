@@ -1,0 +1,187 @@
+"""GPU parity: libfsl (HIP, through the C ABI) against the CPU oracle.
+
+Bars (from the projection core's parity contract):
+  * labels, velocity-grid activity, per-cell particle counts, the fluid-leaf
+    index and positions after advection: bit-exact;
+  * velocity, divergence, pressure: relative L2 <= 1e-5;
+  * PCG iterations within +-2 of the oracle under the reference's rule.
+"""
+import numpy as np
+import pytest
+
+import fsl
+from fsl import scenes
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    den = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (den if den > 0 else 1.0))
+
+
+def sort_particles(pos, *rest):
+    bits = pos.view(np.uint32)
+    order = np.lexsort((bits[:, 2], bits[:, 1], bits[:, 0]))
+    return [pos[order]] + [r[order] for r in rest]
+
+
+def make_pair(n, mask, ppc, seed, flip=0.95, g=9.8, tank=False, vel0=None):
+    pos, vel = scenes.seed_particles(mask, ppc, seed)
+    if vel0 is not None:
+        vel[:] = vel0
+    ctx = fsl.Context(n, capacity=pos.shape[0] + 64)
+    orc = Oracle(n)
+    for s in (ctx, orc):
+        s.set_params(g, (0.0, g, 0.0), flip)
+        s.set_box_boundary(tank)
+        s.set_particles(pos, vel)
+    return ctx, orc
+
+
+def check_grids(ctx, orc, stage, vel_tol=TOL):
+    gm, om = ctx.get_int(fsl.GRID_MARKER), orc.get_int(0)
+    assert np.array_equal(gm, om), f"{stage}: marker differs at {np.argwhere(gm != om)[:5]}"
+    gv, ga = ctx.get_vec(fsl.GRID_VELOCITY)
+    ov, oa = orc.get_vec(0)
+    assert np.array_equal(ga, oa), f"{stage}: v activity differs at {np.argwhere(ga != oa)[:5]}"
+    e = rel_l2(gv, ov)
+    assert e <= vel_tol, f"{stage}: v rel L2 {e}"
+
+
+def fluid_leaves_from_marker(marker):
+    m = marker[1:-1, 1:-1, 1:-1] == 1
+    nx, ny, nz = m.shape
+    lv = m.reshape(nx // 8, 8, ny // 8, 8, nz // 8, 8).any(axis=(1, 3, 5))
+    return {tuple(int(c) * 8 for c in ijk) for ijk in np.argwhere(lv)}
+
+
+def run_stage_parity(n, mask, ppc, seed, steps=2, tank=False):
+    ctx, orc = make_pair(n, mask, ppc, seed, tank=tank)
+    prm = fsl.solve_params(dims=n)
+    step = 0.0
+    for s in range(steps):
+        dt, step_o = orc.compute_dt(1, 1.0 / 30, step)
+        gdt, step_g = ctx.compute_dt(1, 1.0 / 30, step)
+        assert gdt == dt and step_g == step_o
+        step = step_o if step_o + 1e-5 < 1.0 / 30 else 0.0
+        ctx.move_particles(dt); orc.move_particles(dt)
+        gp, gvel, ggv = ctx.get_particles()
+        op, ovel, ogv, _ = orc.get_particles()
+        assert gp.shape == op.shape
+        gp, gvel = sort_particles(gp, gvel)
+        op, ovel = sort_particles(op, ovel)
+        assert np.array_equal(gp, op), "positions after advection"
+        assert rel_l2(gvel, ovel) <= TOL
+        ctx.clear(); orc.clear()
+        assert np.array_equal(ctx.get_int(fsl.GRID_CELL_COUNT), orc.cell_counts())
+        ctx.p2g(); orc.p2g()
+        check_grids(ctx, orc, f"step {s} p2g")
+        ctx.apply_bc(); orc.apply_bc()
+        check_grids(ctx, orc, f"step {s} bc")
+        ctx.save_velocities(); orc.save_velocities()
+        gu, gua = ctx.get_vec(fsl.GRID_VELOCITY_UPDATE)
+        ou, oua = orc.get_vec(1)
+        assert np.array_equal(gua, oua) and rel_l2(gu, ou) <= TOL
+        ctx.delete_outside(); orc.delete_outside()
+        assert ctx.num_particles() == orc.num_particles()
+        ctx.add_gravity(dt); orc.add_gravity(dt)
+        check_grids(ctx, orc, f"step {s} gravity")
+        ctx.apply_bc(); orc.apply_bc()
+        check_grids(ctx, orc, f"step {s} bc2")
+        ctx.divergence(); orc.divergence()
+        assert rel_l2(ctx.get_scalar(fsl.GRID_DIVERGENCE), orc.get_scalar(0)) <= TOL
+        gl = {tuple(int(c) for c in o) for o in ctx.active_leaves(fluid=True)}
+        assert gl == fluid_leaves_from_marker(orc.get_int(0)), "fluid leaf index"
+        st = ctx.solve(prm)
+        rc, it, _ = orc.solve()
+        assert rc == 0
+        assert st["n_rows"] == orc.solver_sizes()[0]
+        assert abs(st["iterations"] - it) <= 2, (st["iterations"], it)
+        assert np.array_equal(ctx.get_int(fsl.GRID_ROW_INDEX), orc.get_int(2))
+        assert rel_l2(ctx.get_scalar(fsl.GRID_PRESSURE), orc.get_scalar(1)) <= TOL
+        ctx.pressure_update(); orc.pressure_update()
+        check_grids(ctx, orc, f"step {s} pressure update")
+        ctx.velocity_update(); assert orc.velocity_update() == 0
+        gu, gua = ctx.get_vec(fsl.GRID_VELOCITY_UPDATE)
+        ou, oua = orc.get_vec(1)
+        assert np.array_equal(gua, oua) and rel_l2(gu, ou) <= TOL
+        ctx.g2p(); orc.g2p()
+        gp, gvel, ggv = ctx.get_particles()
+        op, ovel, ogv, _ = orc.get_particles()
+        gp, gvel, ggv = sort_particles(gp, gvel, ggv)
+        op, ovel, ogv = sort_particles(op, ovel, ogv)
+        assert np.array_equal(gp, op)
+        assert rel_l2(gvel, ovel) <= TOL and rel_l2(ggv, ogv) <= TOL
+    ctx.close()
+
+
+def test_stage_parity_sphere_drop_64():
+    n = (64, 64, 64)
+    run_stage_parity(n, scenes.WORKLOADS["sphere_drop_64"][1](n), 8, 1, steps=2)
+
+
+def test_stage_parity_small_dam_tank():
+    # Open tank (no solid top): particles can reach the top layer and are
+    # deleted after P2G (DeleteOutsideParticles).
+    n = (32, 40, 24)
+    run_stage_parity(n, scenes.dam_break_mask(n, 12, 30), 4, 7, steps=2, tank=True)
+
+
+def test_full_step_matches_oracle_sphere_drop():
+    n = (64, 64, 64)
+    ctx, orc = make_pair(n, scenes.WORKLOADS["sphere_drop_64"][1](n), 8, 1)
+    prm = fsl.solve_params(dims=n)
+    step = 0.0
+    for s in range(4):
+        dt, step = orc.compute_dt(1, 1.0 / 30, step)
+        if step + 1e-5 >= 1.0 / 30:
+            step = 0.0
+        st = ctx.step(dt, prm)
+        rc, it = orc.run_one_step(dt)
+        assert rc == 0 and abs(st["iterations"] - it) <= 2
+        assert np.array_equal(ctx.get_int(fsl.GRID_CELL_COUNT), orc.cell_counts())
+        assert np.array_equal(ctx.get_int(fsl.GRID_MARKER), orc.get_int(0))
+        assert rel_l2(ctx.get_vec()[0], orc.get_vec(0)[0]) <= TOL
+        assert rel_l2(ctx.get_scalar(fsl.GRID_PRESSURE), orc.get_scalar(1)) <= TOL
+    ctx.close()
+
+
+def test_step_is_deterministic():
+    n = (32, 32, 32)
+    mask = scenes.sphere_drop_mask(n, 8, (16, 23, 16), 5)
+    outs = []
+    for _ in range(2):
+        pos, vel = scenes.seed_particles(mask, 8, 3)
+        ctx = fsl.Context(n, capacity=pos.shape[0])
+        ctx.set_params(9.8, (0, 9.8, 0), 0.95)
+        ctx.set_box_boundary()
+        ctx.set_particles(pos, vel)
+        prm = fsl.solve_params(precision=fsl.PCG_MIXED, stop_mode=fsl.STOP_REL, threshold=1e-6)
+        for _ in range(3):
+            ctx.step(1.0 / 30, prm)
+        outs.append((ctx.get_particles(), ctx.get_vec()[0], ctx.get_scalar(fsl.GRID_PRESSURE)))
+        ctx.close()
+    (p0, v0, g0), a0, b0 = outs[0]
+    (p1, v1, g1), a1, b1 = outs[1]
+    for x, y in ((p0, p1), (v0, v1), (g0, g1), (a0, a1), (b0, b1)):
+        assert np.array_equal(x.view(np.uint32), y.view(np.uint32))
+
+
+def test_device_seeding_matches_host():
+    n = (40, 32, 24)
+    mask = scenes.dam_break_mask(n, 17, 13)
+    pos, vel = scenes.seed_particles(mask, 8, 9)
+    a = fsl.Context(n, capacity=pos.shape[0])
+    a.seed_particles(mask, 8, 9)
+    b = fsl.Context(n, capacity=pos.shape[0])
+    b.set_particles(pos, vel)
+    pa, va, ga = a.get_particles()
+    pb, vb, gb = b.get_particles()
+    assert np.array_equal(pa, pb) and np.array_equal(va, vb) and np.array_equal(ga, gb)
+    a.close(); b.close()
