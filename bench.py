@@ -1,0 +1,285 @@
+#!/usr/bin/env python3
+"""FLIP step benchmark: 256^3 dam break on one MI355X.
+
+Metric: FLIP steps/s with the mixed-precision PCG at relative residual 1e-6,
+plus the PCG's HBM bandwidth and the mixed-versus-plain comparison.
+
+Workload (dam break 256^3): fluid block x < 96, y < 160 (all z), solid walls on
+all faces, 8 Philox-jittered particles per fluid cell (31,457,280 particles,
+SoA float32), gravity 9.8 voxel/s^2 in -y, flip 0.95, cfl 1, 30 frames/s, the
+reference's 2x2x2 partitions as 8 tiles of one domain.
+
+A step is one FlipDriver::RunOneStep (advection + sort, clear, P2G, boundary
+conditions, gravity, divergence, PCG solve, pressure update, velocity update,
+extrapolation, G2P) with all data resident in HBM.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "fluid-sim-lb_amd"))
+sys.path.insert(0, ROOT)
+
+import fsl  # noqa: E402
+from fsl import scenes  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per fluid row (float vectors, 1-byte label; halo re-reads
+# not counted). Pass A reads r, p, label and writes p, Ap; pass B reads x, p,
+# r, Ap, label and writes x, r.
+PASS_A_BYTES = {fsl.PCG_FP32: 17, fsl.PCG_MIXED: 17, fsl.PCG_FP64: 33}
+PASS_B_BYTES = {fsl.PCG_FP32: 25, fsl.PCG_MIXED: 25, fsl.PCG_FP64: 49}
+PREC_NAME = {fsl.PCG_FP32: "fp32", fsl.PCG_FP64: "fp64", fsl.PCG_MIXED: "mixed"}
+
+
+def sync():
+    try:
+        import torch
+
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+
+
+def setup(dims, tiles, mask, seed, ppc=8):
+    ncell = int(mask.sum())
+    ctx = fsl.Context(dims, tiles=tiles, capacity=ncell * ppc + 1024)
+    ctx.set_params(9.8, (0.0, 9.8, 0.0), 0.95)
+    ctx.set_box_boundary()
+    ctx.seed_particles(mask, ppc, seed)
+    return ctx
+
+
+def pre_solve_stages(ctx, dt):
+    """The stages of RunOneStep that precede the solve (flip_app_sim.cc:213-244)."""
+    ctx.move_particles(dt)
+    ctx.clear()
+    ctx.p2g()
+    ctx.apply_bc()
+    ctx.save_velocities()
+    ctx.delete_outside()
+    ctx.add_gravity(dt)
+    ctx.apply_bc()
+    ctx.divergence()
+
+
+def pcg_compare(ctx, dims):
+    """Solve the same system with each precision (x0 = 0 every time)."""
+    out = {}
+    pressures = {}
+    runs = [
+        ("mixed", fsl.PCG_MIXED, fsl.STOP_REL, 1e-6),
+        ("fp32", fsl.PCG_FP32, fsl.STOP_REL, 1e-6),
+        ("fp64", fsl.PCG_FP64, fsl.STOP_REL, 1e-6),
+        ("fp32_reference_rule", fsl.PCG_FP32, fsl.STOP_ABS_RR, None),
+        ("fp64_tight", fsl.PCG_FP64, fsl.STOP_REL, 1e-12),
+    ]
+    for name, prec, stop, tol in runs:
+        prm = fsl.solve_params(precision=prec, stop_mode=stop, threshold=tol, dims=dims,
+                               max_iterations=20000)
+        ctx.set_timing(True)
+        st = ctx.solve(prm)
+        t = ctx.timing()
+        ctx.set_timing(False)
+        it = max(st["iterations"], 1)
+        rows = st["n_rows"]
+        bytes_it = (PASS_A_BYTES[prec] + PASS_B_BYTES[prec]) * rows
+        rec = {
+            "iterations": st["iterations"],
+            "solve_s": st["seconds"],
+            "ms_per_iteration": 1e3 * st["seconds"] / it,
+            "true_rel_residual": st["true_rel_residual"],
+            "pcg_GBps": bytes_it * it / st["seconds"] / 1e9 if st["seconds"] > 0 else None,
+            "pass_a_ms": t["pass_a"] / max(t["pass_a_n"], 1),
+            "pass_b_ms": t["pass_b"] / max(t["pass_b_n"], 1),
+        }
+        rec["pass_a_GBps"] = PASS_A_BYTES[prec] * rows / (rec["pass_a_ms"] * 1e-3) / 1e9
+        rec["pass_b_GBps"] = PASS_B_BYTES[prec] * rows / (rec["pass_b_ms"] * 1e-3) / 1e9
+        out[name] = rec
+        pressures[name] = ctx.get_scalar(fsl.GRID_PRESSURE).astype(np.float64)
+    ref = pressures["fp64_tight"]
+    den = np.linalg.norm(ref)
+    for name in ("mixed", "fp32", "fp64", "fp32_reference_rule"):
+        out[name]["pressure_rel_err_vs_fp64_1e-12"] = float(np.linalg.norm(pressures[name] - ref) / den)
+    f64 = pressures["fp64"]
+    out["mixed"]["pressure_rel_err_vs_fp64_same_rule"] = float(
+        np.linalg.norm(pressures["mixed"] - f64) / np.linalg.norm(f64))
+    out["speedup_mixed_vs_fp64"] = out["fp64"]["solve_s"] / out["mixed"]["solve_s"]
+    out["speedup_mixed_vs_fp32"] = out["fp32"]["solve_s"] / out["mixed"]["solve_s"]
+    out["rows"] = rows
+    return out
+
+
+def cpu_baseline(iters_256, ppc=8):
+    """Time the CPU oracle on the 128^3 dam break (1/8 of the 256^3 work: same
+    fill fractions, 491,520 rows, 3,932,160 particles) for one step with a
+    bounded PCG, then scale to a 256^3 step: non-PCG stages and the per-row
+    iteration cost x8, times the iteration count the 256^3 step needed."""
+    from oracle.oracle import Oracle
+
+    dims = (128, 128, 128)
+    mask = scenes.dam_break_mask(dims, 48, 80)
+    g = setup(dims, (1, 1, 1), mask, 2, ppc)
+    pos, vel, gv = g.get_particles()
+    g.close()
+    o = Oracle(dims)
+    o.set_params(9.8, (0.0, 9.8, 0.0), 0.95)
+    o.set_box_boundary()
+    o.set_particles(pos, vel, gv)
+    dt, _ = o.compute_dt(1, 1.0 / 30, 0.0)
+    t0 = time.perf_counter()
+    o.move_particles(dt); o.clear(); o.p2g(); o.apply_bc(); o.save_velocities()
+    o.delete_outside(); o.add_gravity(dt); o.apply_bc(); o.divergence()
+    t1 = time.perf_counter()
+    o.solve(precision=0, stop_mode=1, threshold=1e-6, max_iter=0)
+    t2 = time.perf_counter()
+    k = 40
+    _, it, _ = o.solve(precision=0, stop_mode=1, threshold=1e-6, max_iter=k)
+    t3 = time.perf_counter()
+    o.pressure_update(); o.velocity_update(); o.g2p()
+    t4 = time.perf_counter()
+    per_iter = ((t3 - t2) - (t2 - t1)) / max(it, 1)
+    step_256 = 8 * ((t1 - t0) + (t2 - t1) + (t4 - t3)) + 8 * per_iter * iters_256
+    return {
+        "value": 1.0 / step_256,
+        "unit": "FLIP steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"oracle (CPU restatement of the reference step, FP32 Jacobi-PCG) on the "
+                   f"128^3 dam break, one step with {it} PCG iterations: non-PCG stages "
+                   f"{t1 - t0 + t4 - t3:.2f} s, assembly {t2 - t1:.2f} s, "
+                   f"{per_iter * 1e3:.1f} ms/iteration; scaled x8 to 256^3 with the "
+                   f"{iters_256} iterations of the timed GPU step"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="dam_break_256")
+    ap.add_argument("--no-compare", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU only; the multi-GPU sharding of the "
+              "tiles is described in DESIGN.md and not run", file=sys.stderr)
+        sys.exit(2)
+
+    dims, mk, seed = scenes.WORKLOADS[a.workload]
+    mask = mk(dims)
+    tiles = (2, 2, 2) if dims[0] >= 256 else (1, 1, 1)
+    t_setup = time.perf_counter()
+    ctx = setup(dims, tiles, mask, seed)
+    n_particles = ctx.num_particles()
+    t_setup = time.perf_counter() - t_setup
+    prm = fsl.solve_params(precision=fsl.PCG_MIXED, stop_mode=fsl.STOP_REL, threshold=1e-6,
+                           dims=dims, max_iterations=20000)
+    frame = np.float32(1.0 / 30)
+    step_t = 0.0
+
+    def one_step():
+        nonlocal step_t
+        dt, step_t = ctx.compute_dt(1, frame, step_t)
+        if step_t + 1e-5 >= frame:
+            step_t = 0.0
+        return ctx.step(dt, prm)
+
+    for _ in range(a.warmup):
+        one_step()
+    ctx.set_timing(True)
+    sync()
+    t0 = time.perf_counter()
+    stats = [one_step() for _ in range(a.steps)]
+    sync()
+    elapsed = time.perf_counter() - t0
+    tm = ctx.timing()
+    ctx.set_timing(False)
+    steps_per_s = a.steps / elapsed
+    iters = [s["iterations"] for s in stats]
+    rows = [s["n_rows"] for s in stats]
+    rows_w = float(np.average(rows, weights=np.maximum(iters, 1)))
+    pa_ms = tm["pass_a"] / max(tm["pass_a_n"], 1)
+    pb_ms = tm["pass_b"] / max(tm["pass_b_n"], 1)
+    achieved = PASS_A_BYTES[fsl.PCG_MIXED] * rows_w / (pa_ms * 1e-3) / 1e9
+    pcg_time = tm["solve"] * 1e-3
+    pcg_gbps = sum((PASS_A_BYTES[2] + PASS_B_BYTES[2]) * r * i for r, i in zip(rows, iters)) / pcg_time / 1e9
+    stage_ms = {k: tm[k] / a.steps for k in ("move", "clear", "p2g", "bc_save", "gravity_bc",
+                                             "divergence", "solve", "update", "g2p")}
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "pcg_pass_a_traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            per_row = json.load(f).get("hbm_bytes_per_row")
+        if per_row:
+            traffic = per_row * rows_w
+
+    compare = None
+    if not a.no_compare:
+        dt, _ = ctx.compute_dt(1, frame, 0.0)
+        pre_solve_stages(ctx, dt)
+        compare = pcg_compare(ctx, dims)
+    ctx.close()
+
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline(int(round(float(np.mean(iters)))))
+
+    line = {
+        "metric": "FLIP steps/s at 256^3 (mixed-precision PCG, rel. residual 1e-6)",
+        "value": steps_per_s,
+        "unit": "FLIP steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * elapsed / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (stencil, axpys) + f64 (dot products, residual refresh)",
+        "data": "synthetic (Philox-seeded dam break)",
+        "config": {
+            "workload": f"{a.workload}: FLIP step, fluid x<96 y<160, 8 ppc, 8 tiles (2x2x2)",
+            "dims": list(dims),
+            "particles": n_particles,
+            "fluid_rows_mean": rows_w,
+            "pcg_iterations_per_step": iters,
+            "parallelism": "1 GPU, tiles as work units",
+        },
+        "pcg_GBps": pcg_gbps,
+        "pcg_frac_of_hbm_peak": pcg_gbps / HBM_PEAK_GBS,
+        "stage_ms_per_step": stage_ms,
+        "roofline": {
+            "kernel": "PCG pass A (p = z + beta p, 7-point SpMV from LDS, p.Ap)",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "bytes_per_row": PASS_A_BYTES[fsl.PCG_MIXED],
+            "rows": rows_w,
+            "avg_launch_ms": pa_ms,
+            "pass_b_avg_launch_ms": pb_ms,
+            "pass_b_GBps": PASS_B_BYTES[fsl.PCG_MIXED] * rows_w / (pb_ms * 1e-3) / 1e9,
+        },
+        "pcg_compare": compare,
+        "cpu_baseline": cpu,
+        "setup_s": t_setup,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

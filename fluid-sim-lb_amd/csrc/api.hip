@@ -167,6 +167,7 @@ int fsl_destroy(fsl_ctx *ctx) {
     if (ctx->P.a[a]) (void)hipFree(ctx->P.a[a]);
     if (ctx->Q.a[a]) (void)hipFree(ctx->Q.a[a]);
   }
+  for (hipEvent_t e : ctx->tev) if (e) (void)hipEventDestroy(e);
   if (ctx->flags_host) (void)hipHostFree(ctx->flags_host);
   if (ctx->pcg_host) (void)hipHostFree(ctx->pcg_host);
   if (ctx->host_u32) (void)hipHostFree(ctx->host_u32);
@@ -474,20 +475,57 @@ int fsl_g2p(fsl_ctx *ctx) {
 
 int fsl_step(fsl_ctx *ctx, float dt, const fsl_solve_params *params, fsl_solve_stats *stats) {
   CHECK_CTX();
+  if (!params) return FSL_ERR_ARG;
   int rc;
-  if ((rc = fsl_move_particles(ctx, dt))) return rc;
-  if ((rc = fsl_clear(ctx))) return rc;
-  if ((rc = fsl_p2g(ctx))) return rc;
-  if ((rc = fsl_apply_bc(ctx))) return rc;
-  if ((rc = fsl_save_velocities(ctx))) return rc;
+  auto mark = [&](int ev) -> int {
+    if (ctx->timing) FSL_HIP(hipEventRecord(ctx->tev[ev], ctx->stream));
+    return FSL_OK;
+  };
+#define STAGE(s, body)                     \
+  if ((rc = mark(2 * (s)))) return rc;     \
+  body;                                    \
+  if ((rc = mark(2 * (s) + 1))) return rc;
+  STAGE(FSL_T_MOVE, if ((rc = fsl_move_particles(ctx, dt))) return rc)
+  STAGE(FSL_T_CLEAR, if ((rc = fsl_clear(ctx))) return rc)
+  STAGE(FSL_T_P2G, if ((rc = fsl_p2g(ctx))) return rc)
+  // ApplyBoundaryConditions + SaveGridVelocities in one pass.
+  STAGE(FSL_T_BC_SAVE, FSL_HIP(fsl::launch_bc_save(ctx)); ctx->bc_applied = true)
   if ((rc = fsl_delete_outside(ctx))) return rc;
-  if ((rc = fsl_add_gravity(ctx, dt))) return rc;
-  if ((rc = fsl_apply_bc(ctx))) return rc;
-  if ((rc = fsl_divergence(ctx))) return rc;
-  if ((rc = fsl_pcg_solve(ctx, params, stats))) return rc;
-  if ((rc = fsl_pressure_update(ctx))) return rc;
-  if ((rc = fsl_velocity_update(ctx))) return rc;
-  return fsl_g2p(ctx);
+  STAGE(FSL_T_GRAVITY_BC, if ((rc = fsl_add_gravity(ctx, dt))) return rc;
+        if ((rc = fsl_apply_bc(ctx))) return rc)
+  STAGE(FSL_T_DIVERGENCE, if ((rc = fsl_divergence(ctx))) return rc)
+  STAGE(FSL_T_SOLVE, if ((rc = fsl_pcg_solve(ctx, params, stats))) return rc)
+  STAGE(FSL_T_UPDATE, if ((rc = fsl_pressure_update(ctx))) return rc;
+        if ((rc = fsl_velocity_update(ctx))) return rc)
+  STAGE(FSL_T_G2P, if ((rc = fsl_g2p(ctx))) return rc)
+#undef STAGE
+  if (ctx->timing) {
+    FSL_HIP(hipStreamSynchronize(ctx->stream));
+    for (int s = FSL_T_MOVE; s <= FSL_T_G2P; ++s) {
+      float ms = 0.0f;
+      FSL_HIP(hipEventElapsedTime(&ms, ctx->tev[2 * s], ctx->tev[2 * s + 1]));
+      ctx->tacc[s] += ms;
+    }
+    ctx->tacc[FSL_T_STEPS] += 1;
+  }
+  return FSL_OK;
+}
+
+int fsl_set_timing(fsl_ctx *ctx, int enable) {
+  CHECK_CTX();
+  if (enable && !ctx->tev[0]) {
+    for (hipEvent_t &e : ctx->tev) FSL_HIP(hipEventCreate(&e));
+  }
+  ctx->timing = enable != 0;
+  for (double &t : ctx->tacc) t = 0.0;
+  return FSL_OK;
+}
+
+int fsl_get_timing(fsl_ctx *ctx, double *out, int n) {
+  CHECK_CTX();
+  if (!out || n < 0) return FSL_ERR_ARG;
+  for (int q = 0; q < n && q < FSL_T_COUNT; ++q) out[q] = ctx->tacc[q];
+  return FSL_OK;
 }
 
 int fsl_compute_dt(fsl_ctx *ctx, int cfl, float frame_step, float *step, float *dt) {

@@ -126,6 +126,10 @@ struct fsl_ctx {
   // Flags and staging.
   fsl::DevFlags *flags = nullptr, *flags_host = nullptr;
   uint32_t *host_u32 = nullptr;  // pinned scratch for small readbacks
+  // Measurement hook.
+  bool timing = false;
+  double tacc[FSL_T_COUNT] = {};
+  hipEvent_t tev[128] = {};  // [0,32): stage pairs, [32,128): PCG pass triples
   std::string err;
 };
 
@@ -152,6 +156,7 @@ hipError_t launch_cell_counts_box(fsl_ctx *c, int32_t *out_dev);
 hipError_t launch_clear_leaves(fsl_ctx *c, const int32_t *list, int64_t n);
 hipError_t launch_bc(fsl_ctx *c);
 hipError_t launch_save(fsl_ctx *c);
+hipError_t launch_bc_save(fsl_ctx *c);
 hipError_t launch_gravity(fsl_ctx *c, float dt);
 hipError_t launch_divergence(fsl_ctx *c);
 hipError_t launch_pressure_update(fsl_ctx *c);

@@ -308,6 +308,13 @@ hipError_t launch_bc(fsl_ctx *c) {
                                                    0, nullptr, nullptr, nullptr, nullptr);
   return hipGetLastError();
 }
+hipError_t launch_bc_save(fsl_ctx *c) {
+  if (c->n_act == 0) return hipSuccess;
+  k_bc<<<xcd_grid(c->n_act), 512, 0, c->stream>>>(c->dom, c->act_list, c->n_act, c->solid,
+                                                   c->marker, c->vflag, c->v[0], c->v[1], c->v[2],
+                                                   1, c->u[0], c->u[1], c->u[2], c->vflag);
+  return hipGetLastError();
+}
 hipError_t launch_save(fsl_ctx *c) {
   if (c->n_act == 0) return hipSuccess;
   k_save<<<xcd_grid(c->n_act), 512, 0, c->stream>>>(c->dom, c->act_list, c->n_act, c->v[0],

@@ -23,6 +23,7 @@
 //           iterations the float iterate is folded into a double solution and
 //           the true residual b - A x is recomputed in double (a reliable
 //           update; CG continues with the same direction).
+#include <algorithm>
 #include <type_traits>
 
 #include "device_util.h"
@@ -139,8 +140,8 @@ __device__ inline bool converged(const PcgState *st, double rr) {
 template <typename V, typename A, bool ORD, bool MIXED>
 __global__ __launch_bounds__(512) void k_prepare(Dom d, const int32_t *flist, int64_t nf,
                                                  const uint8_t *lab, const float *div,
-                                                 const float *pres, V *x, V *r, double *x64,
-                                                 A *partials, PcgState *st) {
+                                                 const float *pres, int warm, V *x, V *r,
+                                                 double *x64, A *partials, PcgState *st) {
 #pragma clang fp contract(off)
   using W = typename std::conditional<MIXED, double, V>::type;  // residual arithmetic
   __shared__ W s_x[1000];
@@ -154,14 +155,15 @@ __global__ __launch_bounds__(512) void k_prepare(Dom d, const int32_t *flist, in
   const int64_t v = (leaf << 9) | t;
   const uint8_t l = lab[v];
   const bool f = is_fluid(l);
-  s_x[tix(lx, ly, lz)] = f ? W(pres[v]) : W(0);
+  const float x0 = warm && f ? pres[v] : 0.0f;
+  s_x[tix(lx, ly, lz)] = W(x0);
   s_f[tix(lx, ly, lz)] = f;
   if (t < 384) {
     int hx, hy, hz;
     halo_cell(t, hx, hy, hz);
     const int64_t hv = vox_of(d, o[0] + hx, o[1] + hy, o[2] + hz);
     const bool hf = is_fluid(lab[hv]);
-    s_x[tix(hx, hy, hz)] = hf ? W(pres[hv]) : W(0);
+    s_x[tix(hx, hy, hz)] = warm && hf ? W(pres[hv]) : W(0);
     s_f[tix(hx, hy, hz)] = hf;
   }
   __syncthreads();
@@ -170,8 +172,8 @@ __global__ __launch_bounds__(512) void k_prepare(Dom d, const int32_t *flist, in
     const W ax = stencil<W, ORD>(s_x, s_f, lx, ly, lz, l >> 2);
     const W bw = W(div[v]);
     const V rv = V(bw - ax);
-    if (MIXED) { x64[v] = double(pres[v]); x[v] = V(0); }
-    else x[v] = V(pres[v]);
+    if (MIXED) { x64[v] = double(x0); x[v] = V(0); }
+    else x[v] = V(x0);
     r[v] = rv;
     const V z = rv * dinv<V>(l >> 2);
     vals[0] = A(rv) * A(rv);
@@ -423,18 +425,24 @@ __global__ __launch_bounds__(512) void k_true_residual(Dom d, const int32_t *fli
 
 template <typename V, typename A, bool ORD, bool MIXED>
 struct Kernels {
-  static void prepare(fsl_ctx *c) {
+  static void prepare(fsl_ctx *c, int warm) {
     k_prepare<V, A, ORD, MIXED><<<xcd_grid(c->n_fluid), 512, 0, c->stream>>>(
-        c->dom, c->fluid_list, c->n_fluid, c->lab, c->div, c->pres, (V *)c->x, (V *)c->r,
+        c->dom, c->fluid_list, c->n_fluid, c->lab, c->div, c->pres, warm, (V *)c->x, (V *)c->r,
         c->x64, (A *)c->partials, c->pcg);
   }
-  static void iterate(fsl_ctx *c) {
+  static void pass_a(fsl_ctx *c) {
     k_pass_a<V, A, ORD><<<xcd_grid(c->n_fluid), 512, 0, c->stream>>>(
         c->dom, c->fluid_list, c->n_fluid, c->lab, (const V *)c->r, (V *)c->p[0], (V *)c->p[1],
         (V *)c->Ap, (A *)c->partials, c->pcg);
+  }
+  static void pass_b(fsl_ctx *c) {
     k_pass_b<V, A, ORD><<<xcd_grid(c->n_fluid), 512, 0, c->stream>>>(
         c->fluid_list, c->n_fluid, c->lab, (const V *)c->p[0], (const V *)c->p[1],
         (const V *)c->Ap, (V *)c->x, (V *)c->r, (A *)c->partials, c->pcg);
+  }
+  static void iterate(fsl_ctx *c) {
+    pass_a(c);
+    pass_b(c);
   }
   static void finish(fsl_ctx *c) {
     k_finish<V, MIXED><<<xcd_grid(c->n_act), 512, 0, c->stream>>>(
@@ -490,7 +498,8 @@ int pcg_prepare(fsl_ctx *ctx, const fsl_solve_params *prm) {
                : prm->threshold;
   FSL_HIP(hipMemcpyAsync(ctx->pcg, h, sizeof(PcgState), hipMemcpyHostToDevice, ctx->stream));
   if (ctx->n_fluid > 0) {
-    dispatch(ctx->pcg_precision, [&](auto k) { decltype(k)::prepare(ctx); });
+    dispatch(ctx->pcg_precision,
+             [&](auto k) { decltype(k)::prepare(ctx, prm->warm_start ? 1 : 0); });
     FSL_HIP(hipGetLastError());
   }
   int rc = sync_state(ctx);
@@ -529,15 +538,36 @@ int pcg_run(fsl_ctx *ctx, const fsl_solve_params *prm, fsl_solve_stats *st) {
   FSL_HIP(hipEventRecord(e0, ctx->stream));
   int it_host = ctx->pcg_host->it;
   const bool mixed = ctx->pcg_precision == FSL_PCG_MIXED;
+  // With the measurement hook on, every pass is bracketed by events (at most
+  // 32 iterations per batch, three events per iteration).
+  const int nb = ctx->timing ? std::min(batch, 32) : batch;
+  hipEvent_t *pev = ctx->tev + 32;
   while (!ctx->pcg_host->done) {
-    for (int b = 0; b < batch; ++b) {
-      dispatch(ctx->pcg_precision, [&](auto k) { decltype(k)::iterate(ctx); });
+    const int it_before = ctx->pcg_host->it;
+    for (int b = 0; b < nb; ++b) {
+      if (ctx->timing) FSL_HIP(hipEventRecord(pev[3 * b], ctx->stream));
+      dispatch(ctx->pcg_precision, [&](auto k) { decltype(k)::pass_a(ctx); });
+      if (ctx->timing) FSL_HIP(hipEventRecord(pev[3 * b + 1], ctx->stream));
+      dispatch(ctx->pcg_precision, [&](auto k) { decltype(k)::pass_b(ctx); });
+      if (ctx->timing) FSL_HIP(hipEventRecord(pev[3 * b + 2], ctx->stream));
       ++it_host;
       if (mixed && it_host % ctx->pcg_refresh == 0) launch_refresh(ctx, 0);
     }
     FSL_HIP(hipGetLastError());
     rc = sync_state(ctx);
     if (rc) return rc;
+    if (ctx->timing) {
+      const int worked = std::min(ctx->pcg_host->it - it_before, nb);
+      for (int b = 0; b < worked; ++b) {
+        float a = 0.0f, bb = 0.0f;
+        FSL_HIP(hipEventElapsedTime(&a, pev[3 * b], pev[3 * b + 1]));
+        FSL_HIP(hipEventElapsedTime(&bb, pev[3 * b + 1], pev[3 * b + 2]));
+        ctx->tacc[FSL_T_PASS_A] += a;
+        ctx->tacc[FSL_T_PASS_B] += bb;
+        ctx->tacc[FSL_T_PASS_A_N] += 1;
+        ctx->tacc[FSL_T_PASS_B_N] += 1;
+      }
+    }
     it_host = ctx->pcg_host->it;
     if (ctx->pcg_host->status) break;
     if (ctx->pcg_host->done && mixed) {

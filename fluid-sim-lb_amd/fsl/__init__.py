@@ -27,11 +27,14 @@ STOP_ABS_RR, STOP_REL = 0, 1
 GRID_VELOCITY, GRID_VELOCITY_UPDATE = 0, 1
 GRID_DIVERGENCE, GRID_PRESSURE = 0, 1
 GRID_MARKER, GRID_SOLID, GRID_ROW_INDEX, GRID_CELL_COUNT = 0, 1, 2, 3
+TIMING_NAMES = ("move", "clear", "p2g", "bc_save", "gravity_bc", "divergence", "solve",
+                "update", "g2p", "pass_a", "pass_b", "pass_a_n", "pass_b_n", "steps")
 
 
 class SolveParams(C.Structure):
     _fields_ = [("precision", C.c_int), ("stop_mode", C.c_int), ("threshold", C.c_double),
-                ("max_iterations", C.c_int), ("refresh_every", C.c_int), ("batch", C.c_int)]
+                ("max_iterations", C.c_int), ("refresh_every", C.c_int), ("batch", C.c_int),
+                ("warm_start", C.c_int)]
 
 
 class SolveStats(C.Structure):
@@ -91,6 +94,8 @@ SIGNATURES = {
     "fsl_pcg_prepare": [_P, C.POINTER(SolveParams), _DP, _DP],
     "fsl_pcg_iterate": [_P, _DP, _DP, _DP, _DP, _DP],
     "fsl_pcg_finish": [_P, C.POINTER(SolveStats)],
+    "fsl_set_timing": [_P, _I],
+    "fsl_get_timing": [_P, _DP, _I],
     "fsl_project_forward": [_P, C.POINTER(SolveParams), C.POINTER(SolveStats)],
     "fsl_project_backward": [_P, C.POINTER(SolveParams), _P, _P, C.POINTER(SolveStats)],
 }
@@ -131,7 +136,7 @@ def solve_params(precision=PCG_FP32, stop_mode=STOP_ABS_RR, threshold=None, dims
     if threshold is None:
         vol = int(np.prod(dims)) if dims is not None else 64 ** 3
         threshold = float(np.float32(np.float32(vol) * np.float32(0.5e-8)))
-    return SolveParams(precision, stop_mode, threshold, max_iterations, refresh_every, batch)
+    return SolveParams(precision, stop_mode, threshold, max_iterations, refresh_every, batch, 0)
 
 
 class Context:
@@ -275,6 +280,15 @@ class Context:
         st = SolveStats()
         self.call("fsl_project_backward", C.byref(params), _ptr(g), _ptr(out), C.byref(st))
         return out, st.as_dict()
+
+    # -- measurement hook
+    def set_timing(self, enable=True):
+        self.call("fsl_set_timing", int(enable))
+
+    def timing(self) -> dict:
+        out = (C.c_double * len(TIMING_NAMES))()
+        self.call("fsl_get_timing", out, len(TIMING_NAMES))
+        return dict(zip(TIMING_NAMES, list(out)))
 
     # -- PCG stage API
     def pcg_prepare(self, params: SolveParams):

@@ -69,6 +69,8 @@ typedef struct fsl_solve_params {
   int max_iterations; /* reference default 1500 (flip_app.h:52) */
   int refresh_every;  /* MIXED: FP64 residual refresh period (0 = 32) */
   int batch;          /* iterations launched between host checks (0 = 16) */
+  int warm_start;     /* 0: x0 = 0 (the cleared pressure grid, solver.cc:86-88);
+                         1: x0 = the current pressure grid */
 } fsl_solve_params;
 
 typedef struct fsl_solve_stats {
@@ -174,6 +176,28 @@ int fsl_pcg_iterate(fsl_ctx *ctx, double *pAp, double *alpha, double *rr, double
                     double *beta);
 /* CopySolution (solver.cc:93-99). */
 int fsl_pcg_finish(fsl_ctx *ctx, fsl_solve_stats *stats);
+
+/* ---- measurement hook (no reference counterpart) ----
+ * When enabled, fsl_step and fsl_pcg_solve record HIP events on the
+ * context's stream around each stage and each PCG pass; fsl_get_timing
+ * returns the accumulated milliseconds (and launch counts) since enabling. */
+#define FSL_T_MOVE 0      /* advection + counting sort */
+#define FSL_T_CLEAR 1     /* clear + active-leaf list */
+#define FSL_T_P2G 2
+#define FSL_T_BC_SAVE 3
+#define FSL_T_GRAVITY_BC 4
+#define FSL_T_DIVERGENCE 5 /* divergence + labels + fluid-leaf list */
+#define FSL_T_SOLVE 6     /* whole PCG solve */
+#define FSL_T_UPDATE 7    /* pressure update + velocity update */
+#define FSL_T_G2P 8       /* extrapolation + G2P */
+#define FSL_T_PASS_A 9    /* PCG pass A (p update + SpMV + p.Ap), summed */
+#define FSL_T_PASS_B 10   /* PCG pass B (axpys + r.r, r.z), summed */
+#define FSL_T_PASS_A_N 11 /* pass A launches that did work */
+#define FSL_T_PASS_B_N 12
+#define FSL_T_STEPS 13    /* fsl_step calls timed */
+#define FSL_T_COUNT 14
+int fsl_set_timing(fsl_ctx *ctx, int enable);
+int fsl_get_timing(fsl_ctx *ctx, double *out, int n);
 
 /* ---- differentiable projection (no reference counterpart) ---- */
 /* Forward: u_post = P(u) with the current labels, i.e. divergence, PCG,

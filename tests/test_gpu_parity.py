@@ -98,13 +98,33 @@ def run_stage_parity(n, mask, ppc, seed, steps=2, tank=False):
         assert rel_l2(ctx.get_scalar(fsl.GRID_DIVERGENCE), orc.get_scalar(0)) <= TOL
         gl = {tuple(int(c) for c in o) for o in ctx.active_leaves(fluid=True)}
         assert gl == fluid_leaves_from_marker(orc.get_int(0)), "fluid leaf index"
+        # The reference's rule (FP32, r.r < N^3 * 0.5e-8): iteration counts
+        # within +-2. Both sides sum FP32 dot products in different orders, so
+        # alpha and beta differ at ~1e-5 relative and the iterates drift by
+        # that much; the pressures must agree far inside the accuracy the rule
+        # itself guarantees (its distance to the exact solution).
         st = ctx.solve(prm)
+        p_gpu_rule = ctx.get_scalar(fsl.GRID_PRESSURE)
         rc, it, _ = orc.solve()
         assert rc == 0
         assert st["n_rows"] == orc.solver_sizes()[0]
         assert abs(st["iterations"] - it) <= 2, (st["iterations"], it)
         assert np.array_equal(ctx.get_int(fsl.GRID_ROW_INDEX), orc.get_int(2))
-        assert rel_l2(ctx.get_scalar(fsl.GRID_PRESSURE), orc.get_scalar(1)) <= TOL
+        p_ref_rule = orc.get_scalar(1)
+        # Then both solve to FP64 1e-10, and the rest of the step runs on that.
+        tight = fsl.solve_params(precision=fsl.PCG_FP64, stop_mode=fsl.STOP_REL, threshold=1e-10,
+                                 max_iterations=5000)
+        st64 = ctx.solve(tight)
+        rc, it64, _ = orc.solve(precision=1, stop_mode=1, threshold=1e-10, max_iter=5000)
+        assert rc == 0 and abs(st64["iterations"] - it64) <= 2
+        p_exact = orc.get_scalar(1)
+        rule_err = rel_l2(p_ref_rule, p_exact)
+        assert rel_l2(ctx.get_scalar(fsl.GRID_PRESSURE), p_exact) <= 1e-6
+        assert st64["true_rel_residual"] <= 1e-9
+        assert rel_l2(p_gpu_rule, p_ref_rule) <= 0.1 * rule_err + 1e-6, (
+            rel_l2(p_gpu_rule, p_ref_rule), rule_err)
+        print(f"step {s}: iterations gpu {st['iterations']} oracle {it}; reference-rule "
+              f"pressure error {rule_err:.2e}")
         ctx.pressure_update(); orc.pressure_update()
         check_grids(ctx, orc, f"step {s} pressure update")
         ctx.velocity_update(); assert orc.velocity_update() == 0
@@ -136,14 +156,15 @@ def test_stage_parity_small_dam_tank():
 def test_full_step_matches_oracle_sphere_drop():
     n = (64, 64, 64)
     ctx, orc = make_pair(n, scenes.WORKLOADS["sphere_drop_64"][1](n), 8, 1)
-    prm = fsl.solve_params(dims=n)
+    prm = fsl.solve_params(precision=fsl.PCG_FP64, stop_mode=fsl.STOP_REL, threshold=1e-10,
+                           max_iterations=5000)
     step = 0.0
     for s in range(4):
         dt, step = orc.compute_dt(1, 1.0 / 30, step)
         if step + 1e-5 >= 1.0 / 30:
             step = 0.0
         st = ctx.step(dt, prm)
-        rc, it = orc.run_one_step(dt)
+        rc, it = orc.run_one_step(dt, precision=1, stop_mode=1, threshold=1e-10, max_iter=5000)
         assert rc == 0 and abs(st["iterations"] - it) <= 2
         assert np.array_equal(ctx.get_int(fsl.GRID_CELL_COUNT), orc.cell_counts())
         assert np.array_equal(ctx.get_int(fsl.GRID_MARKER), orc.get_int(0))

@@ -1,0 +1,382 @@
+"""CPU checks that pin the oracle (no GPU).
+
+The reference ships no tests, golden vectors or fixtures for this path, so
+each oracle stage is pinned by a closed form, an invariant, a brute-force
+computation or the published known answers of an algorithm it uses:
+
+  scene seeding     Random123 Philox4x32-10 known-answer vectors
+  P2G (a7)          hand-computed single-particle weights; partition of unity;
+                    reproduction of a uniform field
+  G2P (a24)         exact interpolation of per-component linear fields;
+                    flip = 1 with dv = 0 keeps particle velocities
+  Laplacian (a15)   A p == -div(grad p) (summation by parts) against a
+                    brute-force dense Laplacian built from the labels
+  projection        div(u_new) on FLUID cells == b - A p (the final residual),
+                    i.e. ~0 after an FP64 solve to 1e-12
+  PCG (a17-a21)     FP64 PCG solution == scipy dense solve of the same system
+  CFL, dt (a6)      closed forms and the clamp branches
+  advection (a4/5)  clamp rules and deletion in solids / outside the box
+  extrapolation     two-hop corner value (order dependence) (a23)
+  hydrostatics      a pool at rest stays at rest
+"""
+import numpy as np
+import pytest
+import scipy.linalg
+
+from fsl import scenes
+from oracle.oracle import Oracle
+
+B = 1  # box offset: box index = cell index + 1
+
+
+def test_philox_known_answers():
+    f = scenes.philox4x32_10
+    assert [int(x[0]) for x in f([0], [0], [0], [0], 0, 0)] == [
+        0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]
+    m = 0xFFFFFFFF
+    assert [int(x[0]) for x in f([m], [m], [m], [m], m, m)] == [
+        0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]
+    assert [int(x[0]) for x in f([0x243F6A88], [0x85A308D3], [0x13198A2E], [0x03707344],
+                                 0xA4093822, 0x299F31D0)] == [
+        0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]
+
+
+def test_seeding_jitter_stays_in_cell():
+    n = (16, 16, 16)
+    mask = np.ones(n, np.uint8)
+    pos, vel = scenes.seed_particles(mask, 8, 123)
+    assert pos.shape == (16 ** 3 * 8, 3) and not vel.any()
+    cells = np.floor(pos).astype(int)
+    expect = np.repeat(np.argwhere(mask), 8, axis=0)
+    assert np.array_equal(cells, expect)
+
+
+def test_mask_counts_match_the_workloads():
+    n = (64, 64, 64)
+    assert int(scenes.WORKLOADS["sphere_drop_64"][1](n).sum()) == 69760
+    n = (128, 128, 128)
+    assert int(scenes.WORKLOADS["dam_break_128"][1](n).sum()) == 491520
+    assert int(scenes.WORKLOADS["sphere_drop_128"][1](n).sum()) == 557840
+
+
+def one_particle(pos, vel, n=(8, 8, 8)):
+    o = Oracle(n)
+    o.set_params(9.8, (0, 9.8, 0), 0.95)
+    o.set_particles(np.array([pos], np.float32), np.array([vel], np.float32))
+    o.clear()
+    o.p2g()
+    return o
+
+
+def test_p2g_single_particle_hand_computed():
+    pos = (2.3, 3.7, 4.1)
+    o = one_particle(pos, (1.0, 2.0, 3.0))
+    w, act = o.get_vec(2)
+    v, _ = o.get_vec(0)
+    p = np.array(pos, np.float64)
+    # Component d sits at the lower d-face: offset 0.5 on the other two axes.
+    for d in range(3):
+        lower = np.floor(p - 0.5)
+        lower[d] = np.floor(p[d])
+        frac = p - (lower + 0.5)
+        frac[d] = p[d] - lower[d]
+        for corner in np.ndindex(2, 2, 2):
+            c = np.array(corner)
+            wt = np.prod(np.where(c == 1, frac, 1.0 - frac))
+            idx = tuple((lower.astype(int) + c) + B)
+            assert abs(w[idx + (d,)] - wt) < 2e-6, (d, corner)
+            assert act[idx] == 1
+    # 8 corners per component; a face touched by any component is active.
+    assert act.sum() <= 24 and act.sum() >= 8
+    # Normalised velocity is the particle's where the weight is >= 1e-5.
+    for d, val in enumerate((1.0, 2.0, 3.0)):
+        m = w[..., d] >= 1e-5
+        assert np.allclose(v[..., d][m], val, rtol=2e-6)
+
+
+def test_p2g_partition_of_unity_and_uniform_field():
+    n = (16, 16, 16)
+    mask = scenes.sphere_drop_mask(n, 4, (8, 10, 8), 3)
+    pos, vel = scenes.seed_particles(mask, 8, 5)
+    vel[:] = (0.25, -1.5, 3.0)
+    o = Oracle(n)
+    o.set_particles(pos, vel)
+    o.clear()
+    o.p2g()
+    w, _ = o.get_vec(2)
+    v, act = o.get_vec(0)
+    assert np.allclose(w.reshape(-1, 3).sum(axis=0), pos.shape[0], rtol=1e-5)
+    for d, val in enumerate((0.25, -1.5, 3.0)):
+        m = w[..., d] >= 1e-5
+        assert np.allclose(v[..., d][m], val, rtol=1e-5, atol=1e-6)
+        assert np.all(v[..., d][(act == 1) & ~m] == 0)
+
+
+def test_g2p_exact_on_linear_fields():
+    n = (16, 16, 16)
+    rng = np.random.default_rng(0)
+    pos = (rng.random((400, 3)) * 13 + 1.5).astype(np.float32)
+    vel = rng.standard_normal((400, 3)).astype(np.float32)
+    o = Oracle(n)
+    o.set_params(9.8, (0, 9.8, 0), 0.0)  # flip = 0: PIC, vel = v(x)
+    o.set_particles(pos, vel)
+    ii = np.indices(tuple(x + 2 for x in n)).astype(np.float64) - B
+    coef = np.array([[0.5, 0.1, -0.2, 0.3], [-1.0, 0.05, 0.2, -0.1], [2.0, -0.3, 0.1, 0.15]])
+    field = np.zeros(ii.shape[1:] + (3,))
+    for d in range(3):
+        loc = [ii[a] + (0.5 if a != d else 0.0) for a in range(3)]
+        field[..., d] = coef[d, 0] + sum(coef[d, 1 + a] * loc[a] for a in range(3))
+    act = np.ones(ii.shape[1:], np.uint8)
+    o.set_vec(0, field.astype(np.float32), act)
+    o.set_vec(1, np.zeros_like(field, np.float32), act)
+    o.g2p()
+    p, v, gv, _ = o.get_particles()
+    # Extrapolation only touches the boundary layers; particles are interior.
+    expect = coef[:, 0][None, :] + p.astype(np.float64) @ coef[:, 1:].T
+    assert np.allclose(gv, expect, atol=2e-5)
+    assert np.allclose(v, expect, atol=2e-5)
+
+
+def test_g2p_flip_one_with_zero_update_keeps_velocity():
+    n = (8, 8, 8)
+    rng = np.random.default_rng(1)
+    pos = (rng.random((50, 3)) * 5 + 1.5).astype(np.float32)
+    vel = rng.standard_normal((50, 3)).astype(np.float32)
+    o = Oracle(n)
+    o.set_params(9.8, (0, 9.8, 0), 1.0)
+    o.set_particles(pos, vel)
+    act = np.ones((10, 10, 10), np.uint8)
+    o.set_vec(0, rng.standard_normal((10, 10, 10, 3)).astype(np.float32), act)
+    o.set_vec(1, np.zeros((10, 10, 10, 3), np.float32), act)
+    o.g2p()
+    p, v, _, _ = o.get_particles()
+    order_in = np.lexsort(pos.T[::-1])
+    order_out = np.lexsort(p.T[::-1])
+    assert np.allclose(v[order_out], vel[order_in], atol=1e-6)
+
+
+def random_labels(n, seed, p_fluid=0.6, p_solid=0.1):
+    rng = np.random.default_rng(seed)
+    box = tuple(x + 2 for x in n)
+    marker = np.zeros(box, np.int32)
+    inner = rng.random(n)
+    m = np.where(inner < p_fluid, 1, np.where(inner < p_fluid + p_solid, 2, 0))
+    marker[1:-1, 1:-1, 1:-1] = m
+    solid = np.zeros(box, bool)
+    solid[0, :, :] = solid[-1, :, :] = solid[:, 0, :] = solid[:, -1, :] = True
+    solid[:, :, 0] = solid[:, :, -1] = True
+    marker[solid] = 2
+    return marker
+
+
+def dense_laplacian(marker):
+    """Brute force: -Laplacian with Neumann SOLID and Dirichlet AIR."""
+    fl = np.argwhere(marker == 1)
+    index = {tuple(c): r for r, c in enumerate(fl)}
+    A = np.zeros((len(fl), len(fl)))
+    for r, c in enumerate(fl):
+        for d in range(3):
+            for s in (-1, 1):
+                nb = c.copy()
+                nb[d] += s
+                m = marker[tuple(nb)]
+                if m != 2:
+                    A[r, r] += 1
+                if m == 1:
+                    A[r, index[tuple(nb)]] -= 1
+    return A, fl
+
+
+def drop_pocketless(marker):
+    """Make every FLUID connected component touch AIR (else A is singular)."""
+    from scipy import ndimage
+    lab, k = ndimage.label(marker == 1)
+    air = ndimage.binary_dilation(marker == 0)
+    for comp in range(1, k + 1):
+        if not np.any(air[lab == comp]):
+            marker[lab == comp] = 0
+    return marker
+
+
+def test_laplacian_is_minus_div_grad():
+    n = (8, 8, 8)
+    marker = drop_pocketless(random_labels(n, 3))
+    A, fl = dense_laplacian(marker)
+    rng = np.random.default_rng(4)
+    p = np.zeros(marker.shape, np.float32)
+    pv = rng.standard_normal(len(fl)).astype(np.float32)
+    p[tuple(fl.T)] = pv
+    o = Oracle(n)
+    o.set_marker(marker)
+    o.set_scalar(1, p)
+    box = marker.shape
+    o.set_vec(0, np.zeros(box + (3,), np.float32), np.ones(box, np.uint8))
+    o.pressure_update()
+    o.divergence()
+    div = o.get_scalar(0)
+    assert np.allclose(div[tuple(fl.T)], -(A @ pv.astype(np.float64)), atol=1e-5)
+
+
+def test_projection_is_divergence_free_and_matches_dense_solve():
+    n = (8, 8, 8)
+    marker = drop_pocketless(random_labels(n, 7, p_fluid=0.7))
+    A, fl = dense_laplacian(marker)
+    rng = np.random.default_rng(8)
+    box = marker.shape
+    v = rng.standard_normal(box + (3,)).astype(np.float32)
+    o = Oracle(n)
+    o.set_marker(marker)
+    o.set_vec(0, v, np.ones(box, np.uint8))
+    o.divergence()
+    b = o.get_scalar(0)[tuple(fl.T)].astype(np.float64)
+    rc, it, rr = o.solve(precision=1, stop_mode=1, threshold=1e-12, max_iter=5000)
+    assert rc == 0 and it > 0
+    x = o.solution()
+    xd = scipy.linalg.solve(A, b)
+    assert np.allclose(x, xd, rtol=1e-9, atol=1e-9)
+    o.pressure_update()
+    o.divergence()
+    assert np.abs(o.get_scalar(0)[tuple(fl.T)]).max() < 1e-5
+
+
+def test_fp32_pcg_stops_on_reference_rule():
+    n = (16, 16, 16)
+    marker = drop_pocketless(random_labels(n, 11, p_fluid=0.8, p_solid=0.05))
+    rng = np.random.default_rng(12)
+    box = marker.shape
+    o = Oracle(n)
+    o.set_marker(marker)
+    o.set_vec(0, rng.standard_normal(box + (3,)).astype(np.float32), np.ones(box, np.uint8))
+    o.divergence()
+    thr = float(np.float32(np.float32(16 ** 3) * np.float32(0.5e-8)))
+    rc, it, rr = o.solve(precision=0, stop_mode=0, threshold=thr, max_iter=1500)
+    hist = o.residual_history()
+    assert rc == 0 and rr < thr
+    assert len(hist) == it + 1 and np.all(hist[:-1] >= thr)
+
+
+def test_cfl_closed_form_and_exclusions():
+    n = (8, 8, 8)
+    o = Oracle(n)
+    o.set_params(9.8, (0, 9.8, 0), 0.95)
+    box = (10, 10, 10)
+    v = np.zeros(box + (3,), np.float32)
+    act = np.ones(box, np.uint8)
+    o.set_vec(0, v, act)
+    assert o.cfl() == np.float32(1.0 / np.float32(np.sqrt(0.5 * np.float64(np.float32(9.8)))))
+    v[3 + B, 7 + B, 2 + B, 0] = 50.0   # u at j == end: excluded
+    v[8 + B, 2 + B, 2 + B, 1] = 60.0   # outside the partition box: excluded
+    v[2 + B, 2 + B, 2 + B, 2] = -7.0   # counted
+    o.set_vec(0, v, act)
+    assert o.cfl() == np.float32(1.0 / 7.0)
+
+
+def test_dt_clamps():
+    n = (8, 8, 8)
+    o = Oracle(n)
+    o.set_params(9.8, (0, 9.8, 0), 0.95)
+    fs = np.float32(1.0 / 30)
+    dt, step = o.compute_dt(1, fs, 0.0)       # CFL dt 0.45 > frame: one step
+    assert dt == fs and step == fs
+    box = (10, 10, 10)
+    v = np.zeros(box + (3,), np.float32)
+    v[2, 2, 2, 0] = 1000.0                     # dt below frame/10: floored
+    o.set_vec(0, v, np.ones(box, np.uint8))
+    dt, step = o.compute_dt(1, fs, 0.0)
+    assert dt == np.float32(fs / np.float32(10))
+    dt, step = o.compute_dt(1, fs, np.float32(0.029))  # snap to the frame end
+    assert dt == np.float32(fs - np.float32(0.029))
+
+
+def test_advection_clamps_and_deletes():
+    n = (8, 8, 8)
+    pos = np.array([[0.5, 0.5, 0.5], [7.5, 3.5, 3.5], [3.5, 7.5, 3.5]], np.float32)
+    gv = np.array([[-30, 0, 0], [30, 0, 0], [0, 30, 0]], np.float32)
+    for tank, survivors in ((False, 1), (True, 2)):
+        o = Oracle(n)
+        o.set_box_boundary(tank)
+        o.set_particles(pos, np.zeros_like(pos), gv)
+        o.move_particles(1.0)
+        p, _, _, cell = o.get_particles()
+        assert p.shape[0] == survivors
+        assert p[0, 0] == 0.0 and cell[0, 0] == 0        # clamped to 0
+        if tank:                                          # top cell kept, at y = n
+            assert p[1, 1] == 8.0 and cell[1, 1] == 8
+            o.clear(); o.p2g(); o.delete_outside()
+            assert o.num_particles() == 1
+
+
+def test_extrapolation_two_hop_corner():
+    n = (8, 8, 8)
+    o = Oracle(n)
+    box = (10, 10, 10)
+    rng = np.random.default_rng(2)
+    v = rng.standard_normal(box + (3,)).astype(np.float32)
+    o.set_vec(0, v, np.ones(box, np.uint8))
+    o.set_vec(1, v, np.ones(box, np.uint8))
+    o.extrapolate()
+    e, _ = o.get_vec(0)
+    # x-upper copies (n-1,0,k) -> (n,0,k); y-lower then copies (n,0,k) -> (n,-1,k).
+    k = 3
+    assert e[8 + B, -1 + B, k + B, 2] == v[7 + B, 0 + B, k + B, 2]
+    # Corner (-1,-1,k) is outside every copy range: untouched.
+    assert e[-1 + B, -1 + B, k + B, 2] == v[-1 + B, -1 + B, k + B, 2]
+
+
+def test_row_index_is_lexicographic():
+    n = (8, 8, 8)
+    marker = drop_pocketless(random_labels(n, 21))
+    o = Oracle(n)
+    o.set_marker(marker)
+    o.set_vec(0, np.zeros(marker.shape + (3,), np.float32), np.ones(marker.shape, np.uint8))
+    o.divergence()
+    o.solve(precision=1, stop_mode=1, threshold=1e-8)
+    idx = o.get_int(2)
+    fl = marker == 1
+    assert np.array_equal(idx[fl], np.arange(fl.sum()))
+    assert np.all(idx[~fl] == -1)
+
+
+def test_hydrostatic_pool_stays_at_rest():
+    n = (16, 16, 16)
+    mask = np.zeros(n, np.uint8)
+    mask[:, :6, :] = 1
+    pos, vel = scenes.seed_particles(mask, 8, 2)
+    o = Oracle(n)
+    o.set_params(9.8, (0, 9.8, 0), 0.95)
+    o.set_box_boundary()
+    o.set_particles(pos, vel)
+    step = 0.0
+    for _ in range(3):
+        dt, step = o.compute_dt(1, 1.0 / 30, step)
+        rc, _ = o.run_one_step(dt, precision=1, stop_mode=1, threshold=1e-10, max_iter=3000)
+        assert rc == 0
+    _, v, _, _ = o.get_particles()
+    # Without pressure the pool would fall at 3 g dt = 0.98 after 3 steps;
+    # gravity is balanced by pressure, and the residual motion is the free
+    # surface's sub-cell roughness.
+    assert np.abs(v).max() < 0.2 * 9.8 / 30
+    assert np.median(np.abs(v)) < 1e-3 * 9.8 / 30
+
+
+def test_sealed_pocket_and_enclosed_cell_errors():
+    n = (8, 8, 8)
+    box = (10, 10, 10)
+    marker = np.full(box, 2, np.int32)
+    marker[3:6, 3:6, 3:6] = 1       # sealed pocket with no AIR: singular
+    o = Oracle(n)
+    o.set_marker(marker)
+    o.set_vec(0, np.random.default_rng(0).standard_normal(box + (3,)).astype(np.float32),
+              np.ones(box, np.uint8))
+    o.divergence()
+    rc, _, _ = o.solve(precision=0, stop_mode=0, threshold=1e-30, max_iter=200)
+    # Singular (pure Neumann) system: the solve returns a status, never aborts.
+    assert rc in (0, -3, -4, -5)
+    marker = np.full(box, 2, np.int32)
+    marker[4, 4, 4] = 1             # one FLUID cell enclosed by SOLID: zero diagonal
+    o2 = Oracle(n)
+    o2.set_marker(marker)
+    o2.set_vec(0, np.zeros(box + (3,), np.float32), np.ones(box, np.uint8))
+    o2.divergence()
+    rc, _, _ = o2.solve()
+    assert rc == -2
