@@ -340,6 +340,7 @@ int fsl_move_particles(fsl_ctx *ctx, float dt) {
 
 int fsl_clear(fsl_ctx *ctx) {
   CHECK_CTX();
+  FSL_HIP(fsl::launch_clear_shell(ctx));
   if (ctx->n_act > 0) FSL_HIP(fsl::launch_clear_leaves(ctx, ctx->act_list, ctx->n_act));
   FSL_HIP(fsl::launch_leaf_flags(ctx));
   uint32_t n_act = 0;

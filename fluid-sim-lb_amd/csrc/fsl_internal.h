@@ -162,6 +162,7 @@ hipError_t launch_divergence(fsl_ctx *c);
 hipError_t launch_pressure_update(fsl_ctx *c);
 hipError_t launch_velocity_update(fsl_ctx *c);
 hipError_t launch_extrapolate(fsl_ctx *c);
+hipError_t launch_clear_shell(fsl_ctx *c);
 hipError_t launch_cfl(fsl_ctx *c);
 hipError_t launch_box_vec(fsl_ctx *c, int which, float *out3, uint8_t *act, bool to_box);
 hipError_t launch_box_scalar(fsl_ctx *c, float *grid, float *box, bool to_box);

@@ -194,10 +194,12 @@ __global__ __launch_bounds__(512) void k_velocity_update(Dom d, const int32_t *l
 // VectorGrid::ExtrapolateToBoundary (vector_grid.cc:302-381) for one axis
 // (lower and upper layers), on v and dv. The reference runs x, y, z in
 // sequence and later axes read values earlier axes wrote, so each axis is
-// its own launch. Destinations outside active leaves only ever receive zeros
-// and are skipped.
-__global__ void k_extrapolate(Dom d, int axis, const uint32_t *lact, float *vx, float *vy,
-                              float *vz, float *ux, float *uy, float *uz, uint8_t *vflag) {
+// its own launch. Every face in the copy ranges becomes active, as set()
+// activates in the reference; outside active leaves the copied values are the
+// background zeros, and the activity flags of the boundary layers are cleared
+// again by ClearGridData (k_clear_shell).
+__global__ void k_extrapolate(Dom d, int axis, float *vx, float *vy, float *vz, float *ux,
+                              float *uy, float *uz, uint8_t *vflag) {
   const int a = axis == 0 ? 1 : 0, b = axis == 2 ? 1 : 2;
   const int na = d.n[a] + 1, nb = d.n[b] + 1;
   const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -210,13 +212,24 @@ __global__ void k_extrapolate(Dom d, int axis, const uint32_t *lact, float *vx, 
   out[a] = in[a] = ia;
   out[b] = in[b] = ib;
   const int64_t vo = vox_of(d, out[0], out[1], out[2]);
-  if (!lact[vo >> 9]) return;
   const int64_t vi = vox_of(d, in[0], in[1], in[2]);
   float *gv[3] = {vx, vy, vz};
   float *gu[3] = {ux, uy, uz};
   gv[a][vo] = gv[a][vi]; gv[b][vo] = gv[b][vi];
   gu[a][vo] = gu[a][vi]; gu[b][vo] = gu[b][vi];
   vflag[vo] = 3;
+}
+
+// Activity flags of the six boundary layers (index -1 and n on each axis),
+// which extrapolation sets; part of ClearGridData.
+__global__ void k_clear_shell(Dom d, uint8_t *vflag) {
+  const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t per = int64_t(d.n[0] + 2) * (d.n[1] + 2) * (d.n[2] + 2);
+  if (q >= per) return;
+  const int by = d.n[1] + 2, bz = d.n[2] + 2;
+  const int i = int(q / (int64_t(by) * bz)) - 1, j = int((q / bz) % by) - 1, k = int(q % bz) - 1;
+  if (i == -1 || j == -1 || k == -1 || i == d.n[0] || j == d.n[1] || k == d.n[2])
+    vflag[vox_of(d, i, j, k)] = 0;
 }
 
 // CFL (flip_simulation.cc:459-488): max |v| over active faces inside the
@@ -354,8 +367,13 @@ hipError_t launch_extrapolate(fsl_ctx *c) {
     const int a = axis == 0 ? 1 : 0, b = axis == 2 ? 1 : 2;
     const int64_t n = 2 * int64_t(c->dom.n[a] + 1) * (c->dom.n[b] + 1);
     k_extrapolate<<<blocks_for(n, kThreads), kThreads, 0, c->stream>>>(
-        c->dom, axis, c->lact, c->v[0], c->v[1], c->v[2], c->u[0], c->u[1], c->u[2], c->vflag);
+        c->dom, axis, c->v[0], c->v[1], c->v[2], c->u[0], c->u[1], c->u[2], c->vflag);
   }
+  return hipGetLastError();
+}
+hipError_t launch_clear_shell(fsl_ctx *c) {
+  const int64_t nb = int64_t(c->dom.n[0] + 2) * (c->dom.n[1] + 2) * (c->dom.n[2] + 2);
+  k_clear_shell<<<blocks_for(nb, kThreads), kThreads, 0, c->stream>>>(c->dom, c->vflag);
   return hipGetLastError();
 }
 hipError_t launch_cfl(fsl_ctx *c) {
