@@ -1,6 +1,7 @@
 // C ABI of libfsl: context management, stage sequencing and host transfers.
 // All device work is enqueued on the context's stream; host reads of small
 // results (counts, the CFL maximum, solver scalars) go through pinned memory.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -121,17 +122,38 @@ int fsl_create(fsl_ctx **out, const int dims[3], const int tiles[3], int device,
   double *tmp = nullptr;
   if (!rc) { rc = dev_alloc(ctx, &tmp, nv); ctx->x = tmp; }
   if (!rc) { rc = dev_alloc(ctx, &tmp, nv); ctx->r = tmp; }
+  if (!rc) { rc = dev_alloc(ctx, &tmp, nv); ctx->z = tmp; }
   if (!rc) { rc = dev_alloc(ctx, &tmp, nv); ctx->p[0] = tmp; }
   if (!rc) { rc = dev_alloc(ctx, &tmp, nv); ctx->p[1] = tmp; }
   if (!rc) { rc = dev_alloc(ctx, &tmp, nv); ctx->Ap = tmp; }
   if (!rc) rc = dev_alloc(ctx, &ctx->x64, nv);
-  if (!rc) { rc = dev_alloc(ctx, &tmp, 3 * nl + 16); ctx->partials = tmp; }
+  if (!rc) { rc = dev_alloc(ctx, &tmp, fsl::pcg_partials_bytes() / sizeof(double)); ctx->partials = tmp; }
   if (!rc) rc = dev_alloc(ctx, &ctx->pcg, 1);
   if (!rc) rc = dev_alloc(ctx, &ctx->lhas, nl);
   if (!rc) rc = dev_alloc(ctx, &ctx->lact, nl);
   if (!rc) rc = dev_alloc(ctx, &ctx->lfluid, nl);
   if (!rc) rc = dev_alloc(ctx, &ctx->lscan, nl + 1);
   if (!rc) rc = dev_alloc(ctx, &ctx->act_list, nl);
+  if (!rc) rc = dev_alloc(ctx, &ctx->morton, nl);
+  if (!rc) rc = dev_alloc(ctx, &ctx->lflag_m, nl);
+  if (!rc) {
+    // Leaves sorted by the Morton code of their coordinates.
+    std::vector<std::pair<uint64_t, int32_t>> keyed(nl);
+    for (int64_t l = 0; l < ctx->nleaves; ++l) {
+      const int64_t l12 = int64_t(ctx->dom.L[1]) * ctx->dom.L[2];
+      const uint32_t c[3] = {uint32_t(l / l12), uint32_t((l / ctx->dom.L[2]) % ctx->dom.L[1]),
+                             uint32_t(l % ctx->dom.L[2])};
+      uint64_t code = 0;
+      for (int b = 0; b < 21; ++b)
+        for (int a = 0; a < 3; ++a) code |= uint64_t((c[a] >> b) & 1u) << (3 * b + (2 - a));
+      keyed[size_t(l)] = {code, int32_t(l)};
+    }
+    std::sort(keyed.begin(), keyed.end());
+    std::vector<int32_t> order(nl);
+    for (size_t q = 0; q < nl; ++q) order[q] = keyed[q].second;
+    if (hipMemcpy(ctx->morton, order.data(), nl * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess)
+      rc = FSL_ERR_HIP;
+  }
   if (!rc) rc = dev_alloc(ctx, &ctx->fluid_list, nl);
   for (int a = 0; a < 9 && !rc; ++a) rc = dev_alloc(ctx, &ctx->P.a[a], size_t(ctx->cap));
   for (int a = 0; a < 9 && !rc; ++a) rc = dev_alloc(ctx, &ctx->Q.a[a], size_t(ctx->cap));
@@ -158,9 +180,10 @@ int fsl_destroy(fsl_ctx *ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   void *bufs[] = {ctx->v[0], ctx->v[1], ctx->v[2], ctx->u[0], ctx->u[1], ctx->u[2], ctx->vflag,
-                  ctx->marker, ctx->solid, ctx->lab, ctx->div, ctx->pres, ctx->x, ctx->r,
+                  ctx->marker, ctx->solid, ctx->lab, ctx->div, ctx->pres, ctx->x, ctx->r, ctx->z,
                   ctx->p[0], ctx->p[1], ctx->Ap, ctx->x64, ctx->partials, ctx->pcg, ctx->lhas,
-                  ctx->lact, ctx->lfluid, ctx->lscan, ctx->act_list, ctx->fluid_list, ctx->key,
+                  ctx->lact, ctx->lfluid, ctx->lscan, ctx->act_list, ctx->fluid_list, ctx->morton,
+                  ctx->lflag_m, ctx->key,
                   ctx->perm, ctx->vcount, ctx->vstart, ctx->scan_tmp, ctx->flags};
   for (void *b : bufs) if (b) (void)hipFree(b);
   for (int a = 0; a < 9; ++a) {
@@ -400,8 +423,7 @@ int fsl_divergence(fsl_ctx *ctx) {
   FSL_HIP(hipMemsetAsync(ctx->lfluid, 0, size_t(ctx->nleaves) * sizeof(uint32_t), ctx->stream));
   FSL_HIP(hipMemsetAsync(ctx->flags, 0, sizeof(fsl::DevFlags), ctx->stream));
   FSL_HIP(fsl::launch_divergence(ctx));
-  FSL_HIP(fsl::exclusive_scan(ctx->lfluid, ctx->lscan, ctx->nleaves, ctx->scan_tmp, ctx->stream));
-  FSL_HIP(fsl::launch_compact(ctx, ctx->lfluid, ctx->lscan, ctx->fluid_list));
+  FSL_HIP(fsl::launch_compact(ctx, ctx->lfluid, ctx->fluid_list));
   uint32_t nf = 0;
   rc = read_u32(ctx, ctx->lscan + ctx->nleaves, &nf);
   if (rc) return rc;

@@ -66,7 +66,8 @@ __host__ __device__ inline int tile_of(const Dom &d, int i, int j, int k) {
 // Scalars of one PCG solve, resident on the device.
 struct PcgState {
   double rz, rz_old, rr, pAp, alpha, beta;
-  double bnorm2, thr, rr_true, b_unused;
+  double bnorm2, thr, rr_true;
+  double rzh[2];  // rz of iteration k in slot k & 1
   int it, done, status, max_iter;
   int stop_mode, precision, refresh_due, pad;
   unsigned cnt_a, cnt_b, cnt_r, cnt_x;
@@ -100,7 +101,7 @@ struct fsl_ctx {
   uint8_t *marker = nullptr, *solid = nullptr, *lab = nullptr;
   float *div = nullptr, *pres = nullptr;
   // PCG vectors (sized for doubles; FP32 modes use the first half).
-  void *x = nullptr, *r = nullptr, *p[2] = {}, *Ap = nullptr;
+  void *x = nullptr, *r = nullptr, *z = nullptr, *p[2] = {}, *Ap = nullptr;
   double *x64 = nullptr;
   void *partials = nullptr;
   fsl::PcgState *pcg = nullptr;
@@ -114,6 +115,8 @@ struct fsl_ctx {
   uint32_t *lfluid = nullptr; // leaf holds a FLUID cell
   uint32_t *lscan = nullptr;  // scan scratch over leaves (nleaves + 1)
   int32_t *act_list = nullptr, *fluid_list = nullptr;
+  int32_t *morton = nullptr;   // leaf indices in Morton (Z-curve) order of leaf coordinates
+  uint32_t *lflag_m = nullptr; // flags gathered in Morton order (scan input)
   int64_t n_act = 0, n_fluid = 0;
   bool bc_applied = false, grid_ready = false, outside_deleted = false;
   // Particles.
@@ -129,7 +132,7 @@ struct fsl_ctx {
   // Measurement hook.
   bool timing = false;
   double tacc[FSL_T_COUNT] = {};
-  hipEvent_t tev[128] = {};  // [0,32): stage pairs, [32,128): PCG pass triples
+  hipEvent_t tev[240] = {};  // [0,32): stage pairs, [32,232): 5 per PCG iteration
   std::string err;
 };
 
@@ -147,7 +150,7 @@ hipError_t launch_advect(fsl_ctx *c, float dt, bool drop_outside);
 hipError_t launch_sort_tail(fsl_ctx *c, uint32_t n_alive);
 hipError_t launch_mask_count(fsl_ctx *c, const uint8_t *mask_dev, int64_t n, uint32_t *cnt);
 hipError_t launch_leaf_flags(fsl_ctx *c);
-hipError_t launch_compact(fsl_ctx *c, const uint32_t *flag, const uint32_t *pos, int32_t *list);
+hipError_t launch_compact(fsl_ctx *c, const uint32_t *flag, int32_t *list);
 hipError_t launch_p2g(fsl_ctx *c);
 hipError_t launch_g2p(fsl_ctx *c);
 hipError_t launch_cell_counts_box(fsl_ctx *c, int32_t *out_dev);
@@ -170,6 +173,7 @@ hipError_t launch_box_u8(fsl_ctx *c, uint8_t *grid, int32_t *box, bool to_box);
 hipError_t launch_all_leaves(fsl_ctx *c);
 
 // pcg.hip
+size_t pcg_partials_bytes();
 int pcg_prepare(fsl_ctx *c, const fsl_solve_params *prm);
 int pcg_run(fsl_ctx *c, const fsl_solve_params *prm, fsl_solve_stats *st);
 int pcg_iterate_once(fsl_ctx *c);

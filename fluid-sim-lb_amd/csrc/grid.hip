@@ -297,12 +297,13 @@ __global__ void k_box_u8(Dom d, uint8_t *g, int32_t *box, int to_box) {
   else g[v] = uint8_t(box[q]);
 }
 
-__global__ void k_all_leaves(int64_t n, uint32_t *lact, uint32_t *lhas, int32_t *list) {
+__global__ void k_all_leaves(int64_t n, const int32_t *morton, uint32_t *lact, uint32_t *lhas,
+                             int32_t *list) {
   const int64_t l = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (l >= n) return;
   lact[l] = 1u;
   lhas[l] = 0u;
-  list[l] = int32_t(l);
+  list[l] = morton[l];
 }
 
 }  // namespace
@@ -404,7 +405,7 @@ hipError_t launch_box_u8(fsl_ctx *c, uint8_t *grid, int32_t *box, bool to_box) {
 }
 hipError_t launch_all_leaves(fsl_ctx *c) {
   k_all_leaves<<<blocks_for(c->nleaves, kThreads), kThreads, 0, c->stream>>>(
-      c->nleaves, c->lact, c->lhas, c->act_list);
+      c->nleaves, c->morton, c->lact, c->lhas, c->act_list);
   return hipGetLastError();
 }
 

@@ -154,9 +154,18 @@ __global__ void k_leaf_dilate(Dom d, const uint32_t *lhas, uint32_t *lact) {
   lact[l] = act;
 }
 
-__global__ void k_compact(int64_t n, const uint32_t *flag, const uint32_t *pos, int32_t *list) {
-  const int64_t l = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (l < n && flag[l]) list[pos[l]] = int32_t(l);
+// Leaf lists are emitted in Morton (Z-curve) order of the leaf coordinates:
+// a contiguous chunk of a list is then a compact 3-D block, so the per-XCD
+// chunks of the leaf kernels re-read their halos from their own L2.
+__global__ void k_gather_flags(int64_t n, const int32_t *order, const uint32_t *flag,
+                               uint32_t *fm) {
+  const int64_t m = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (m < n) fm[m] = flag[order[m]];
+}
+__global__ void k_compact(int64_t n, const int32_t *order, const uint32_t *fm,
+                          const uint32_t *pos, int32_t *list) {
+  const int64_t m = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (m < n && fm[m]) list[pos[m]] = order[m];
 }
 
 // ------------------------------------------------------------------ P2G
@@ -478,16 +487,19 @@ hipError_t launch_leaf_flags(fsl_ctx *c) {
   const int64_t nl = c->nleaves;
   k_leaf_has<<<blocks_for(nl, kThreads), kThreads, 0, c->stream>>>(nl, c->vstart, c->lhas);
   k_leaf_dilate<<<blocks_for(nl, kThreads), kThreads, 0, c->stream>>>(c->dom, c->lhas, c->lact);
-  hipError_t e = exclusive_scan(c->lact, c->lscan, nl, c->scan_tmp, c->stream);
-  if (e != hipSuccess) return e;
-  k_compact<<<blocks_for(nl, kThreads), kThreads, 0, c->stream>>>(nl, c->lact, c->lscan,
-                                                                   c->act_list);
-  return hipGetLastError();
+  return launch_compact(c, c->lact, c->act_list);
 }
 
-hipError_t launch_compact(fsl_ctx *c, const uint32_t *flag, const uint32_t *pos, int32_t *list) {
-  k_compact<<<blocks_for(c->nleaves, kThreads), kThreads, 0, c->stream>>>(c->nleaves, flag, pos,
-                                                                           list);
+// Compacts the flagged leaves into `list` in Morton order; the count is
+// left in lscan[nleaves].
+hipError_t launch_compact(fsl_ctx *c, const uint32_t *flag, int32_t *list) {
+  const int64_t nl = c->nleaves;
+  k_gather_flags<<<blocks_for(nl, kThreads), kThreads, 0, c->stream>>>(nl, c->morton, flag,
+                                                                        c->lflag_m);
+  hipError_t e = exclusive_scan(c->lflag_m, c->lscan, nl, c->scan_tmp, c->stream);
+  if (e != hipSuccess) return e;
+  k_compact<<<blocks_for(nl, kThreads), kThreads, 0, c->stream>>>(nl, c->morton, c->lflag_m,
+                                                                   c->lscan, list);
   return hipGetLastError();
 }
 

@@ -5,14 +5,22 @@
 // (solver.cc:262-366, solver_driver.cc:318-343). Here the operator is applied
 // from the 1-byte solver label of each cell (marker | diag << 2): row c has
 // diag(c) on the diagonal and -1 for every FLUID neighbour (solver.cc:
-// 214-231). One iteration is two passes over the fluid leaves:
-//   A: p = z + beta p (z = r / diag), Ap, p.Ap        (reads r, p, label)
-//   B: x += alpha p, r -= alpha Ap, r.r, r.z          (reads x, p, r, Ap, label)
-// with alpha and beta computed on the device by the last workgroup of each
-// pass, so the host only polls convergence every `batch` iterations. Each
-// workgroup owns one 8^3 leaf and stages p of the leaf plus its six face
-// halos in LDS; halo values of p are recomputed from r and the previous p, so
-// p is double-buffered.
+// 214-231). Iteration k is two kernels over the fluid leaves:
+//   A(k): beta = rz_k / rz_{k-1}; p = z + beta p; Ap; p.Ap
+//   B(k): alpha = rz_k / pAp; x += alpha p; r -= alpha Ap; z = r / diag; r.r; r.z
+// Each kernel publishes one partial sum per workgroup, and every workgroup of
+// the next kernel sums those partials itself, in the same fixed order, so all
+// workgroups hold bit-identical scalars: no atomics, no fences, no extra
+// reduction launches, and the host only polls convergence every `batch`
+// iterations. B's partials are double-buffered by iteration parity because
+// A(k) needs rz_k and rz_{k-1}.
+//
+// Workgroups are persistent over the leaf list: G = 8 x (blocks per XCD),
+// and XCD x walks the contiguous chunk x of the (spatially ordered) list, so a
+// leaf's halo neighbours are usually processed by the same XCD and hit its L2.
+// A group of 256 threads handles one leaf at a time (two cells per thread),
+// staging p of the leaf and its six face halos in LDS; halo values of p are
+// recomputed from z and the previous p, so p is double-buffered.
 //
 // Precision modes:
 //   FP32  - the reference's precision: float vectors, float dot products,
@@ -23,6 +31,8 @@
 //           iterations the float iterate is folded into a double solution and
 //           the true residual b - A x is recomputed in double (a reliable
 //           update; CG continues with the same direction).
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <type_traits>
 
@@ -32,6 +42,11 @@
 namespace fsl {
 
 namespace {
+
+constexpr int kG = 256;             // threads per leaf group (two cells each)
+constexpr int kGroups = 4;          // leaves in flight per workgroup
+constexpr int kT = kG * kGroups;    // threads per workgroup
+constexpr int kBlocksPerXcd = 64;   // 2 workgroups per CU, 32 CUs per XCD
 
 template <typename V>
 __device__ inline V dinv(int diag);
@@ -50,21 +65,21 @@ __device__ inline bool is_fluid(uint8_t lab) { return (lab & 3) == kFluid; }
 
 // LDS tile of a leaf plus its six face halos: index (x+1)*100 + (y+1)*10 + z+1.
 __device__ inline int tix(int x, int y, int z) { return (x + 1) * 100 + (y + 1) * 10 + (z + 1); }
+__device__ inline int tix_of_local(int c) { return tix(c >> 6, (c >> 3) & 7, c & 7); }
 
-// Halo cell handled by thread t < 384: face t/64 (x-, x+, y-, y+, z-, z+).
-__device__ inline void halo_cell(int t, int &x, int &y, int &z) {
-  const int f = t >> 6, a = (t >> 3) & 7, b = t & 7;
+// Halo cell h in [0, 384): face h/64 (x-, x+, y-, y+, z-, z+).
+__device__ inline void halo_cell(int h, int &x, int &y, int &z) {
+  const int f = h >> 6, a = (h >> 3) & 7, b = h & 7;
   const int s = (f & 1) ? 8 : -1;
   if (f < 2) { x = s; y = a; z = b; }
   else if (f < 4) { x = a; y = s; z = b; }
   else { x = a; y = b; z = s; }
 }
 
-// Apply the operator at local cell (x,y,z) from the LDS tile.
+// Apply the operator at LDS index c.
 template <typename V, bool ORD>
-__device__ inline V stencil(const V *sp, const uint8_t *sf, int x, int y, int z, int diag) {
+__device__ inline V stencil(const V *sp, const uint8_t *sf, int c, int diag) {
 #pragma clang fp contract(off)
-  const int c = tix(x, y, z);
   if (ORD) {
     // Row dot over columns in ascending row-index order (solver.cc:127-148):
     // x-1, y-1, z-1, self, z+1, y+1, x+1.
@@ -85,382 +100,635 @@ __device__ inline V stencil(const V *sp, const uint8_t *sf, int x, int y, int z,
     nb += sf[c + 10] ? sp[c + 10] : V(0);
     nb += sf[c - 1] ? sp[c - 1] : V(0);
     nb += sf[c + 1] ? sp[c + 1] : V(0);
-    return V(diag) * sp[c] - nb;
+    return __builtin_fma(V(diag), sp[c], -nb);
   }
 }
 
-template <typename T>
-__device__ inline T ld_coherent(const T *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// Operator from a tile in which every non-FLUID cell holds 0 (so no flags).
+template <typename V, bool ORD>
+__device__ inline V stencil0(const V *sp, int c, int diag) {
+#pragma clang fp contract(off)
+  if (ORD) {
+    // Column order of the reference's row dot (solver.cc:127-148); adding
+    // -0 for a non-FLUID neighbour leaves the sum unchanged.
+    V s = V(0);
+    s = s + (-sp[c - 100]);
+    s = s + (-sp[c - 10]);
+    s = s + (-sp[c - 1]);
+    { const V m = V(diag) * sp[c]; s = s + m; }
+    s = s + (-sp[c + 1]);
+    s = s + (-sp[c + 10]);
+    s = s + (-sp[c + 100]);
+    return s;
+  } else {
+    const V nb = ((sp[c - 100] + sp[c + 100]) + (sp[c - 10] + sp[c + 10])) + (sp[c - 1] + sp[c + 1]);
+    return __builtin_fma(V(diag), sp[c], -nb);
+  }
 }
 
-// Block-reduce K values, publish this workgroup's partials, and return true
-// in the last workgroup to finish, which receives the totals (summed over
-// workgroups in list order, so the result is deterministic).
+// Persistent walk over the leaf list: XCD x = blockIdx % 8 owns the list
+// chunk x; its workgroups stride through it, kGroups leaves per round (leaf
+// group g = threadIdx / kG takes every kGroups-th leaf of the workgroup).
+struct Walk {
+  int64_t w, end, step;
+  __device__ bool more() const { return w < end; }
+  __device__ int64_t mine() const { return w + int64_t(threadIdx.x / kG) * step; }
+  __device__ bool valid() const { return mine() < end; }
+  __device__ void next() { w += int64_t(kGroups) * step; }
+};
+__device__ inline Walk walk(int64_t nf) {
+  const int64_t per = gridDim.x / kXcd;
+  const int64_t chunk = (nf + kXcd - 1) / kXcd;
+  const int64_t beg = int64_t(blockIdx.x % kXcd) * chunk;
+  return {beg + blockIdx.x / kXcd, beg + chunk < nf ? beg + chunk : nf, per};
+}
+__device__ inline int gtid() { return threadIdx.x % kG; }
+
+// Block-reduce K values and publish this workgroup's partial sums.
 template <typename A, int K>
-__device__ bool reduce_last(A vals[K], A *partials, int64_t w, int64_t nwork, unsigned *counter,
-                            double out[K]) {
+__device__ inline void publish(A vals[K], A *partials) {
   __shared__ A s_red[K][16];
-  __shared__ int s_last;
 #pragma unroll
   for (int k = 0; k < K; ++k) vals[k] = block_sum<A>(vals[k], s_red[k]);
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int k = 0; k < K; ++k) partials[w * K + k] = vals[k];
-    __threadfence();
-    const unsigned prev = atomicAdd(counter, 1u);
-    s_last = prev == unsigned(nwork - 1);
+    for (int k = 0; k < K; ++k) partials[int64_t(blockIdx.x) * K + k] = vals[k];
   }
-  __syncthreads();
-  if (!s_last) return false;
-  __threadfence();
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    if constexpr (sizeof(A) == 8) {
-      double s = 0.0, c = 0.0;
-      for (int64_t q = threadIdx.x; q < nwork; q += blockDim.x)
-        neumaier_add(s, c, double(ld_coherent(&partials[q * K + k])));
-      out[k] = double(block_sum<double>(s + c, reinterpret_cast<double *>(s_red[k])));
-    } else {
-      A s = A(0);
-      for (int64_t q = threadIdx.x; q < nwork; q += blockDim.x) s += ld_coherent(&partials[q * K + k]);
-      out[k] = double(block_sum<A>(s, s_red[k]));
-    }
+}
+
+// Total of value k over the n published partials (stride K), summed by the
+// whole workgroup in a fixed order: every workgroup gets the same bits.
+template <typename A, int K>
+__device__ inline double total(const A *partials, int n, int k) {
+  if constexpr (sizeof(A) == 8) {
+    __shared__ double s_d[16];
+    double s = 0.0, c = 0.0;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) neumaier_add(s, c, double(partials[q * K + k]));
+    return block_sum<double>(s + c, s_d);
+  } else {
+    __shared__ A s_a[16];
+    A s = A(0);
+    for (int q = threadIdx.x; q < n; q += blockDim.x) s += partials[q * K + k];
+    return double(block_sum<A>(s, s_a));
   }
-  if (threadIdx.x == 0) *counter = 0u;
-  return true;
 }
 
 __device__ inline bool converged(const PcgState *st, double rr) {
   return st->stop_mode == FSL_STOP_ABS_RR ? rr < st->thr : sqrt(rr) < st->thr;
 }
 
-// PreparePCGSolver (solver.cc:262-286): x = x0 (the pressure grid), r = b - A x,
-// z = M^-1 r, rz, rr; also ||b||^2 for the relative stopping rule.
-template <typename V, typename A, bool ORD, bool MIXED>
-__global__ __launch_bounds__(512) void k_prepare(Dom d, const int32_t *flist, int64_t nf,
-                                                 const uint8_t *lab, const float *div,
-                                                 const float *pres, int warm, V *x, V *r,
-                                                 double *x64, A *partials, PcgState *st) {
-#pragma clang fp contract(off)
-  using W = typename std::conditional<MIXED, double, V>::type;  // residual arithmetic
-  __shared__ W s_x[1000];
-  __shared__ uint8_t s_f[1000];
-  const int64_t w = xcd_work(nf);
-  if (w >= nf) return;
-  const int64_t leaf = flist[w];
-  int o[3];
-  leaf_origin(d, leaf, o);
-  const int t = threadIdx.x, lx = t >> 6, ly = (t >> 3) & 7, lz = t & 7;
-  const int64_t v = (leaf << 9) | t;
-  const uint8_t l = lab[v];
-  const bool f = is_fluid(l);
-  const float x0 = warm && f ? pres[v] : 0.0f;
-  s_x[tix(lx, ly, lz)] = W(x0);
-  s_f[tix(lx, ly, lz)] = f;
-  if (t < 384) {
-    int hx, hy, hz;
-    halo_cell(t, hx, hy, hz);
-    const int64_t hv = vox_of(d, o[0] + hx, o[1] + hy, o[2] + hz);
-    const bool hf = is_fluid(lab[hv]);
-    s_x[tix(hx, hy, hz)] = warm && hf ? W(pres[hv]) : W(0);
-    s_f[tix(hx, hy, hz)] = hf;
+// Loads p of a leaf and its halo into LDS, where cell values come from
+// value(global voxel index, label) and non-FLUID cells hold 0.
+template <typename V, typename F>
+__device__ inline void fill_tile(const Dom &d, const uint8_t *lab, int64_t leaf, const int o[3],
+                                 V *sp, uint8_t *sf, uint8_t own_lab[2], F value) {
+  const int t = gtid();
+  uint8_t hl[2] = {0, 0};
+  int64_t hv[2] = {0, 0};
+  int hix[2] = {0, 0};
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const int c = t + s * kG;
+    own_lab[s] = lab[(leaf << 9) | c];
+    const int h = t + s * kG;
+    if (h < 384) {
+      int hx, hy, hz;
+      halo_cell(h, hx, hy, hz);
+      hv[s] = vox_of(d, o[0] + hx, o[1] + hy, o[2] + hz);
+      hl[s] = lab[hv[s]];
+      hix[s] = tix(hx, hy, hz);
+    }
   }
-  __syncthreads();
-  A vals[3] = {A(0), A(0), A(0)};
-  if (f) {
-    const W ax = stencil<W, ORD>(s_x, s_f, lx, ly, lz, l >> 2);
-    const W bw = W(div[v]);
-    const V rv = V(bw - ax);
-    if (MIXED) { x64[v] = double(x0); x[v] = V(0); }
-    else x[v] = V(x0);
-    r[v] = rv;
-    const V z = rv * dinv<V>(l >> 2);
-    vals[0] = A(rv) * A(rv);
-    vals[1] = A(rv) * A(z);
-    vals[2] = A(div[v]) * A(div[v]);
+  // Values are read for every cell and selected afterwards, so the data
+  // loads do not wait for the label loads (storage covers every leaf, so the
+  // reads stay in bounds; values of non-FLUID cells are discarded).
+  V own_val[2], halo_val[2] = {V(0), V(0)};
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    own_val[s] = value((leaf << 9) | (t + s * kG), own_lab[s]);
+    if (t + s * kG < 384) halo_val[s] = value(hv[s], hl[s]);
   }
-  double tot[3];
-  if (reduce_last<A, 3>(vals, partials, w, nf, &st->cnt_x, tot) && threadIdx.x == 0) {
-    st->rr = tot[0];
-    st->rz = tot[1];
-    st->bnorm2 = tot[2];
-    if (st->stop_mode == FSL_STOP_REL) st->thr = st->thr * sqrt(tot[2]);
-    st->it = 0;
-    st->status = isnan(tot[0]) ? FSL_ERR_NAN : 0;
-    st->done = (st->status != 0 || converged(st, tot[0]) || st->max_iter <= 0) ? 1 : 0;
-    st->pAp = 0; st->alpha = 0; st->beta = 0; st->rz_old = 0;
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const int c = t + s * kG;
+    const bool f = is_fluid(own_lab[s]);
+    sp[tix_of_local(c)] = f ? own_val[s] : V(0);
+    sf[tix_of_local(c)] = f;
+    if (t + s * kG < 384) {
+      const bool hf = is_fluid(hl[s]);
+      sp[hix[s]] = hf ? halo_val[s] : V(0);
+      sf[hix[s]] = hf;
+    }
   }
 }
 
-// Pass A: p = z + beta p (p = z on the first iteration); Ap = A p; p.Ap;
-// alpha = rz / pAp (ComputeLocal1 + ComputeLocal3, solver.cc:296-309, 355-366).
-template <typename V, typename A, bool ORD>
-__global__ __launch_bounds__(512) void k_pass_a(Dom d, const int32_t *flist, int64_t nf,
-                                                const uint8_t *lab, const V *r, V *p0, V *p1,
-                                                V *Ap, A *partials, PcgState *st) {
+// PreparePCGSolver (solver.cc:262-286): x = x0 (0, or the pressure grid when
+// warm starting), r = b - A x, z = M^-1 r; publishes (r.r, r.z) as B(-1) and
+// b.b for the relative stopping rule.
+template <typename V, typename A, bool ORD, bool MIXED>
+__global__ __launch_bounds__(kT) void k_prepare(Dom d, const int32_t *flist, int64_t nf,
+                                                const uint8_t *lab, const float *div,
+                                                const float *pres, int warm, V *x, V *r,
+                                                V *zv, double *x64, A *part_b, A *part_bb) {
 #pragma clang fp contract(off)
-  __shared__ V s_p[1000];
-  __shared__ uint8_t s_f[1000];
-  if (st->done) return;
-  const int64_t w = xcd_work(nf);
-  if (w >= nf) return;
-  const bool first = st->it == 0;
-  const V beta = V(st->beta);
-  const V *pin = st->pad ? p1 : p0;
-  V *pout = st->pad ? p0 : p1;
-  const int64_t leaf = flist[w];
-  int o[3];
-  leaf_origin(d, leaf, o);
-  const int t = threadIdx.x, lx = t >> 6, ly = (t >> 3) & 7, lz = t & 7;
-  const int64_t v = (leaf << 9) | t;
-  const uint8_t l = lab[v];
-  const bool f = is_fluid(l);
-  V pn = V(0);
-  if (f) {
-    const V z = r[v] * dinv<V>(l >> 2);
-    if (first) pn = z;
-    else { const V m = beta * pin[v]; pn = z + m; }
-  }
-  s_p[tix(lx, ly, lz)] = pn;
-  s_f[tix(lx, ly, lz)] = f;
-  if (t < 384) {
-    int hx, hy, hz;
-    halo_cell(t, hx, hy, hz);
-    const int64_t hv = vox_of(d, o[0] + hx, o[1] + hy, o[2] + hz);
-    const uint8_t hl = lab[hv];
-    const bool hf = is_fluid(hl);
-    V hp = V(0);
-    if (hf) {
-      const V z = r[hv] * dinv<V>(hl >> 2);
-      if (first) hp = z;
-      else { const V m = beta * pin[hv]; hp = z + m; }
+  using W = typename std::conditional<MIXED, double, V>::type;  // residual arithmetic
+  __shared__ W s_x[kGroups][1000];
+  __shared__ uint8_t s_f[kGroups][1000];
+  const int g = threadIdx.x / kG;
+  A acc[3] = {A(0), A(0), A(0)};
+  for (Walk q = walk(nf); q.more(); q.next()) {
+    const bool valid = q.valid();
+    const int64_t leaf = valid ? flist[q.mine()] : 0;
+    int o[3];
+    leaf_origin(d, leaf, o);
+    uint8_t ol[2] = {0, 0};
+    if (valid)
+      fill_tile<W>(d, lab, leaf, o, s_x[g], s_f[g], ol,
+                   [&](int64_t v, uint8_t) { return warm ? W(pres[v]) : W(0); });
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int c = gtid() + s * kG;
+      const int64_t v = (leaf << 9) | c;
+      if (!valid || !is_fluid(ol[s])) continue;
+      const W ax = stencil<W, ORD>(s_x[g], s_f[g], tix_of_local(c), ol[s] >> 2);
+      const float b = div[v];
+      const V rv = V(W(b) - ax);
+      const float x0 = warm ? pres[v] : 0.0f;
+      if (MIXED) { x64[v] = double(x0); x[v] = V(0); }
+      else x[v] = V(x0);
+      r[v] = rv;
+      const V z = rv * dinv<V>(ol[s] >> 2);
+      zv[v] = z;
+      acc[0] += A(rv) * A(rv);
+      acc[1] += A(rv) * A(z);
+      acc[2] += A(b) * A(b);
     }
-    s_p[tix(hx, hy, hz)] = hp;
-    s_f[tix(hx, hy, hz)] = hf;
+    __syncthreads();
   }
-  __syncthreads();
-  V ap = V(0);
-  if (f) ap = stencil<V, ORD>(s_p, s_f, lx, ly, lz, l >> 2);
-  pout[v] = pn;
-  Ap[v] = ap;
-  A vals[1] = {f ? A(pn) * A(ap) : A(0)};
-  double tot[1];
-  if (reduce_last<A, 1>(vals, partials, w, nf, &st->cnt_a, tot) && threadIdx.x == 0) {
-    const A pAp = A(tot[0]);
-    const A alpha = A(st->rz) / pAp;
-    st->pAp = double(pAp);
-    st->alpha = double(alpha);
-    if (!(double(alpha) <= 1e5)) {  // CHECK_LE(alpha, LARGE_FLOAT), solver.cc:333
+  A two[2] = {acc[0], acc[1]};
+  publish<A, 2>(two, part_b);
+  A one[1] = {acc[2]};
+  publish<A, 1>(one, part_bb);
+}
+
+// z, p0 and p1 start at 0 on every cell of the active leaves (the PCG
+// kernels keep them 0 off the FLUID cells).
+template <typename V>
+__global__ __launch_bounds__(kT) void k_zero(const int32_t *list, int64_t n, V *a, V *b, V *c) {
+  for (Walk q = walk(n); q.more(); q.next()) {
+    if (!q.valid()) continue;
+    const int64_t base = int64_t(list[q.mine()]) << 9;
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int64_t v = base | (gtid() + s * kG);
+      a[v] = V(0); b[v] = V(0); c[v] = V(0);
+    }
+  }
+}
+
+// Scalars after preparation: thresholds and the initial stopping test.
+template <typename A>
+__global__ __launch_bounds__(1024) void k_prepare_scalars(const A *part_b, const A *part_bb,
+                                                          int g, PcgState *st) {
+  const double rr = total<A, 2>(part_b, g, 0), rz = total<A, 2>(part_b, g, 1);
+  const double bb = total<A, 1>(part_bb, g, 0);
+  if (threadIdx.x != 0) return;
+  st->rr = rr;
+  st->rz = rz;
+  st->rzh[0] = rz;
+  st->bnorm2 = bb;
+  if (st->stop_mode == FSL_STOP_REL) st->thr = st->thr * sqrt(bb);
+  st->it = 0;
+  st->status = isnan(rr) ? FSL_ERR_NAN : 0;
+  st->done = (st->status != 0 || converged(st, rr) || st->max_iter <= 0) ? 1 : 0;
+}
+
+// A(k): ComputeLocal3 of iteration k-1 (beta, p = z + beta p,
+// solver.cc:355-366), the While test (solver_driver.cc:330) and ComputeLocal1
+// (Ap, pAp, solver.cc:296-309). rz_{k-1} comes from the device scalars
+// (slot (k-1)&1); rz_k is published there for B(k) and A(k+1).
+// z and p hold 0 on every non-FLUID cell of the active leaves, so the halo
+// needs no labels: p = z + beta p is 0 there and drops out of the stencil.
+template <typename V, typename A, bool ORD>
+__global__ __launch_bounds__(kT) void k_pass_a(Dom d, const int32_t *flist, int64_t nf,
+                                               const uint8_t *lab, const V *zv, const V *pin,
+                                               V *pout, V *Ap, const A *part_b1, A *part_a,
+                                               int g_parts, int k, PcgState *st) {
+#pragma clang fp contract(off)
+  __shared__ V s_p[2][kGroups][1000];
+  if (st->done) return;
+  V beta = V(0);
+  if (k > 0) {
+    const double rr = total<A, 2>(part_b1, g_parts, 0);
+    const A rz = A(total<A, 2>(part_b1, g_parts, 1));
+    const A rz_old = A(st->rzh[(k - 1) & 1]);
+    const A b = rz / rz_old;
+    int status = 0;
+    if (isnan(rr)) status = FSL_ERR_NAN;
+    else if (!(double(b) <= 1e5)) status = FSL_ERR_BETA;  // CHECK_LE(beta, LARGE_FLOAT), solver.cc:361
+    const bool stop = status != 0 || converged(st, rr) || k >= st->max_iter;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->it = k;
+      st->rr = rr;
+      st->rz = double(rz);
+      st->rz_old = double(rz_old);
+      st->beta = double(b);
+      st->rzh[k & 1] = double(rz);
+      if (status) st->status = status;
+      if (stop) st->done = 1;
+    }
+    if (stop) return;
+    beta = V(b);
+  }
+  const int g = threadIdx.x / kG, t = gtid();
+  A acc = A(0);
+  auto pval = [&](V zz, V pp) {
+    if (k == 0) return zz;
+    const V m = beta * pp;
+    return zz + m;
+  };
+  // Software pipeline: the loads of round j+1 are in flight while round j is
+  // staged, synchronised and computed; the LDS tile is double-buffered, so
+  // one barrier per round suffices.
+  struct Pre {
+    V oz[2], op[2], hz[2], hp[2];
+    uint8_t ol[2];
+    int hix[2];
+    int64_t leaf;
+    bool valid;
+  };
+  auto load = [&](const Walk &q, Pre &x) {
+    x.valid = q.valid();
+    x.leaf = x.valid ? flist[q.mine()] : 0;
+    x.ol[0] = x.ol[1] = 0;
+    if (!x.valid) return;
+    int o[3];
+    leaf_origin(d, x.leaf, o);
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int64_t v = (x.leaf << 9) | (t + s * kG);
+      x.ol[s] = lab[v];
+      x.oz[s] = zv[v];
+      x.op[s] = k ? pin[v] : V(0);
+      const int h = t + s * kG;
+      x.hz[s] = V(0);
+      x.hp[s] = V(0);
+      x.hix[s] = 0;
+      if (h < 384) {
+        int hx, hy, hz_;
+        halo_cell(h, hx, hy, hz_);
+        const int64_t hv = vox_of(d, o[0] + hx, o[1] + hy, o[2] + hz_);
+        x.hz[s] = zv[hv];
+        x.hp[s] = k ? pin[hv] : V(0);
+        x.hix[s] = tix(hx, hy, hz_);
+      }
+    }
+  };
+  Walk q = walk(nf);
+  Pre cur;
+  if (q.more()) load(q, cur);
+  for (int j = 0; q.more(); ++j) {
+    V *tile = s_p[j & 1][g];
+    if (cur.valid) {
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        tile[tix_of_local(t + s * kG)] = pval(cur.oz[s], cur.op[s]);
+        if (t + s * kG < 384) tile[cur.hix[s]] = pval(cur.hz[s], cur.hp[s]);
+      }
+    }
+    const bool valid = cur.valid;
+    const int64_t leaf = cur.leaf;
+    const uint8_t ol0 = cur.ol[0], ol1 = cur.ol[1];
+    q.next();
+    if (q.more()) load(q, cur);
+    __syncthreads();
+    if (valid) {
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        const uint8_t l = s ? ol1 : ol0;
+        const int c = t + s * kG;
+        const int64_t v = (leaf << 9) | c;
+        const int ti = tix_of_local(c);
+        const V pn = tile[ti];
+        const V ap = is_fluid(l) ? stencil0<V, ORD>(tile, ti, l >> 2) : V(0);
+        pout[v] = pn;
+        Ap[v] = ap;
+        acc += A(pn) * A(ap);
+      }
+    }
+  }
+  A vals[1] = {acc};
+  publish<A, 1>(vals, part_a);
+}
+
+// B(k): ComputeLocal2 (solver.cc:323-346): alpha = rz_k / pAp, x += alpha p,
+// r -= alpha Ap, z = M^-1 r; publishes (r.r, r.z).
+template <typename V, typename A, bool ORD>
+__global__ __launch_bounds__(kT) void k_pass_b(const int32_t *flist, int64_t nf,
+                                               const uint8_t *lab, const V *p, const V *Ap, V *x,
+                                               V *r, V *zv, const A *part_a, A *part_b,
+                                               int g_parts, int k, PcgState *st) {
+#pragma clang fp contract(off)
+  if (st->done) return;
+  const A pAp = A(total<A, 1>(part_a, g_parts, 0));
+  const A rz = A(st->rzh[k & 1]);
+  const A alpha_a = rz / pAp;
+  if (!(double(alpha_a) <= 1e5)) {  // CHECK_LE(alpha, LARGE_FLOAT), solver.cc:333
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
       st->status = FSL_ERR_ALPHA;
       st->done = 1;
     }
+    return;
   }
-}
-
-// Pass B: x += alpha p, r -= alpha Ap, z = M^-1 r, rr, rz; beta = rz/rz_old
-// (ComputeLocal2 + the While test, solver.cc:323-346, solver_driver.cc:330).
-template <typename V, typename A, bool ORD>
-__global__ __launch_bounds__(512) void k_pass_b(const int32_t *flist, int64_t nf,
-                                                const uint8_t *lab, const V *p0, const V *p1,
-                                                const V *Ap, V *x, V *r, A *partials,
-                                                PcgState *st) {
-#pragma clang fp contract(off)
-  if (st->done) return;
-  const int64_t w = xcd_work(nf);
-  if (w >= nf) return;
-  const V alpha = V(st->alpha);
-  const V *p = st->pad ? p0 : p1;  // the buffer pass A wrote
-  const int64_t v = (int64_t(flist[w]) << 9) | threadIdx.x;
-  const uint8_t l = lab[v];
-  A vals[2] = {A(0), A(0)};
-  if (is_fluid(l)) {
-    V xn, rn;
-    if (ORD) {
-      const V m = alpha * p[v];
-      xn = m + x[v];
-      const V m2 = (-alpha) * Ap[v];
-      rn = m2 + r[v];
-    } else {
-      xn = __builtin_fma(alpha, p[v], x[v]);
-      rn = __builtin_fma(-alpha, Ap[v], r[v]);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->pAp = double(pAp);
+    st->alpha = double(alpha_a);
+  }
+  const V alpha = V(alpha_a);
+  A acc[2] = {A(0), A(0)};
+  // Software pipeline: round j+1's loads are issued before round j computes.
+  struct Pre {
+    V p[2], a[2], x[2], r[2];
+    uint8_t l[2];
+    int64_t base;
+    bool valid;
+  };
+  auto load = [&](const Walk &q, Pre &c) {
+    c.valid = q.valid();
+    if (!c.valid) return;
+    c.base = int64_t(flist[q.mine()]) << 9;
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int64_t v = c.base | (gtid() + s * kG);
+      c.l[s] = lab[v];
+      c.p[s] = p[v];
+      c.a[s] = Ap[v];
+      c.x[s] = x[v];
+      c.r[s] = r[v];
     }
-    x[v] = xn;
-    r[v] = rn;
-    const V z = rn * dinv<V>(l >> 2);
-    vals[0] = A(rn) * A(rn);
-    vals[1] = A(rn) * A(z);
+  };
+  Walk q = walk(nf);
+  Pre cur;
+  cur.valid = false;
+  if (q.more()) load(q, cur);
+  while (q.more()) {
+    const Pre c = cur;
+    q.next();
+    cur.valid = false;
+    if (q.more()) load(q, cur);
+    if (!c.valid) continue;
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      if (!is_fluid(c.l[s])) continue;
+      const int64_t v = c.base | (gtid() + s * kG);
+      V xn, rn;
+      if (ORD) {
+        const V m = alpha * c.p[s];
+        xn = m + c.x[s];
+        const V m2 = (-alpha) * c.a[s];
+        rn = m2 + c.r[s];
+      } else {
+        xn = __builtin_fma(alpha, c.p[s], c.x[s]);
+        rn = __builtin_fma(-alpha, c.a[s], c.r[s]);
+      }
+      x[v] = xn;
+      r[v] = rn;
+      const V z = rn * dinv<V>(c.l[s] >> 2);
+      zv[v] = z;
+      acc[0] += A(rn) * A(rn);
+      acc[1] += A(rn) * A(z);
+    }
   }
-  double tot[2];
-  if (reduce_last<A, 2>(vals, partials, w, nf, &st->cnt_b, tot) && threadIdx.x == 0) {
-    const A rz_old = A(st->rz), rz = A(tot[1]);
-    const A beta = rz / rz_old;
-    st->rr = tot[0];
-    st->rz_old = double(rz_old);
-    st->rz = double(rz);
-    st->beta = double(beta);
-    st->it += 1;
-    st->pad ^= 1;
-    if (isnan(tot[0])) st->status = FSL_ERR_NAN;
-    else if (!(double(beta) <= 1e5)) st->status = FSL_ERR_BETA;  // solver.cc:361
-    st->done = (st->status != 0 || converged(st, tot[0]) || st->it >= st->max_iter) ? 1 : 0;
-  }
+  publish<A, 2>(acc, part_b);
 }
 
 // MIXED refresh, part 1: fold the float iterate into the double solution.
-__global__ __launch_bounds__(512) void k_refresh_x(const int32_t *flist, int64_t nf,
-                                                   const uint8_t *lab, float *x, double *x64,
-                                                   const PcgState *st, int force) {
-  if (!force && st->done) return;
-  if (st->status) return;
-  const int64_t w = xcd_work(nf);
-  if (w >= nf) return;
-  const int64_t v = (int64_t(flist[w]) << 9) | threadIdx.x;
-  if (!is_fluid(lab[v])) return;
-  x64[v] += double(x[v]);
-  x[v] = 0.0f;
+__global__ __launch_bounds__(kT) void k_refresh_x(const int32_t *flist, int64_t nf,
+                                                  const uint8_t *lab, float *x, double *x64,
+                                                  const PcgState *st, int force) {
+  if ((!force && st->done) || st->status) return;
+  for (Walk q = walk(nf); q.more(); q.next()) {
+    if (!q.valid()) continue;
+    const int64_t base = int64_t(flist[q.mine()]) << 9;
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int64_t v = base | (gtid() + s * kG);
+      if (!is_fluid(lab[v])) continue;
+      x64[v] += double(x[v]);
+      x[v] = 0.0f;
+    }
+  }
 }
 
-// MIXED refresh, part 2: r = b - A x64 in double, stored as float; rr and rz
-// replace the recursive values and beta is recomputed from the true rz.
-__global__ __launch_bounds__(512) void k_refresh_r(Dom d, const int32_t *flist, int64_t nf,
-                                                   const uint8_t *lab, const float *div,
-                                                   const double *x64, float *r,
-                                                   double *partials, PcgState *st, int force) {
-  __shared__ double s_x[1000];
-  __shared__ uint8_t s_f[1000];
-  if (!force && st->done) return;
-  if (st->status) return;
-  const int64_t w = xcd_work(nf);
-  if (w >= nf) return;
-  const int64_t leaf = flist[w];
-  int o[3];
-  leaf_origin(d, leaf, o);
-  const int t = threadIdx.x, lx = t >> 6, ly = (t >> 3) & 7, lz = t & 7;
-  const int64_t v = (leaf << 9) | t;
-  const uint8_t l = lab[v];
-  const bool f = is_fluid(l);
-  s_x[tix(lx, ly, lz)] = f ? x64[v] : 0.0;
-  s_f[tix(lx, ly, lz)] = f;
-  if (t < 384) {
-    int hx, hy, hz;
-    halo_cell(t, hx, hy, hz);
-    const int64_t hv = vox_of(d, o[0] + hx, o[1] + hy, o[2] + hz);
-    const bool hf = is_fluid(lab[hv]);
-    s_x[tix(hx, hy, hz)] = hf ? x64[hv] : 0.0;
-    s_f[tix(hx, hy, hz)] = hf;
+// MIXED refresh, part 2: r = b - A x64 in double, stored as float; the
+// partials (r.r, r.z) replace those of the last B pass.
+__global__ __launch_bounds__(kT) void k_refresh_r(Dom d, const int32_t *flist, int64_t nf,
+                                                  const uint8_t *lab, const float *div,
+                                                  const double *x64, float *r, float *zv,
+                                                  double *part_b, const PcgState *st, int force) {
+  __shared__ double s_x[kGroups][1000];
+  __shared__ uint8_t s_f[kGroups][1000];
+  if ((!force && st->done) || st->status) return;
+  const int g = threadIdx.x / kG;
+  double acc[2] = {0.0, 0.0};
+  for (Walk q = walk(nf); q.more(); q.next()) {
+    const bool valid = q.valid();
+    const int64_t leaf = valid ? flist[q.mine()] : 0;
+    int o[3];
+    leaf_origin(d, leaf, o);
+    uint8_t ol[2] = {0, 0};
+    if (valid)
+      fill_tile<double>(d, lab, leaf, o, s_x[g], s_f[g], ol,
+                        [&](int64_t v, uint8_t) { return x64[v]; });
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int c = gtid() + s * kG;
+      const int64_t v = (leaf << 9) | c;
+      if (!valid || !is_fluid(ol[s])) continue;
+      const double rt =
+          double(div[v]) - stencil<double, false>(s_x[g], s_f[g], tix_of_local(c), ol[s] >> 2);
+      const float rf = float(rt);
+      const float zf = rf * dinv<float>(ol[s] >> 2);
+      r[v] = rf;
+      zv[v] = zf;
+      acc[0] += double(rf) * double(rf);
+      acc[1] += double(rf) * double(zf);
+    }
+    __syncthreads();
   }
-  __syncthreads();
-  double vals[2] = {0.0, 0.0};
-  if (f) {
-    const double rt = double(div[v]) - stencil<double, false>(s_x, s_f, lx, ly, lz, l >> 2);
-    const float rf = float(rt);
-    r[v] = rf;
-    vals[0] = rt * rt;
-    vals[1] = double(rf) * double(rf * dinv<float>(l >> 2));
-  }
-  double tot[2];
-  if (reduce_last<double, 2>(vals, partials, w, nf, &st->cnt_r, tot) && threadIdx.x == 0) {
-    st->rr_true = tot[0];
-    st->rr = tot[0];
-    st->rz = tot[1];
-    st->beta = st->rz_old != 0.0 ? tot[1] / st->rz_old : 0.0;
-    if (isnan(tot[0])) st->status = FSL_ERR_NAN;
-    st->done = (st->status != 0 || converged(st, tot[0]) || st->it >= st->max_iter) ? 1 : 0;
-  }
+  publish<double, 2>(acc, part_b);
+}
+
+// Stage API: after B(k), the scalars and the stopping test of iteration k+1
+// (the same computation A(k+1) starts with).
+template <typename A>
+__global__ __launch_bounds__(1024) void k_stage_scalars(const A *part_b_new, int g, int k,
+                                                        PcgState *st) {
+  const double rr = total<A, 2>(part_b_new, g, 0);
+  const A rz = A(total<A, 2>(part_b_new, g, 1));
+  if (threadIdx.x != 0 || st->done) return;
+  const A rz_old = A(st->rzh[k & 1]);
+  const A b = rz / rz_old;
+  st->it = k + 1;
+  st->rr = rr;
+  st->rz = double(rz);
+  st->rz_old = double(rz_old);
+  st->beta = double(b);
+  if (isnan(rr)) st->status = FSL_ERR_NAN;
+  else if (!(double(b) <= 1e5)) st->status = FSL_ERR_BETA;
+  st->done = (st->status != 0 || converged(st, rr) || k + 1 >= st->max_iter) ? 1 : 0;
+}
+
+// After a forced refresh: the true r.r decides whether the solve is done.
+__global__ __launch_bounds__(1024) void k_refresh_scalars(const double *part_b, int g,
+                                                          PcgState *st) {
+  const double rr = total<double, 2>(part_b, g, 0);
+  if (threadIdx.x != 0) return;
+  st->rr_true = rr;
+  st->rr = rr;
+  if (isnan(rr)) st->status = FSL_ERR_NAN;
+  st->done = (st->status != 0 || converged(st, rr) || st->it >= st->max_iter) ? 1 : 0;
 }
 
 // CopySolution (solver.cc:93-99): the pressure grid holds the solution on
-// FLUID cells and 0 elsewhere.
+// FLUID cells and 0 elsewhere (all active leaves are written).
 template <typename V, bool MIXED>
-__global__ __launch_bounds__(512) void k_finish(const int32_t *list, int64_t nlist,
-                                                const uint8_t *lab, const V *x,
-                                                const double *x64, float *pres) {
-  const int64_t w = xcd_work(nlist);
-  if (w >= nlist) return;
-  const int64_t v = (int64_t(list[w]) << 9) | threadIdx.x;
-  float out = 0.0f;
-  if (is_fluid(lab[v])) out = MIXED ? float(x64[v] + double(x[v])) : float(x[v]);
-  pres[v] = out;
+__global__ __launch_bounds__(kT) void k_finish(const int32_t *list, int64_t nlist,
+                                               const uint8_t *lab, const V *x, const double *x64,
+                                               float *pres) {
+  for (Walk q = walk(nlist); q.more(); q.next()) {
+    if (!q.valid()) continue;
+    const int64_t base = int64_t(list[q.mine()]) << 9;
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int64_t v = base | (gtid() + s * kG);
+      float out = 0.0f;
+      if (is_fluid(lab[v])) out = MIXED ? float(x64[v] + double(x[v])) : float(x[v]);
+      pres[v] = out;
+    }
+  }
 }
 
 // ||b - A x||^2 in double for the solver's own solution (stats only).
 template <typename V, bool MIXED>
-__global__ __launch_bounds__(512) void k_true_residual(Dom d, const int32_t *flist, int64_t nf,
-                                                       const uint8_t *lab, const float *div,
-                                                       const V *x, const double *x64,
-                                                       double *partials, PcgState *st) {
-  __shared__ double s_x[1000];
-  __shared__ uint8_t s_f[1000];
-  const int64_t w = xcd_work(nf);
-  if (w >= nf) return;
-  const int64_t leaf = flist[w];
-  int o[3];
-  leaf_origin(d, leaf, o);
-  const int t = threadIdx.x, lx = t >> 6, ly = (t >> 3) & 7, lz = t & 7;
-  const int64_t v = (leaf << 9) | t;
-  const uint8_t l = lab[v];
-  const bool f = is_fluid(l);
-  auto sol = [&](int64_t q) { return MIXED ? x64[q] + double(x[q]) : double(x[q]); };
-  s_x[tix(lx, ly, lz)] = f ? sol(v) : 0.0;
-  s_f[tix(lx, ly, lz)] = f;
-  if (t < 384) {
-    int hx, hy, hz;
-    halo_cell(t, hx, hy, hz);
-    const int64_t hv = vox_of(d, o[0] + hx, o[1] + hy, o[2] + hz);
-    const bool hf = is_fluid(lab[hv]);
-    s_x[tix(hx, hy, hz)] = hf ? sol(hv) : 0.0;
-    s_f[tix(hx, hy, hz)] = hf;
+__global__ __launch_bounds__(kT) void k_true_residual(Dom d, const int32_t *flist, int64_t nf,
+                                                      const uint8_t *lab, const float *div,
+                                                      const V *x, const double *x64,
+                                                      double *part) {
+  __shared__ double s_x[kGroups][1000];
+  __shared__ uint8_t s_f[kGroups][1000];
+  const int g = threadIdx.x / kG;
+  double acc = 0.0;
+  for (Walk q = walk(nf); q.more(); q.next()) {
+    const bool valid = q.valid();
+    const int64_t leaf = valid ? flist[q.mine()] : 0;
+    int o[3];
+    leaf_origin(d, leaf, o);
+    uint8_t ol[2] = {0, 0};
+    if (valid)
+      fill_tile<double>(d, lab, leaf, o, s_x[g], s_f[g], ol, [&](int64_t v, uint8_t) {
+        return MIXED ? x64[v] + double(x[v]) : double(x[v]);
+      });
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int c = gtid() + s * kG;
+      const int64_t v = (leaf << 9) | c;
+      if (!valid || !is_fluid(ol[s])) continue;
+      const double rt =
+          double(div[v]) - stencil<double, false>(s_x[g], s_f[g], tix_of_local(c), ol[s] >> 2);
+      acc += rt * rt;
+    }
+    __syncthreads();
   }
-  __syncthreads();
-  double vals[1] = {0.0};
-  if (f) {
-    const double rt = double(div[v]) - stencil<double, false>(s_x, s_f, lx, ly, lz, l >> 2);
-    vals[0] = rt * rt;
-  }
-  double tot[1];
-  if (reduce_last<double, 1>(vals, partials, w, nf, &st->cnt_r, tot) && threadIdx.x == 0)
-    st->rr_true = tot[0];
+  double vals[1] = {acc};
+  publish<double, 1>(vals, part);
+}
+
+__global__ __launch_bounds__(1024) void k_total_scalar(const double *part, int g, PcgState *st) {
+  const double t = total<double, 1>(part, g, 0);
+  if (threadIdx.x == 0) st->rr_true = t;
+}
+
+// Workgroups for a list of n leaves: up to kBlocksPerXcd per XCD.
+inline int grid_for(int64_t n) {
+  const int64_t chunk = (n + kXcd - 1) / kXcd;
+  const int64_t per = (chunk + kGroups - 1) / kGroups;
+  return int(kXcd * std::max<int64_t>(1, std::min<int64_t>(per, kBlocksPerXcd)));
+}
+
+// Partial-sum buffers, carved from ctx->partials (sized for 8 x 1024 x 3 doubles).
+struct Parts {
+  void *a, *b[2], *bb, *r;
+};
+inline Parts parts(fsl_ctx *c) {
+  char *base = static_cast<char *>(c->partials);
+  const size_t slot = size_t(kXcd) * kBlocksPerXcd * 2 * sizeof(double);
+  return {base, {base + slot, base + 2 * slot}, base + 3 * slot, base + 4 * slot};
 }
 
 template <typename V, typename A, bool ORD, bool MIXED>
 struct Kernels {
   static void prepare(fsl_ctx *c, int warm) {
-    k_prepare<V, A, ORD, MIXED><<<xcd_grid(c->n_fluid), 512, 0, c->stream>>>(
+    const int g = grid_for(c->n_fluid);
+    Parts P = parts(c);
+    k_zero<V><<<grid_for(c->n_act), kT, 0, c->stream>>>(c->act_list, c->n_act, (V *)c->z,
+                                                         (V *)c->p[0], (V *)c->p[1]);
+    k_prepare<V, A, ORD, MIXED><<<g, kT, 0, c->stream>>>(
         c->dom, c->fluid_list, c->n_fluid, c->lab, c->div, c->pres, warm, (V *)c->x, (V *)c->r,
-        c->x64, (A *)c->partials, c->pcg);
+        (V *)c->z, c->x64, (A *)P.b[1], (A *)P.bb);
+    k_prepare_scalars<A><<<1, 1024, 0, c->stream>>>((const A *)P.b[1], (const A *)P.bb, g, c->pcg);
   }
-  static void pass_a(fsl_ctx *c) {
-    k_pass_a<V, A, ORD><<<xcd_grid(c->n_fluid), 512, 0, c->stream>>>(
-        c->dom, c->fluid_list, c->n_fluid, c->lab, (const V *)c->r, (V *)c->p[0], (V *)c->p[1],
-        (V *)c->Ap, (A *)c->partials, c->pcg);
+  // With events, hipExtLaunchKernelGGL records the dispatch's own start and
+  // end (the kernel duration, as a profiler reports it).
+  static void pass_a(fsl_ctx *c, int k, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+    const int g = grid_for(c->n_fluid);
+    Parts P = parts(c);
+    hipExtLaunchKernelGGL(k_pass_a<V, A, ORD>, dim3(g), dim3(kT), 0, c->stream, e0, e1, 0,
+                          c->dom, (const int32_t *)c->fluid_list, c->n_fluid,
+                          (const uint8_t *)c->lab, (const V *)c->z,
+                          (const V *)c->p[(k + 1) & 1], (V *)c->p[k & 1], (V *)c->Ap,
+                          (const A *)P.b[(k + 1) & 1], (A *)P.a, g, k, c->pcg);
   }
-  static void pass_b(fsl_ctx *c) {
-    k_pass_b<V, A, ORD><<<xcd_grid(c->n_fluid), 512, 0, c->stream>>>(
-        c->fluid_list, c->n_fluid, c->lab, (const V *)c->p[0], (const V *)c->p[1],
-        (const V *)c->Ap, (V *)c->x, (V *)c->r, (A *)c->partials, c->pcg);
+  static void pass_b(fsl_ctx *c, int k, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+    const int g = grid_for(c->n_fluid);
+    Parts P = parts(c);
+    hipExtLaunchKernelGGL(k_pass_b<V, A, ORD>, dim3(g), dim3(kT), 0, c->stream, e0, e1, 0,
+                          (const int32_t *)c->fluid_list, c->n_fluid, (const uint8_t *)c->lab,
+                          (const V *)c->p[k & 1], (const V *)c->Ap, (V *)c->x, (V *)c->r,
+                          (V *)c->z, (const A *)P.a, (A *)P.b[k & 1], g, k, c->pcg);
   }
-  static void iterate(fsl_ctx *c) {
-    pass_a(c);
-    pass_b(c);
+  static void stage_scalars(fsl_ctx *c, int k) {
+    const int g = grid_for(c->n_fluid);
+    Parts P = parts(c);
+    k_stage_scalars<A><<<1, 1024, 0, c->stream>>>((const A *)P.b[k & 1], g, k, c->pcg);
   }
   static void finish(fsl_ctx *c) {
-    k_finish<V, MIXED><<<xcd_grid(c->n_act), 512, 0, c->stream>>>(
+    k_finish<V, MIXED><<<grid_for(c->n_act), kT, 0, c->stream>>>(
         c->act_list, c->n_act, c->lab, (const V *)c->x, c->x64, c->pres);
   }
   static void true_residual(fsl_ctx *c) {
-    k_true_residual<V, MIXED><<<xcd_grid(c->n_fluid), 512, 0, c->stream>>>(
-        c->dom, c->fluid_list, c->n_fluid, c->lab, c->div, (const V *)c->x, c->x64,
-        (double *)c->partials, c->pcg);
+    const int g = grid_for(c->n_fluid);
+    Parts P = parts(c);
+    k_true_residual<V, MIXED><<<g, kT, 0, c->stream>>>(c->dom, c->fluid_list, c->n_fluid, c->lab,
+                                                      c->div, (const V *)c->x, c->x64,
+                                                      (double *)P.r);
+    k_total_scalar<<<1, 1024, 0, c->stream>>>((const double *)P.r, g, c->pcg);
   }
 };
 
-void launch_refresh(fsl_ctx *c, int force) {
-  k_refresh_x<<<xcd_grid(c->n_fluid), 512, 0, c->stream>>>(c->fluid_list, c->n_fluid, c->lab,
-                                                            (float *)c->x, c->x64, c->pcg, force);
-  k_refresh_r<<<xcd_grid(c->n_fluid), 512, 0, c->stream>>>(
-      c->dom, c->fluid_list, c->n_fluid, c->lab, c->div, c->x64, (float *)c->r,
-      (double *)c->partials, c->pcg, force);
+// The refresh after B(k) overwrites B(k)'s partials.
+void launch_refresh(fsl_ctx *c, int k, int force) {
+  const int g = grid_for(c->n_fluid);
+  Parts P = parts(c);
+  k_refresh_x<<<g, kT, 0, c->stream>>>(c->fluid_list, c->n_fluid, c->lab, (float *)c->x, c->x64,
+                                       c->pcg, force);
+  k_refresh_r<<<g, kT, 0, c->stream>>>(c->dom, c->fluid_list, c->n_fluid, c->lab, c->div,
+                                       c->x64, (float *)c->r, (float *)c->z, (double *)P.b[k & 1],
+                                       c->pcg, force);
+  if (force) k_refresh_scalars<<<1, 1024, 0, c->stream>>>((const double *)P.b[k & 1], g, c->pcg);
 }
 
 template <typename F>
@@ -477,7 +745,16 @@ int sync_state(fsl_ctx *ctx) {
   return FSL_OK;
 }
 
+int clear_done(fsl_ctx *ctx) {
+  ctx->pcg_host->done = 0;
+  FSL_HIP(hipMemcpyAsync(&ctx->pcg->done, &ctx->pcg_host->done, sizeof(int),
+                         hipMemcpyHostToDevice, ctx->stream));
+  return FSL_OK;
+}
+
 }  // namespace
+
+size_t pcg_partials_bytes() { return size_t(5) * kXcd * kBlocksPerXcd * 2 * sizeof(double); }
 
 int pcg_prepare(fsl_ctx *ctx, const fsl_solve_params *prm) {
   if (prm->precision < 0 || prm->precision > 2 || prm->stop_mode < 0 || prm->stop_mode > 1) {
@@ -496,6 +773,7 @@ int pcg_prepare(fsl_ctx *ctx, const fsl_solve_params *prm) {
   h->thr = prm->precision == FSL_PCG_FP32 && prm->stop_mode == FSL_STOP_ABS_RR
                ? double(float(prm->threshold))
                : prm->threshold;
+  if (ctx->n_fluid == 0) h->done = 1;
   FSL_HIP(hipMemcpyAsync(ctx->pcg, h, sizeof(PcgState), hipMemcpyHostToDevice, ctx->stream));
   if (ctx->n_fluid > 0) {
     dispatch(ctx->pcg_precision,
@@ -504,25 +782,23 @@ int pcg_prepare(fsl_ctx *ctx, const fsl_solve_params *prm) {
   }
   int rc = sync_state(ctx);
   if (rc) return rc;
-  if (ctx->n_fluid == 0) {
-    ctx->pcg_host->done = 1;
-    FSL_HIP(hipMemcpyAsync(ctx->pcg, ctx->pcg_host, sizeof(PcgState), hipMemcpyHostToDevice,
-                           ctx->stream));
-  }
   ctx->pcg_ready = true;
   return ctx->pcg_host->status;
 }
 
+// One While-loop body, launched and synchronised on its own (stage API).
 int pcg_iterate_once(fsl_ctx *ctx) {
   if (!ctx->pcg_ready) { ctx->err = "pcg not prepared"; return FSL_ERR_STATE; }
-  if (ctx->n_fluid == 0) return FSL_OK;
-  dispatch(ctx->pcg_precision, [&](auto k) { decltype(k)::iterate(ctx); });
+  if (ctx->n_fluid == 0 || ctx->pcg_host->done) return ctx->pcg_host->status;
+  const int k = ctx->pcg_host->it;
+  dispatch(ctx->pcg_precision, [&](auto K) {
+    decltype(K)::pass_a(ctx, k);
+    decltype(K)::pass_b(ctx, k);
+  });
+  if (ctx->pcg_precision == FSL_PCG_MIXED && (k + 1) % ctx->pcg_refresh == 0)
+    launch_refresh(ctx, k, 0);
+  dispatch(ctx->pcg_precision, [&](auto K) { decltype(K)::stage_scalars(ctx, k); });
   FSL_HIP(hipGetLastError());
-  if (ctx->pcg_precision == FSL_PCG_MIXED && ctx->pcg_host->it + 1 > 0 &&
-      (ctx->pcg_host->it + 1) % ctx->pcg_refresh == 0) {
-    launch_refresh(ctx, 0);
-    FSL_HIP(hipGetLastError());
-  }
   int rc = sync_state(ctx);
   if (rc) return rc;
   return ctx->pcg_host->status;
@@ -536,47 +812,65 @@ int pcg_run(fsl_ctx *ctx, const fsl_solve_params *prm, fsl_solve_stats *st) {
   FSL_HIP(hipEventCreate(&e0));
   FSL_HIP(hipEventCreate(&e1));
   FSL_HIP(hipEventRecord(e0, ctx->stream));
-  int it_host = ctx->pcg_host->it;
   const bool mixed = ctx->pcg_precision == FSL_PCG_MIXED;
-  // With the measurement hook on, every pass is bracketed by events (at most
-  // 32 iterations per batch, three events per iteration).
-  const int nb = ctx->timing ? std::min(batch, 32) : batch;
+  // With the measurement hook on, each pass records its own start and end
+  // events (at most 50 iterations per batch, four events per iteration).
+  const int nb = ctx->timing ? std::min(batch, 50) : batch;
   hipEvent_t *pev = ctx->tev + 32;
+  int k = 0;  // next iteration to launch
   while (!ctx->pcg_host->done) {
-    const int it_before = ctx->pcg_host->it;
-    for (int b = 0; b < nb; ++b) {
-      if (ctx->timing) FSL_HIP(hipEventRecord(pev[3 * b], ctx->stream));
-      dispatch(ctx->pcg_precision, [&](auto k) { decltype(k)::pass_a(ctx); });
-      if (ctx->timing) FSL_HIP(hipEventRecord(pev[3 * b + 1], ctx->stream));
-      dispatch(ctx->pcg_precision, [&](auto k) { decltype(k)::pass_b(ctx); });
-      if (ctx->timing) FSL_HIP(hipEventRecord(pev[3 * b + 2], ctx->stream));
-      ++it_host;
-      if (mixed && it_host % ctx->pcg_refresh == 0) launch_refresh(ctx, 0);
+    const int k0 = k;
+    for (int b = 0; b < nb; ++b, ++k) {
+      hipEvent_t *e = pev + 4 * b;
+      const bool t = ctx->timing;
+      dispatch(ctx->pcg_precision, [&](auto K) {
+        decltype(K)::pass_a(ctx, k, t ? e[0] : nullptr, t ? e[1] : nullptr);
+      });
+      dispatch(ctx->pcg_precision, [&](auto K) {
+        decltype(K)::pass_b(ctx, k, t ? e[2] : nullptr, t ? e[3] : nullptr);
+      });
+      if (mixed && (k + 1) % ctx->pcg_refresh == 0) launch_refresh(ctx, k, 0);
     }
+    // The stopping test of iteration k runs at the start of A(k).
+    dispatch(ctx->pcg_precision, [&](auto K) { decltype(K)::pass_a(ctx, k); });
     FSL_HIP(hipGetLastError());
     rc = sync_state(ctx);
     if (rc) return rc;
+    const PcgState *h = ctx->pcg_host;
     if (ctx->timing) {
-      const int worked = std::min(ctx->pcg_host->it - it_before, nb);
+      // A(j) and B(j) did work for j < it (done) or j < k (running).
+      const int worked = std::min((h->done ? h->it : k) - k0, nb);
       for (int b = 0; b < worked; ++b) {
-        float a = 0.0f, bb = 0.0f;
-        FSL_HIP(hipEventElapsedTime(&a, pev[3 * b], pev[3 * b + 1]));
-        FSL_HIP(hipEventElapsedTime(&bb, pev[3 * b + 1], pev[3 * b + 2]));
-        ctx->tacc[FSL_T_PASS_A] += a;
-        ctx->tacc[FSL_T_PASS_B] += bb;
+        hipEvent_t *e = pev + 4 * b;
+        float ms[2] = {0.0f, 0.0f};
+        FSL_HIP(hipEventElapsedTime(&ms[0], e[0], e[1]));
+        FSL_HIP(hipEventElapsedTime(&ms[1], e[2], e[3]));
+        ctx->tacc[FSL_T_PASS_A] += ms[0];
+        ctx->tacc[FSL_T_PASS_B] += ms[1];
         ctx->tacc[FSL_T_PASS_A_N] += 1;
         ctx->tacc[FSL_T_PASS_B_N] += 1;
       }
     }
-    it_host = ctx->pcg_host->it;
-    if (ctx->pcg_host->status) break;
-    if (ctx->pcg_host->done && mixed) {
-      // Converged on the recursive residual: confirm on the true residual,
-      // and continue if it does not hold yet.
-      launch_refresh(ctx, 1);
+    if (h->status) break;
+    if (!h->done) {
+      // A(k) ran for real: the next batch starts at B(k).
+      dispatch(ctx->pcg_precision, [&](auto K) { decltype(K)::pass_b(ctx, k); });
+      if (mixed && (k + 1) % ctx->pcg_refresh == 0) launch_refresh(ctx, k, 0);
+      ++k;
+      continue;
+    }
+    if (mixed && h->it > 0 && h->it < h->max_iter) {
+      // Converged on the recursive residual: confirm on the true residual
+      // (refresh after B(it-1)), and continue from A(it) if it does not hold.
+      const int last = h->it;
+      launch_refresh(ctx, last - 1, 1);
       FSL_HIP(hipGetLastError());
       rc = sync_state(ctx);
       if (rc) return rc;
+      if (!ctx->pcg_host->done) {
+        ctx->pcg_host->it = last;
+        k = last;
+      }
     }
   }
   FSL_HIP(hipEventRecord(e1, ctx->stream));

@@ -28,7 +28,7 @@ GRID_VELOCITY, GRID_VELOCITY_UPDATE = 0, 1
 GRID_DIVERGENCE, GRID_PRESSURE = 0, 1
 GRID_MARKER, GRID_SOLID, GRID_ROW_INDEX, GRID_CELL_COUNT = 0, 1, 2, 3
 TIMING_NAMES = ("move", "clear", "p2g", "bc_save", "gravity_bc", "divergence", "solve",
-                "update", "g2p", "pass_a", "pass_b", "pass_a_n", "pass_b_n", "steps")
+                "update", "g2p", "pass_a", "pass_b", "pass_a_n", "pass_b_n", "reduce", "steps")
 
 
 class SolveParams(C.Structure):

@@ -194,8 +194,9 @@ int fsl_pcg_finish(fsl_ctx *ctx, fsl_solve_stats *stats);
 #define FSL_T_PASS_B 10   /* PCG pass B (axpys + r.r, r.z), summed */
 #define FSL_T_PASS_A_N 11 /* pass A launches that did work */
 #define FSL_T_PASS_B_N 12
-#define FSL_T_STEPS 13    /* fsl_step calls timed */
-#define FSL_T_COUNT 14
+#define FSL_T_REDUCE 13   /* PCG scalar reductions (one workgroup each), summed */
+#define FSL_T_STEPS 14    /* fsl_step calls timed */
+#define FSL_T_COUNT 15
 int fsl_set_timing(fsl_ctx *ctx, int enable);
 int fsl_get_timing(fsl_ctx *ctx, double *out, int n);
 
