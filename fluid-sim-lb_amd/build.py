@@ -27,8 +27,10 @@ FLAGS = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-result",
 ]
 
-SOURCES = ["scan.hip", "particles.hip", "grid.hip", "pcg.hip", "adjoint.hip", "api.hip",
-           "host_classes.cpp"]
+SOURCES = ["scan.hip", "particles.hip", "grid.hip", "pcg.hip", "adjoint.hip", "api.hip"]
+# The water application: fsl.hpp's host classes driving libfsl.so.
+APP_SRC = os.path.join(HERE, "app", "water_app.cpp")
+APP = os.path.join(HERE, "fsl_water_app")
 
 
 def _compile(src: str) -> str:
@@ -54,9 +56,21 @@ def build(jobs: int | None = None, verbose: bool = True) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_app()
     if verbose:
         print(f"built {LIB}")
     return LIB
+
+
+def build_app() -> str:
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(ROOT, "include"), APP_SRC,
+           "-L" + HERE, "-lfsl", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib",
+           "-o", APP + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"app build failed\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(APP + ".tmp", APP)
+    return APP
 
 
 if __name__ == "__main__":

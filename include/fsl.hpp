@@ -1,0 +1,414 @@
+/*
+ * fsl.hpp - C++ host classes over the C ABI in fsl.h.
+ *
+ * These classes keep the call surface of the reference's FLIP step
+ * (schinmayee/fluid-sim-lb):
+ *   fsl::FlipSimulation  <- application::FlipSimulation<3,3,float>
+ *                           (common/flip_simulation.h:39-187)
+ *   fsl::PCGDriver       <- common::PCGDriver<ScalarGrid<3,3,float>>
+ *                           (common/solver_driver.h:30-106)
+ *   fsl::FlipDriver      <- FlipDriver's step loop (common/flip_app_sim.cc:211-254,
+ *                           357-462), run in order in one process: the Canary
+ *                           transforms of the reference become plain calls.
+ *
+ * Differences from the reference, all forced by the device-resident design:
+ *  - The grids and particles live in the context on the GPU; the reference's
+ *    set_grid_*() / grid_*() pointer plumbing has no counterpart, and the
+ *    grids are read and written through Get/Set calls in the box format of
+ *    fsl.h.
+ *  - The reference aborts on a failed CHECK; these classes throw fsl::Error
+ *    with the library's message and status code instead.
+ *  - Particle seeding uses Philox4x32-10 keyed by (seed, cell, slot) on cell
+ *    centres inside the shape, instead of glibc random() and a level set.
+ *  - DeleteParticlesInSolids runs inside MoveParticles (one sort per step).
+ */
+#ifndef FSL_HPP_
+#define FSL_HPP_
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "fsl.h"
+
+namespace fsl {
+
+class Error : public std::runtime_error {
+ public:
+  Error(int code, const std::string &what) : std::runtime_error(what), code_(code) {}
+  int code() const { return code_; }
+
+ private:
+  int code_;
+};
+
+struct Coord {
+  int x = 0, y = 0, z = 0;
+  Coord() = default;
+  Coord(int a, int b, int c) : x(a), y(b), z(c) {}
+  int operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+};
+
+struct Vec3 {
+  float x = 0, y = 0, z = 0;
+  Vec3() = default;
+  Vec3(float a, float b, float c) : x(a), y(b), z(c) {}
+};
+
+/* Inclusive integer box, as common::CoordBBox. */
+struct CoordBBox {
+  Coord lo, hi;
+};
+
+/* Cube with inclusive bounds in voxel units (common/primitives.cc:56-64). */
+struct Cube {
+  Vec3 start, end;
+};
+
+enum PreconditionerType { NO_PRECONDITIONER = 0, JACOBI = 1, INCOMPLETE_CHOLESKY = 2 };
+
+class FlipSimulation {
+ public:
+  FlipSimulation() = default;
+  FlipSimulation(const FlipSimulation &) = delete;
+  FlipSimulation &operator=(const FlipSimulation &) = delete;
+  ~FlipSimulation() {
+    if (ctx_) fsl_destroy(ctx_);
+  }
+
+  /* FlipSimulation::InitializeMetadata (flip_simulation.cc:83-108). rank is
+   * kept for the interface; one context holds every partition as a tile.
+   * gravity is the scaled gravity (flip_app_sim.cc:74). */
+  void InitializeMetadata(Coord global_dims, Coord partitions, int rank = 0,
+                          float voxel_len = 1, int num_particles_per_cell = 8,
+                          float gravity = 9.8f, float flip_factor = 1,
+                          std::string output_dir = "output", bool debug_flag = false) {
+    (void)output_dir;
+    (void)debug_flag;
+    if (std::fabs(voxel_len - 1.0f) > 1e-2f)
+      throw Error(FSL_ERR_ARG, "voxel_len must be 1 (flip_simulation.cc:99)");
+    dims_ = global_dims;
+    partitions_ = partitions;
+    rank_ = rank;
+    ppc_ = num_particles_per_cell;
+    gravity_ = gravity;
+    flip_ = flip_factor;
+    force_ = Vec3(0, 1, 0);
+  }
+
+  /* FlipSimulation::InitializeVariables (flip_simulation.cc:110-122): creates
+   * the device context. capacity = most particles the context may hold
+   * (0: ppc x cells / 4, at least 1M). */
+  void InitializeVariables(int device = 0, int64_t capacity = 0) {
+    const int d[3] = {dims_.x, dims_.y, dims_.z};
+    const int t[3] = {partitions_.x, partitions_.y, partitions_.z};
+    if (capacity <= 0) {
+      capacity = int64_t(ppc_) * d[0] * d[1] * d[2] / 4;
+      if (capacity < (1 << 20)) capacity = 1 << 20;
+    }
+    check(fsl_create(&ctx_, d, t, device, capacity));
+    mask_.assign(size_t(d[0]) * d[1] * d[2], 0);
+    push_params();
+  }
+
+  /* InitializeWaterCuboid / InitializeWaterSphere / InitializeReservoir /
+   * InitializeMultipleWaterDrops (flip_simulation.cc:124-188): each call adds
+   * the shape's cells (cell-centre test) to the fluid mask and reseeds all
+   * particles with `velocity`. */
+  void InitializeWaterCuboid(const Cube &box, const Vec3 velocity) {
+    add_shape([&](float x, float y, float z) {
+      return x >= box.start.x && x <= box.end.x && y >= box.start.y && y <= box.end.y &&
+             z >= box.start.z && z <= box.end.z;
+    }, velocity);
+  }
+  void InitializeWaterSphere(const Vec3 &center, const float radius, const Vec3 velocity) {
+    add_shape([&](float x, float y, float z) {
+      const float dx = x - center.x, dy = y - center.y, dz = z - center.z;
+      return dx * dx + dy * dy + dz * dz < radius * radius;
+    }, velocity);
+  }
+  void InitializeReservoir(float height) {
+    add_shape([&](float, float y, float) { return y < height; }, Vec3());
+  }
+  void InitializeMultipleWaterDrops(float height, const std::vector<Vec3> &centers,
+                                    const std::vector<float> &radii) {
+    InitializeReservoir(height);
+    for (size_t i = 0; i < centers.size() && i < radii.size(); ++i)
+      InitializeWaterSphere(centers[i], radii[i], Vec3());
+  }
+  /* Seed for the Philox particle placement (no reference counterpart). */
+  void set_seed(uint32_t seed) { seed_ = seed; }
+
+  /* SetForces (flip_simulation.cc:248-253): force = f * gravity. */
+  void SetForces(const Vec3 &f) {
+    force_ = f;
+    push_params();
+  }
+  void SetBoxBoundary() { check(fsl_set_box_boundary(ctx_, 0)); }
+  void SetTankBoundary() { check(fsl_set_box_boundary(ctx_, 1)); }
+  void AddSolidCuboid(const CoordBBox &box) {
+    const int lo[3] = {box.lo.x, box.lo.y, box.lo.z}, hi[3] = {box.hi.x, box.hi.y, box.hi.z};
+    check(fsl_add_solid_box(ctx_, lo, hi));
+  }
+
+  void ClearGridData() { check(fsl_clear(ctx_)); }
+  size_t FluidCellCount() {
+    std::vector<int32_t> m(box_cells());
+    check(fsl_get_int(ctx_, FSL_GRID_MARKER, m.data()));
+    size_t n = 0;
+    for (int32_t v : m) n += v == 1;
+    return n;
+  }
+  size_t ParticleCount() {
+    int64_t n = 0;
+    check(fsl_num_particles(ctx_, &n));
+    return size_t(n);
+  }
+
+  /* CFL (flip_simulation.cc:459-488). */
+  float CFL() {
+    float s = 0;
+    check(fsl_cfl(ctx_, &s));
+    return s;
+  }
+  void SaveGridVelocities() { check(fsl_save_velocities(ctx_)); }
+  /* MoveParticles (flip_simulation.cc:498-503) + DeleteParticlesInSolids. */
+  void MoveParticles(float dt, bool defragment = true) {
+    (void)defragment;
+    check(fsl_move_particles(ctx_, dt));
+  }
+  void AddGravity(float dt) { check(fsl_add_gravity(ctx_, dt)); }
+  void ApplyBoundaryConditions() { check(fsl_apply_bc(ctx_)); }
+  void ComputeDivergence() { check(fsl_divergence(ctx_)); }
+  void UpdateVelocityFromPressure() { check(fsl_pressure_update(ctx_)); }
+  void GetVelocityUpdate() { check(fsl_velocity_update(ctx_)); }
+  void ExtendGridVelocityIntoBoundaryRegion() { check(fsl_extrapolate(ctx_)); }
+  void TransferParticlesToGrid() { check(fsl_p2g(ctx_)); }
+  /* UpdateParticlesFromGrid (flip_simulation.cc:1024-1189); extrapolates first. */
+  void UpdateParticlesFromGrid() { check(fsl_g2p(ctx_)); }
+  /* Runs inside MoveParticles here; kept for the call sequence. */
+  void DeleteParticlesInSolids() {}
+  void DeleteOutsideParticles() { check(fsl_delete_outside(ctx_)); }
+
+  /* Grid access (box format of fsl.h). */
+  std::vector<float> GetVelocity(int which, std::vector<uint8_t> *active = nullptr) {
+    std::vector<float> v(box_cells() * 3);
+    if (active) active->resize(box_cells());
+    check(fsl_get_vec(ctx_, which, v.data(), active ? active->data() : nullptr));
+    return v;
+  }
+  void SetVelocity(int which, const std::vector<float> &v, const std::vector<uint8_t> &active) {
+    if (v.size() != box_cells() * 3 || active.size() != box_cells())
+      throw Error(FSL_ERR_ARG, "velocity grid size");
+    check(fsl_set_vec(ctx_, which, v.data(), active.data()));
+  }
+  std::vector<float> GetScalar(int which) {
+    std::vector<float> s(box_cells());
+    check(fsl_get_scalar(ctx_, which, s.data()));
+    return s;
+  }
+  std::vector<int32_t> GetInt(int which) {
+    std::vector<int32_t> s(box_cells());
+    check(fsl_get_int(ctx_, which, s.data()));
+    return s;
+  }
+  void GetParticles(std::vector<float> *pos, std::vector<float> *vel, std::vector<float> *gv) {
+    const size_t n = ParticleCount();
+    if (pos) pos->resize(n * 3);
+    if (vel) vel->resize(n * 3);
+    if (gv) gv->resize(n * 3);
+    check(fsl_get_particles(ctx_, pos ? pos->data() : nullptr, vel ? vel->data() : nullptr,
+                            gv ? gv->data() : nullptr));
+  }
+
+  /* Particles::Write (particles.cc:166-200), one partition covering the
+   * domain: start, end (inclusive), count, then per particle its running
+   * index, position and velocity; binary (int, float[3], float[3]) or text
+   * ("%d : %f %f %f, %f %f %f"). Particles are written in device order
+   * (sorted by cell). */
+  void WriteParticles(const std::string &file_name, bool text = false) {
+    std::vector<float> pos, vel;
+    GetParticles(&pos, &vel, nullptr);
+    FILE *fp = std::fopen(file_name.c_str(), text ? "wt" : "wb");
+    if (!fp) throw Error(FSL_ERR_ARG, "could not create " + file_name);
+    const int start[3] = {0, 0, 0}, end[3] = {dims_.x - 1, dims_.y - 1, dims_.z - 1};
+    const int count = int(pos.size() / 3);
+    if (text) {
+      std::fprintf(fp, "%d %d %d\n%d %d %d\n%d\n", start[0], start[1], start[2], end[0], end[1],
+                   end[2], count);
+      for (int p = 0; p < count; ++p)
+        std::fprintf(fp, "%d : %f %f %f, %f %f %f\n", p, pos[3 * p], pos[3 * p + 1],
+                     pos[3 * p + 2], vel[3 * p], vel[3 * p + 1], vel[3 * p + 2]);
+    } else {
+      std::fwrite(start, sizeof(int), 3, fp);
+      std::fwrite(end, sizeof(int), 3, fp);
+      std::fwrite(&count, sizeof(int), 1, fp);
+      for (int p = 0; p < count; ++p) {
+        std::fwrite(&p, sizeof(int), 1, fp);
+        std::fwrite(&pos[3 * p], sizeof(float), 3, fp);
+        std::fwrite(&vel[3 * p], sizeof(float), 3, fp);
+      }
+    }
+    std::fclose(fp);
+  }
+
+  Coord global_dims() const { return dims_; }
+  Coord partitions() const { return partitions_; }
+  float flip_factor() const { return flip_; }
+  fsl_ctx *context() { return ctx_; }
+
+  void check(int rc) const {
+    if (rc != FSL_OK) throw Error(rc, ctx_ ? fsl_last_error(ctx_) : "fsl: no context");
+  }
+
+ private:
+  size_t box_cells() const {
+    return size_t(dims_.x + 2) * (dims_.y + 2) * (dims_.z + 2);
+  }
+  void push_params() {
+    if (!ctx_) return;
+    const float f[3] = {force_.x * gravity_, force_.y * gravity_, force_.z * gravity_};
+    check(fsl_set_params(ctx_, gravity_, f, flip_));
+  }
+  template <typename Inside>
+  void add_shape(Inside inside, const Vec3 &velocity) {
+    if (!ctx_) throw Error(FSL_ERR_STATE, "InitializeVariables first");
+    for (int i = 0; i < dims_.x; ++i)
+      for (int j = 0; j < dims_.y; ++j)
+        for (int k = 0; k < dims_.z; ++k)
+          if (inside(i + 0.5f, j + 0.5f, k + 0.5f))
+            mask_[(size_t(i) * dims_.y + j) * dims_.z + k] = 1;
+    const float v[3] = {velocity.x, velocity.y, velocity.z};
+    check(fsl_seed_particles(ctx_, mask_.data(), ppc_, seed_, v));
+  }
+
+  fsl_ctx *ctx_ = nullptr;
+  Coord dims_, partitions_{1, 1, 1};
+  int rank_ = 0, ppc_ = 8;
+  float gravity_ = 9.8f, flip_ = 1.0f;
+  Vec3 force_{0, 1, 0};
+  uint32_t seed_ = 1;
+  std::vector<uint8_t> mask_;
+};
+
+/* PCGDriver (solver_driver.h:30-106): Jacobi PCG on the FLUID cells of the
+ * simulation's marker grid, b = divergence grid, x = pressure grid. */
+class PCGDriver {
+ public:
+  PCGDriver(std::string name, int max_iterations, float threshold, bool fixed_iterations,
+            PreconditionerType preconditioner_type, Coord partitions, Coord global_dims)
+      : name_(std::move(name)), max_iterations_(max_iterations), threshold_(threshold),
+        fixed_iterations_(fixed_iterations), partitions_(partitions), global_dims_(global_dims) {
+    (void)preconditioner_type;  // the reference runs Jacobi (solver.cc:103-106)
+    params_.precision = FSL_PCG_FP32;
+    params_.stop_mode = FSL_STOP_ABS_RR;
+  }
+
+  /* SetVariables(b_grid, marker, x_grid) (solver_driver.h:89-91): the three
+   * grids belong to the simulation's context. */
+  void SetVariables(FlipSimulation *simulation) { sim_ = simulation; }
+
+  /* Solve (solver_driver.cc:13-69): the reference's stopping rule is
+   * r.r < threshold (FP32); fixed_iterations runs exactly max_iterations. */
+  void Solve() {
+    if (!sim_) throw Error(FSL_ERR_STATE, name_ + ": SetVariables first");
+    params_.threshold = fixed_iterations_ ? 0.0 : double(threshold_);
+    params_.max_iterations = max_iterations_;
+    sim_->check(fsl_pcg_solve(sim_->context(), &params_, &stats_));
+  }
+
+  /* Precision / stopping rule beyond the reference's (FSL_PCG_*, FSL_STOP_*). */
+  void set_solve_params(const fsl_solve_params &p) {
+    params_ = p;
+    threshold_ = float(p.threshold);
+  }
+  const fsl_solve_stats &stats() const { return stats_; }
+  int num_iterations() const { return stats_.iterations; }
+  const std::string &name() const { return name_; }
+
+ private:
+  std::string name_;
+  int max_iterations_;
+  float threshold_;
+  bool fixed_iterations_;
+  Coord partitions_, global_dims_;
+  FlipSimulation *sim_ = nullptr;
+  fsl_solve_params params_{};
+  fsl_solve_stats stats_{};
+};
+
+/* FlipDriver's time loop (flip_app_sim.cc:211-254, 357-462) as plain calls. */
+class FlipDriver {
+ public:
+  struct Params {
+    int frames = 1;
+    int frame_rate = 30;
+    int cfl = 1;
+    int max_iterations = 1500;  // flip_app.h:52
+  };
+
+  FlipDriver(FlipSimulation *simulation, PCGDriver *solver, Params params)
+      : sim_(simulation), solver_(solver), params_(params) {
+    frame_step_ = 1.0f / float(params_.frame_rate);
+    solver_->SetVariables(sim_);
+  }
+
+  /* WhileAndComputeDt's dt for the current step (flip_app_sim.cc:371-410). */
+  float ComputeDt() {
+    float dt = 0;
+    sim_->check(fsl_compute_dt(sim_->context(), params_.cfl, frame_step_, &step_, &dt));
+    return dt;
+  }
+
+  /* RunOneStep (flip_app_sim.cc:211-254), sources and level-set stages
+   * excluded (not part of this core). */
+  void RunOneStep(float dt) {
+    sim_->MoveParticles(dt);
+    sim_->DeleteParticlesInSolids();
+    sim_->ClearGridData();
+    sim_->TransferParticlesToGrid();
+    sim_->ApplyBoundaryConditions();
+    sim_->SaveGridVelocities();
+    sim_->DeleteOutsideParticles();
+    sim_->AddGravity(dt);
+    sim_->ApplyBoundaryConditions();
+    sim_->ComputeDivergence();
+    solver_->Solve();
+    sim_->UpdateVelocityFromPressure();
+    sim_->GetVelocityUpdate();
+    sim_->UpdateParticlesFromGrid();
+    ++steps_;
+  }
+
+  /* One frame: steps until the frame's time is used up (the While loop of
+   * flip_app_sim.cc:357-369), then ResetDt and AdvanceTime. */
+  void NextFrame() {
+    while (step_ + 1e-5f < frame_step_) RunOneStep(ComputeDt());
+    step_ = 0;
+    time_ += frame_step_;
+    ++frame_;
+  }
+  void Run() {
+    for (int f = 0; f < params_.frames; ++f) NextFrame();
+  }
+
+  int steps() const { return steps_; }
+  int frame() const { return frame_; }
+  float time() const { return time_; }
+
+ private:
+  FlipSimulation *sim_;
+  PCGDriver *solver_;
+  Params params_;
+  float frame_step_ = 1.0f / 30;
+  float step_ = 0;
+  float time_ = 0;
+  int steps_ = 0, frame_ = 0;
+};
+
+}  // namespace fsl
+
+#endif  // FSL_HPP_
