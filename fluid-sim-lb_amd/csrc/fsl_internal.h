@@ -21,6 +21,7 @@
 
 namespace fsl {
 
+struct AdjState;  // adjoint buffers and rollout checkpoints (adjoint.hip)
 constexpr int kLeaf = 512;           // voxels per leaf
 constexpr uint32_t kDead = 0xFFFFFFFFu;  // particle key of a deleted particle
 constexpr int kAir = 0, kFluid = 1, kSolid = 2;
@@ -134,6 +135,7 @@ struct fsl_ctx {
   double tacc[FSL_T_COUNT] = {};
   hipEvent_t tev[240] = {};  // [0,32): stage pairs, [32,232): 5 per PCG iteration
   std::string err;
+  fsl::AdjState *adj = nullptr;  // allocated by the first adjoint call
 };
 
 namespace fsl {
@@ -151,7 +153,10 @@ hipError_t launch_sort_tail(fsl_ctx *c, uint32_t n_alive);
 hipError_t launch_mask_count(fsl_ctx *c, const uint8_t *mask_dev, int64_t n, uint32_t *cnt);
 hipError_t launch_leaf_flags(fsl_ctx *c);
 hipError_t launch_compact(fsl_ctx *c, const uint32_t *flag, int32_t *list);
-hipError_t launch_p2g(fsl_ctx *c);
+hipError_t launch_p2g(fsl_ctx *c, float *const wsum[3] = nullptr);
+hipError_t launch_g2p_transpose(fsl_ctx *c, float *const pval[3], float *const out[3]);
+hipError_t launch_p2g_transpose(fsl_ctx *c, float *const g[3], float *const wsum[3], float keep,
+                                float *const io[3]);
 hipError_t launch_g2p(fsl_ctx *c);
 hipError_t launch_cell_counts_box(fsl_ctx *c, int32_t *out_dev);
 
@@ -183,6 +188,7 @@ int pcg_true_residual(fsl_ctx *c, double *rr_true);
 // adjoint.hip
 int project_backward(fsl_ctx *c, const fsl_solve_params *prm, const float *grad_out,
                      float *grad_in, fsl_solve_stats *st);
+void adjoint_free(fsl_ctx *c);
 
 }  // namespace fsl
 

@@ -296,10 +296,17 @@ __device__ inline void p2g_component(const Dom &d, const Particles &P, const int
       }
 }
 
+// ADJ = false: TransferParticlesToGrid; the weight sums are also stored in
+// wsum when it is given (the adjoint's P2G transpose divides by them).
+// ADJ = true: the transpose of G2P's interpolation, i.e. the un-normalised
+// sums sum_p w * value_p with the particle values taken from P.a[3..5]; no
+// labels or activity are written.
+template <bool ADJ>
 __global__ __launch_bounds__(512) void k_p2g(Dom d, const int32_t *list, int64_t nlist,
                                              const uint32_t *vstart, Particles P, float *vx,
                                              float *vy, float *vz, uint8_t *vflag,
-                                             uint8_t *marker) {
+                                             uint8_t *marker, float *wsx, float *wsy,
+                                             float *wsz) {
   __shared__ float s_vw[3][512];
   __shared__ float s_w[3][512];
   __shared__ uint8_t s_touch[512];
@@ -337,6 +344,11 @@ __global__ __launch_bounds__(512) void k_p2g(Dom d, const int32_t *list, int64_t
   // (flip_simulation.cc:852-857).
   const int i = o[0] + (t >> 6), j = o[1] + ((t >> 3) & 7), k = o[2] + (t & 7);
   const int64_t v = (leaf << 9) | t;
+  if (ADJ) {
+    vx[v] = s_vw[0][t]; vy[v] = s_vw[1][t]; vz[v] = s_vw[2][t];
+    return;
+  }
+  if (wsx) { wsx[v] = s_w[0][t]; wsy[v] = s_w[1][t]; wsz[v] = s_w[2][t]; }
   float out[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -429,6 +441,51 @@ __global__ __launch_bounds__(256) void k_g2p(Dom d, const int32_t *list, int64_t
   }
 }
 
+// Transpose of P2G's normalised transfer: for every particle (outside the
+// partition box too, since P2G ran before DeleteOutsideParticles)
+//   out_D = keep * in_D + sum_faces w_D(p, face) * g_D(face) / wsum_D(face),
+// with g / wsum taken as 0 where wsum < 1e-5 (the faces P2G set to 0). The
+// weights are the trilinear ones of the forward transfer, evaluated with the
+// G2P interpolation.
+__global__ __launch_bounds__(256) void k_p2g_transpose(
+    Dom d, const int32_t *list, int64_t nlist, const uint32_t *lhas, const uint32_t *vstart,
+    Particles P, const float *gx, const float *gy, const float *gz, const float *wx,
+    const float *wy, const float *wz, float keep, Particles io) {
+#pragma clang fp contract(off)
+  __shared__ float s_g[3][1000];
+  const int64_t w = xcd_work(nlist);
+  if (w >= nlist) return;
+  const int64_t leaf = list[w];
+  if (!lhas[leaf]) return;
+  int o[3];
+  leaf_origin(d, leaf, o);
+  for (int idx = threadIdx.x; idx < 1000; idx += blockDim.x) {
+    const int i = o[0] - 1 + idx / 100, j = o[1] - 1 + (idx / 10) % 10, k = o[2] - 1 + idx % 10;
+    float a[3] = {0, 0, 0};
+    if (in_store(d, i, j, k)) {
+      const int64_t v = vox_of(d, i, j, k);
+      const float g[3] = {gx[v], gy[v], gz[v]}, ws[3] = {wx[v], wy[v], wz[v]};
+#pragma unroll
+      for (int c = 0; c < 3; ++c) a[c] = double(ws[c]) < 1e-5 ? 0.0f : g[c] / ws[c];
+    }
+    s_g[0][idx] = a[0]; s_g[1][idx] = a[1]; s_g[2][idx] = a[2];
+  }
+  __syncthreads();
+  const uint32_t b = vstart[leaf << 9], e = vstart[(leaf + 1) << 9];
+  for (uint32_t q = b + threadIdx.x; q < e; q += blockDim.x) {
+    const float pos[3] = {P.a[0][q], P.a[1][q], P.a[2][q]};
+    const int c[3] = {int(floorf(pos[0])), int(floorf(pos[1])), int(floorf(pos[2]))};
+    const int lc[3] = {c[0] - o[0] + 1, c[1] - o[1] + 1, c[2] - o[2] + 1};
+    const float t[3] = {interp<0>(pos, c, lc, s_g[0]), interp<1>(pos, c, lc, s_g[1]),
+                        interp<2>(pos, c, lc, s_g[2])};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const float m = keep * io.a[a][q];
+      io.a[a][q] = m + t[a];
+    }
+  }
+}
+
 __global__ void k_cell_counts_box(Dom d, const uint32_t *vstart, int32_t *out) {
   const int bx = d.n[0] + 2, by = d.n[1] + 2, bz = d.n[2] + 2;
   const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -503,11 +560,21 @@ hipError_t launch_compact(fsl_ctx *c, const uint32_t *flag, int32_t *list) {
   return hipGetLastError();
 }
 
-hipError_t launch_p2g(fsl_ctx *c) {
+hipError_t launch_p2g(fsl_ctx *c, float *const wsum[3]) {
   if (c->n_act == 0) return hipSuccess;
-  k_p2g<<<xcd_grid(c->n_act), 512, 0, c->stream>>>(c->dom, c->act_list, c->n_act, c->vstart, c->P,
-                                                     c->v[0], c->v[1], c->v[2], c->vflag,
-                                                     c->marker);
+  k_p2g<false><<<xcd_grid(c->n_act), 512, 0, c->stream>>>(
+      c->dom, c->act_list, c->n_act, c->vstart, c->P, c->v[0], c->v[1], c->v[2], c->vflag,
+      c->marker, wsum ? wsum[0] : nullptr, wsum ? wsum[1] : nullptr, wsum ? wsum[2] : nullptr);
+  return hipGetLastError();
+}
+
+hipError_t launch_g2p_transpose(fsl_ctx *c, float *const pval[3], float *const out[3]) {
+  if (c->n_act == 0) return hipSuccess;
+  Particles A = c->P;
+  for (int a = 0; a < 3; ++a) A.a[3 + a] = pval[a];
+  k_p2g<true><<<xcd_grid(c->n_act), 512, 0, c->stream>>>(
+      c->dom, c->act_list, c->n_act, c->vstart, A, out[0], out[1], out[2], nullptr, nullptr,
+      nullptr, nullptr, nullptr);
   return hipGetLastError();
 }
 
@@ -516,6 +583,17 @@ hipError_t launch_g2p(fsl_ctx *c) {
   k_g2p<<<xcd_grid(c->n_act), 256, 0, c->stream>>>(c->dom, c->act_list, c->n_act, c->lhas,
                                                      c->vstart, c->P, c->v[0], c->v[1], c->v[2],
                                                      c->u[0], c->u[1], c->u[2], c->flip);
+  return hipGetLastError();
+}
+
+hipError_t launch_p2g_transpose(fsl_ctx *c, float *const g[3], float *const wsum[3], float keep,
+                                float *const io[3]) {
+  if (c->n_act == 0) return hipSuccess;
+  Particles A{};
+  for (int a = 0; a < 3; ++a) A.a[a] = io[a];
+  k_p2g_transpose<<<xcd_grid(c->n_act), 256, 0, c->stream>>>(
+      c->dom, c->act_list, c->n_act, c->lhas, c->vstart, c->P, g[0], g[1], g[2], wsum[0],
+      wsum[1], wsum[2], keep, A);
   return hipGetLastError();
 }
 

@@ -174,7 +174,8 @@ __device__ inline double total(const A *partials, int n, int k) {
 }
 
 __device__ inline bool converged(const PcgState *st, double rr) {
-  return st->stop_mode == FSL_STOP_ABS_RR ? rr < st->thr : sqrt(rr) < st->thr;
+  // Relative mode uses <=, so a zero right-hand side (x0 = 0) stops at once.
+  return st->stop_mode == FSL_STOP_ABS_RR ? rr < st->thr : sqrt(rr) <= st->thr;
 }
 
 // Loads p of a leaf and its halo into LDS, where cell values come from

@@ -98,6 +98,12 @@ SIGNATURES = {
     "fsl_get_timing": [_P, _DP, _I],
     "fsl_project_forward": [_P, C.POINTER(SolveParams), C.POINTER(SolveStats)],
     "fsl_project_backward": [_P, C.POINTER(SolveParams), _P, _P, C.POINTER(SolveStats)],
+    "fsl_rollout_begin": [_P, C.c_int],
+    "fsl_rollout_step": [_P, C.c_float, C.POINTER(SolveParams), _P, _P, C.POINTER(SolveStats)],
+    "fsl_rollout_replay": [_P, C.POINTER(SolveParams), _P, _P, _P, C.POINTER(SolveStats)],
+    "fsl_rollout_backward": [_P, C.POINTER(SolveParams), _P, _P, _P, C.POINTER(SolveStats)],
+    "fsl_rollout_checkpoint": [_P, C.c_int, _P, _P, _P, C.POINTER(C.c_int64),
+                               C.POINTER(C.c_int64), C.POINTER(C.c_float)],
 }
 
 _lib = None
@@ -146,6 +152,7 @@ class Context:
         self.lib = load()
         self.dims = tuple(int(d) for d in dims)
         self.box = tuple(d + 2 for d in self.dims)
+        self.force = (0.0, 9.8, 0.0)
         h = C.c_void_p()
         dv = (C.c_int * 3)(*self.dims)
         tv = (C.c_int * 3)(*tiles)
@@ -177,6 +184,7 @@ class Context:
 
     # -- set-up
     def set_params(self, gravity_scaled, force, flip):
+        self.force = tuple(float(x) for x in force)
         f = (C.c_float * 3)(*force)
         self.call("fsl_set_params", float(gravity_scaled), f, float(flip))
 
@@ -280,6 +288,52 @@ class Context:
         st = SolveStats()
         self.call("fsl_project_backward", C.byref(params), _ptr(g), _ptr(out), C.byref(st))
         return out, st.as_dict()
+
+    # -- rollouts (gradients with the recorded particle positions held fixed)
+    def rollout_begin(self, max_substeps: int):
+        self.call("fsl_rollout_begin", int(max_substeps))
+
+    def _box3(self, a):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        assert a.shape == self.box + (3,)
+        return a
+
+    def rollout_step(self, dt, params: SolveParams, u0=None, want_u_post=False):
+        u0 = self._box3(u0)
+        up = np.empty(self.box + (3,), np.float32) if want_u_post else None
+        st = SolveStats()
+        self.call("fsl_rollout_step", float(dt), C.byref(params), _ptr(u0), _ptr(up),
+                  C.byref(st))
+        return (st.as_dict(), up) if want_u_post else st.as_dict()
+
+    def rollout_replay(self, params: SolveParams, u0=None, force=None):
+        u0 = self._box3(u0)
+        f = np.ascontiguousarray(self.force if force is None else force, dtype=np.float32)
+        up = np.empty(self.box + (3,), np.float32)
+        st = SolveStats()
+        self.call("fsl_rollout_replay", C.byref(params), _ptr(u0), _ptr(f), _ptr(up), C.byref(st))
+        return up, st.as_dict()
+
+    def rollout_backward(self, params: SolveParams, loss_grad):
+        g = self._box3(loss_grad)
+        gu = np.empty(self.box + (3,), np.float32)
+        gf = np.zeros(3, np.float64)
+        st = SolveStats()
+        self.call("fsl_rollout_backward", C.byref(params), _ptr(g), _ptr(gu), _ptr(gf),
+                  C.byref(st))
+        return gu, gf, st.as_dict()
+
+    def rollout_checkpoint(self, k: int, cap: int):
+        pos = np.empty((cap, 3), np.float32)
+        vel = np.empty((cap, 3), np.float32)
+        perm = np.empty(cap, np.uint32)
+        n, n_prev, dt = C.c_int64(), C.c_int64(), C.c_float()
+        self.call("fsl_rollout_checkpoint", int(k), _ptr(pos), _ptr(vel), _ptr(perm),
+                  C.byref(n), C.byref(n_prev), C.byref(dt))
+        m = n.value
+        return pos[:m].copy(), vel[:m].copy(), perm[:m].copy(), n_prev.value, dt.value
 
     # -- measurement hook
     def set_timing(self, enable=True):

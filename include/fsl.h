@@ -205,9 +205,45 @@ int fsl_get_timing(fsl_ctx *ctx, double *out, int n);
  * pressure update, on the velocity grid in place. */
 int fsl_project_forward(fsl_ctx *ctx, const fsl_solve_params *params, fsl_solve_stats *stats);
 /* Backward: given g = dL/du_post (box format, 3 floats per voxel), writes
- * dL/du = P^T g (box format). One adjoint PCG solve on the same operator. */
+ * dL/du = P^T g (box format). One adjoint PCG solve on the same operator,
+ * with the labels of the last fsl_divergence / fsl_project_forward. The
+ * divergence and pressure grids are left as the forward pass wrote them. */
 int fsl_project_backward(fsl_ctx *ctx, const fsl_solve_params *params, const float *grad_out,
                          float *grad_in, fsl_solve_stats *stats);
+
+/* ---- rollouts with checkpointing (no reference counterpart) ----
+ * A rollout is a sequence of FLIP substeps (the fsl_step stages). Each
+ * substep's particles are checkpointed right after advection; the backward
+ * pass recomputes every substep's labels, active faces and P2G weights from
+ * its checkpoint and applies the transposed stages in reverse order. The
+ * gradient is that of the substeps with the particle positions (hence the
+ * labels and active faces) of the recorded trajectory held fixed: under that
+ * condition a substep is linear in the particle velocities, the grid
+ * velocity and the body force. */
+/* Allocates checkpoints for up to max_substeps and forgets earlier ones. */
+int fsl_rollout_begin(fsl_ctx *ctx, int max_substeps);
+/* One substep (FlipDriver::RunOneStep with dt) with its checkpoint. For the
+ * first substep, u0 (box format, may be NULL) replaces the P2G velocity on
+ * the faces P2G made active. u_post (may be NULL) receives the velocity right
+ * after the pressure update (box format). */
+int fsl_rollout_step(fsl_ctx *ctx, float dt, const fsl_solve_params *params, const float *u0,
+                     float *u_post, fsl_solve_stats *stats);
+/* Re-runs the recorded substeps with the recorded positions (no advection),
+ * from the first checkpoint's velocities, with u0 (may be NULL) and body
+ * force `force` (in place of fsl_set_params' force); u_post receives the last
+ * substep's post-projection velocity. */
+int fsl_rollout_replay(fsl_ctx *ctx, const fsl_solve_params *params, const float *u0,
+                       const float force[3], float *u_post, fsl_solve_stats *stats);
+/* Given dL/du_post of the last substep (box format), writes dL/du0 on the
+ * faces active after the first P2G (box format, may be NULL) and dL/dforce
+ * (may be NULL). Leaves the context holding the first substep's
+ * recomputation. */
+int fsl_rollout_backward(fsl_ctx *ctx, const fsl_solve_params *params, const float *loss_grad,
+                         float *grad_u0, double grad_force[3], fsl_solve_stats *stats);
+/* Checkpoint k: positions and velocities (float[n][3], may be NULL), the sort
+ * permutation (new index -> index before advection), counts and dt. */
+int fsl_rollout_checkpoint(fsl_ctx *ctx, int k, float *pos, float *vel, uint32_t *perm,
+                           int64_t *n, int64_t *n_prev, float *dt);
 
 #ifdef __cplusplus
 }

@@ -35,6 +35,14 @@
 #include <cstring>
 #include <vector>
 
+// Scalar type of grids, particles and the API: float restates the reference;
+// the double build (-DORACLE_REAL=double, liboracle64.so) is the same
+// algorithm in FP64, for finite-difference checks of the adjoint.
+#ifndef ORACLE_REAL
+#define ORACLE_REAL float
+#endif
+typedef ORACLE_REAL Real;
+
 namespace {
 
 enum CellType { AIR = 0, FLUID = 1, SOLID = 2 };
@@ -73,20 +81,20 @@ struct Dense {
 
 // Vec3 grid: one activity flag per voxel, three components.
 struct DenseVec {
-  Dense<float> c[3];
+  Dense<Real> c[3];
   void init(int x, int y, int z) { for (auto &g : c) g.init(x, y, z); }
   bool isOn(int i, int j, int k) const { return c[0].isOn(i, j, k); }
-  void get(int i, int j, int k, float v[3]) const {
+  void get(int i, int j, int k, Real v[3]) const {
     for (int d = 0; d < 3; ++d) v[d] = c[d].get(i, j, k);
   }
-  void set(int i, int j, int k, const float v[3]) {
+  void set(int i, int j, int k, const Real v[3]) {
     for (int d = 0; d < 3; ++d) c[d].set(i, j, k, v[d]);
   }
   void clear() { for (auto &g : c) g.clear(); }
 };
 
 struct Particle {
-  float pos[3], vel[3], gv[3];
+  Real pos[3], vel[3], gv[3];
   int cell[3];
 };
 
@@ -103,12 +111,12 @@ inline uint64_t leaf_of_key(uint64_t key) { return key >> 9; }
 
 struct Oracle {
   int n[3];
-  float gravity = 9.8f;  // already scaled by N_y/128 (flip_app_sim.cc:74)
-  float fx = 0, fy = 1, fz = 0;
-  float flip = 0.95f;
+  Real gravity = 9.8f;  // already scaled by N_y/128 (flip_app_sim.cc:74)
+  Real fx = 0, fy = 1, fz = 0;
+  Real flip = 0.95f;
   Dense<int32_t> solid, marker;
   DenseVec v, vu, w;   // grid_velocity_, grid_velocity_update_, grid_weight_
-  Dense<float> div, pressure;
+  Dense<Real> div, pressure;
   std::vector<Particle> parts;   // kept in VDB traversal order
   // Solver outputs.
   int n_rows = 0, n_cols = 0, iters = 0;
@@ -154,9 +162,9 @@ struct Oracle {
   }
 
   // CFL (flip_simulation.cc:459-488).
-  float cfl() const {
+  Real cfl() const {
     int e[3] = {n[0] - 1, n[1] - 1, n[2] - 1};
-    float v_inf = 0.0f;
+    Real v_inf = 0.0f;
     for (int i = 0; i <= e[0]; ++i)
       for (int j = 0; j <= e[1]; ++j)
         for (int k = 0; k <= e[2]; ++k) {
@@ -165,16 +173,16 @@ struct Oracle {
           if (i != e[0] && k != e[2]) v_inf = std::fmax(v_inf, std::fabs(v.c[1].get(i, j, k)));
           if (i != e[0] && j != e[1]) v_inf = std::fmax(v_inf, std::fabs(v.c[2].get(i, j, k)));
         }
-    float vv = float(std::fmax(std::sqrt(0.5 * double(gravity)), double(v_inf)));
+    Real vv = Real(std::fmax(std::sqrt(0.5 * double(gravity)), double(v_inf)));
     if (vv < 1e-2) vv = 1e-2f;
-    return float(1.0 / double(vv));
+    return Real(1.0 / double(vv));
   }
 
   // Particles::StepInGrid<VectorGridT,true,1,1> (particles.cc:478-546).
-  void step_in_grid(float dt) {
+  void step_in_grid(Real dt) {
     for (auto &p : parts) {
       for (int d = 0; d < 3; ++d) {
-        float m = p.gv[d] * dt;
+        Real m = p.gv[d] * dt;
         p.pos[d] = p.pos[d] + m;
       }
       int id[3];
@@ -182,7 +190,7 @@ struct Oracle {
       for (int d = 0; d < 3; ++d)
         if (id[d] < 0) { id[d] = 0; p.pos[d] = 0; }
       for (int d = 0; d < 3; ++d)
-        if (id[d] >= n[d]) { id[d] = n[d]; p.pos[d] = float(n[d]); }
+        if (id[d] >= n[d]) { id[d] = n[d]; p.pos[d] = Real(n[d]); }
       for (int d = 0; d < 3; ++d) p.cell[d] = id[d];
     }
     sort_particles();  // AddParticleToData in buffer order == stable sort
@@ -210,8 +218,8 @@ struct Oracle {
 
   // Staggered lower-sample position for component D (flip_simulation.cc:936-973).
   template <int D>
-  static void lower_sample(const float pos[3], const int c[3], float lower[3], int off[3]) {
-    lower[0] = float(c[0]); lower[1] = float(c[1]); lower[2] = float(c[2]);
+  static void lower_sample(const Real pos[3], const int c[3], Real lower[3], int off[3]) {
+    lower[0] = Real(c[0]); lower[1] = Real(c[1]); lower[2] = Real(c[2]);
     off[0] = off[1] = off[2] = 0;
     for (int a = 0; a < 3; ++a) {
       if (a == D) continue;
@@ -223,7 +231,7 @@ struct Oracle {
   // TransferParticlesToGrid (flip_simulation.cc:808-923).
   void p2g() {
     const int B = 10, S = B * B * B;
-    std::vector<float> vb(3 * S), wb(3 * S);
+    std::vector<Real> vb(3 * S), wb(3 * S);
     std::vector<char> mb(3 * S);
     size_t p0 = 0;
     while (p0 < parts.size()) {
@@ -253,7 +261,7 @@ struct Oracle {
             int o = (i * B + j) * B + k;
             if (!mb[o] && !mb[o + S] && !mb[o + 2 * S]) continue;
             int x = org[0] + i, y = org[1] + j, z = org[2] + k;
-            float vv[3], ww[3];
+            Real vv[3], ww[3];
             v.get(x, y, z, vv);
             for (int d = 0; d < 3; ++d) vv[d] = vv[d] + vb[o + d * S];
             v.set(x, y, z, vv);
@@ -267,7 +275,7 @@ struct Oracle {
     for (size_t s = 0; s < v.c[0].on.size(); ++s) {
       if (!v.c[0].on[s]) continue;
       for (int d = 0; d < 3; ++d) {
-        float ws = w.c[d].val[s];
+        Real ws = w.c[d].val[s];
         if (double(ws) < 1e-5) v.c[d].val[s] = 0;  // THRESH is a double literal
         else v.c[d].val[s] = v.c[d].val[s] / ws;
       }
@@ -275,13 +283,13 @@ struct Oracle {
   }
   // ScatterWeight<D> (flip_simulation.cc:925-1009).
   template <int D>
-  void scatter(const Particle &p, const int lc[3], float *vb, float *wb, char *mb) const {
-    float lower[3]; int off[3];
+  void scatter(const Particle &p, const int lc[3], Real *vb, Real *wb, char *mb) const {
+    Real lower[3]; int off[3];
     lower_sample<D>(p.pos, p.cell, lower, off);
     int st[3] = {lc[0] + off[0], lc[1] + off[1], lc[2] + off[2]};
-    float x = p.pos[0] - lower[0], y = p.pos[1] - lower[1], z = p.pos[2] - lower[2];
-    float xy = x * y, yz = y * z, xz = z * x, xyz = xy * z;
-    float wt[2][2][2];
+    Real x = p.pos[0] - lower[0], y = p.pos[1] - lower[1], z = p.pos[2] - lower[2];
+    Real xy = x * y, yz = y * z, xz = z * x, xyz = xy * z;
+    Real wt[2][2][2];
     wt[1][1][1] = xyz;
     wt[1][1][0] = xy - xyz;
     wt[1][0][1] = xz - xyz;
@@ -294,7 +302,7 @@ struct Oracle {
       for (int dj = 0; dj < 2; ++dj)
         for (int dk = 0; dk < 2; ++dk) {
           int o = ((st[0] + di) * 10 + st[1] + dj) * 10 + st[2] + dk;
-          float m = wt[di][dj][dk] * p.vel[D];
+          Real m = wt[di][dj][dk] * p.vel[D];
           vb[o] = vb[o] + m;
           wb[o] = wb[o] + wt[di][dj][dk];
           mb[o] = 1;
@@ -326,10 +334,10 @@ struct Oracle {
         for (int d = 0; d < 3; ++d) { vu.c[d].val[s] = v.c[d].val[s]; vu.c[d].on[s] = 1; }
   }
   // AddGravity (flip_simulation.cc:505-515).
-  void add_gravity(float dt) {
+  void add_gravity(Real dt) {
     for (size_t s = 0; s < v.c[0].on.size(); ++s)
       if (v.c[0].on[s]) {
-        float a = dt * fx, b = dt * fy, c = dt * fz;
+        Real a = dt * fx, b = dt * fy, c = dt * fz;
         v.c[0].val[s] = v.c[0].val[s] + a;
         v.c[1].val[s] = v.c[1].val[s] - b;
         v.c[2].val[s] = v.c[2].val[s] + c;
@@ -344,7 +352,7 @@ struct Oracle {
       for (int j = 0; j < n[1]; ++j)
         for (int k = 0; k < n[2]; ++k)
           if (marker.get(i, j, k) == FLUID) {
-            float d = 0;
+            Real d = 0;
             d += v.c[0].get(i + 1, j, k) - v.c[0].get(i, j, k);
             d += v.c[1].get(i, j + 1, k) - v.c[1].get(i, j, k);
             d += v.c[2].get(i, j, k + 1) - v.c[2].get(i, j, k);
@@ -357,11 +365,11 @@ struct Oracle {
     for (int i = 0; i <= n[0]; ++i)
       for (int j = 0; j <= n[1]; ++j)
         for (int k = 0; k <= n[2]; ++k) {
-          float vv[3];
+          Real vv[3];
           v.get(i, j, k, vv);
           if (marker.get(i, j, k) == SOLID) continue;
           bool update = false;
-          const float pc = pressure.get(i, j, k);
+          const Real pc = pressure.get(i, j, k);
           if ((marker.get(i - 1, j, k) == FLUID || marker.get(i, j, k) == FLUID) &&
               marker.get(i - 1, j, k) != SOLID) {
             vv[0] += (pc - pressure.get(i - 1, j, k)); update = true;
@@ -385,7 +393,7 @@ struct Oracle {
         for (int k = 0; k <= n[2]; ++k) {
           if (v.isOn(i, j, k) && !vu.isOn(i, j, k)) return -1;
           if (v.isOn(i, j, k) && vu.isOn(i, j, k)) {
-            float a[3], b[3];
+            Real a[3], b[3];
             v.get(i, j, k, a); vu.get(i, j, k, b);
             for (int d = 0; d < 3; ++d) a[d] = a[d] - b[d];
             vu.set(i, j, k, a);
@@ -396,7 +404,7 @@ struct Oracle {
   // VectorGrid::ExtrapolateToBoundary (vector_grid.cc:302-381), one partition.
   static void extrapolate(DenseVec &g, const int n[3]) {
     auto copy2 = [&](int oi, int oj, int ok, int ii, int ij, int ik, int keep) {
-      float out[3], in[3];
+      Real out[3], in[3];
       g.get(oi, oj, ok, out); g.get(ii, ij, ik, in);
       for (int d = 0; d < 3; ++d) if (d != keep) out[d] = in[d];
       g.set(oi, oj, ok, out);
@@ -410,22 +418,22 @@ struct Oracle {
   }
   // InterpolateComponent<D> (flip_simulation.cc:1093-1166).
   template <int D>
-  static float interp(const float pos[3], const int c[3], const int lc[3], const float *buf) {
-    float lower[3]; int off[3];
+  static Real interp(const Real pos[3], const int c[3], const int lc[3], const Real *buf) {
+    Real lower[3]; int off[3];
     lower_sample<D>(pos, c, lower, off);
     int i = lc[0] + off[0], j = lc[1] + off[1], k = lc[2] + off[2];
-    float w0 = pos[0] - lower[0], w1 = pos[1] - lower[1], w2 = pos[2] - lower[2];
+    Real w0 = pos[0] - lower[0], w1 = pos[1] - lower[1], w2 = pos[2] - lower[2];
     auto at = [&](int a, int b, int cc) { return buf[(a * 10 + b) * 10 + cc]; };
-    float v000 = at(i, j, k), v001 = at(i, j, k + 1), v010 = at(i, j + 1, k),
+    Real v000 = at(i, j, k), v001 = at(i, j, k + 1), v010 = at(i, j + 1, k),
           v011 = at(i, j + 1, k + 1), v100 = at(i + 1, j, k), v101 = at(i + 1, j, k + 1),
           v110 = at(i + 1, j + 1, k), v111 = at(i + 1, j + 1, k + 1);
-    float t;
-    t = w2 * (v001 - v000); float v00 = v000 + t;
-    t = w2 * (v011 - v010); float v01 = v010 + t;
-    t = w2 * (v101 - v100); float v10 = v100 + t;
-    t = w2 * (v111 - v110); float v11 = v110 + t;
-    t = w1 * (v01 - v00); float v0 = v00 + t;
-    t = w1 * (v11 - v10); float v1 = v10 + t;
+    Real t;
+    t = w2 * (v001 - v000); Real v00 = v000 + t;
+    t = w2 * (v011 - v010); Real v01 = v010 + t;
+    t = w2 * (v101 - v100); Real v10 = v100 + t;
+    t = w2 * (v111 - v110); Real v11 = v110 + t;
+    t = w1 * (v01 - v00); Real v0 = v00 + t;
+    t = w1 * (v11 - v10); Real v1 = v10 + t;
     t = w0 * (v1 - v0); return v0 + t;
   }
   // UpdateParticlesFromGrid (flip_simulation.cc:1024-1091, 1178-1189).
@@ -433,7 +441,7 @@ struct Oracle {
     extrapolate(v, n);
     extrapolate(vu, n);
     const int S = 1000;
-    std::vector<float> vb(3 * S), db(3 * S);
+    std::vector<Real> vb(3 * S), db(3 * S);
     size_t p0 = 0;
     while (p0 < parts.size()) {
       uint64_t lk = leaf_of_key(vdb_key(parts[p0].cell[0], parts[p0].cell[1], parts[p0].cell[2]));
@@ -455,14 +463,14 @@ struct Oracle {
       for (size_t q = p0; q < p1; ++q) {
         Particle &p = parts[q];
         int lc[3] = {p.cell[0] - org[0], p.cell[1] - org[1], p.cell[2] - org[2]};
-        float gv[3] = {interp<0>(p.pos, p.cell, lc, &vb[0]), interp<1>(p.pos, p.cell, lc, &vb[S]),
+        Real gv[3] = {interp<0>(p.pos, p.cell, lc, &vb[0]), interp<1>(p.pos, p.cell, lc, &vb[S]),
                        interp<2>(p.pos, p.cell, lc, &vb[2 * S])};
-        float dv[3] = {interp<0>(p.pos, p.cell, lc, &db[0]), interp<1>(p.pos, p.cell, lc, &db[S]),
+        Real dv[3] = {interp<0>(p.pos, p.cell, lc, &db[0]), interp<1>(p.pos, p.cell, lc, &db[S]),
                        interp<2>(p.pos, p.cell, lc, &db[2 * S])};
         for (int d = 0; d < 3; ++d) {
-          float a = dv[d] + p.vel[d];
-          float b = a - gv[d];
-          float c = flip * b;
+          Real a = dv[d] + p.vel[d];
+          Real b = a - gv[d];
+          Real c = flip * b;
           p.vel[d] = gv[d] + c;
           p.gv[d] = gv[d];
         }
@@ -475,7 +483,7 @@ struct Oracle {
   struct Csr {
     std::vector<int64_t> start;
     std::vector<int32_t> col;
-    std::vector<float> val;
+    std::vector<Real> val;
   };
   // Solver::InitializeMatrixAndVectors + CreateIndexAndMaskTree +
   // CreateLaplacianMatrix (solver.cc:75-91, 114-151, 197-233).
@@ -505,8 +513,8 @@ struct Oracle {
     const int off[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
     for (int r = 0; r < n_rows; ++r) {
       int i = rows_ijk[3 * r], j = rows_ijk[3 * r + 1], k = rows_ijk[3 * r + 2];
-      std::vector<std::pair<int32_t, float>> ent;
-      float diag = 0;
+      std::vector<std::pair<int32_t, Real>> ent;
+      Real diag = 0;
       for (int q = 0; q < 6; ++q) {
         int m = marker.get(i + off[q][0], j + off[q][1], k + off[q][2]);
         if (m == FLUID || m == AIR) diag += 1;
@@ -549,7 +557,7 @@ struct Oracle {
     for (int i = -1; i <= n[0]; ++i) for (int j = -1; j <= n[1]; ++j) for (int k = -1; k <= n[2]; ++k)
       if (idx.isOn(i, j, k)) x[idx.get(i, j, k)] = T(pressure.get(i, j, k));
     for (int q = 0; q < n_rows; ++q) {
-      float dg = 0;
+      Real dg = 0;
       for (int64_t e = A.start[q]; e < A.start[q + 1]; ++e) if (A.col[e] == q) dg = A.val[e];
       dinv[q] = static_cast<T>(1.0 / double(dg));
       if (dg == 0) return -2;  // Jacobi diagonal 0: FLUID cell enclosed by SOLID
@@ -558,7 +566,8 @@ struct Oracle {
     for (int q = 0; q < n_rows; ++q) bnorm += double(b[q]) * double(b[q]);
     bnorm = std::sqrt(bnorm);
     const double thr = stop_mode == 0 ? threshold : threshold * bnorm;
-    auto done = [&](T rr) { return stop_mode == 0 ? double(rr) < thr : std::sqrt(double(rr)) < thr; };
+    // Relative mode uses <=, so a zero right-hand side (x0 = 0) stops at once.
+    auto done = [&](T rr) { return stop_mode == 0 ? double(rr) < thr : std::sqrt(double(rr)) <= thr; };
     // PreparePCGSolver.
     for (int q = 0; q < n_rows; ++q) { T m = row_dot(A, q, x); r[q] = b[q] - m; }
     for (int q = 0; q < n_rows; ++q) z[q] = dinv[q] * r[q];
@@ -590,7 +599,7 @@ struct Oracle {
     // CopySolution (solver.cc:93-99): clear, then interior rows only.
     pressure.clear();
     for (int q = 0; q < n_rows; ++q)
-      pressure.set(rows[3 * q], rows[3 * q + 1], rows[3 * q + 2], float(x[q]));
+      pressure.set(rows[3 * q], rows[3 * q + 1], rows[3 * q + 2], Real(x[q]));
     last_x64.assign(x.begin(), x.begin() + n_rows);
     return 0;
   }
@@ -612,7 +621,7 @@ void *or_create(int nx, int ny, int nz) {
   return o;
 }
 void or_destroy(void *h) { delete static_cast<Oracle *>(h); }
-void or_set_params(void *h, float gravity_scaled, float fx, float fy, float fz, float flip) {
+void or_set_params(void *h, Real gravity_scaled, Real fx, Real fy, Real fz, Real flip) {
   Oracle *o = static_cast<Oracle *>(h);
   o->gravity = gravity_scaled; o->fx = fx; o->fy = fy; o->fz = fz; o->flip = flip;
 }
@@ -622,7 +631,7 @@ void or_add_solid_box(void *h, const int *lo, const int *hi) {
 }
 // Particles are added in the given order (AddParticleToData); their cells are
 // floor(pos), as the reference asserts (flip_simulation.cc:932-934).
-void or_set_particles(void *h, int64_t n, const float *pos, const float *vel, const float *gv) {
+void or_set_particles(void *h, int64_t n, const Real *pos, const Real *vel, const Real *gv) {
   Oracle *o = static_cast<Oracle *>(h);
   o->parts.resize(size_t(n));
   for (int64_t q = 0; q < n; ++q) {
@@ -635,7 +644,7 @@ void or_set_particles(void *h, int64_t n, const float *pos, const float *vel, co
   o->sort_particles();
 }
 int64_t or_num_particles(void *h) { return int64_t(static_cast<Oracle *>(h)->parts.size()); }
-void or_get_particles(void *h, float *pos, float *vel, float *gv, int32_t *cell) {
+void or_get_particles(void *h, Real *pos, Real *vel, Real *gv, int32_t *cell) {
   Oracle *o = static_cast<Oracle *>(h);
   for (size_t q = 0; q < o->parts.size(); ++q)
     for (int d = 0; d < 3; ++d) {
@@ -646,9 +655,9 @@ void or_get_particles(void *h, float *pos, float *vel, float *gv, int32_t *cell)
       if (cell) cell[3 * q + d] = p.cell[d];
     }
 }
-float or_cfl(void *h) { return static_cast<Oracle *>(h)->cfl(); }
+Real or_cfl(void *h) { return static_cast<Oracle *>(h)->cfl(); }
 // MoveParticles + DeleteParticlesInSolids (flip_app_sim.cc:496-517).
-void or_move_particles(void *h, float dt) {
+void or_move_particles(void *h, Real dt) {
   Oracle *o = static_cast<Oracle *>(h);
   o->step_in_grid(dt);
   o->delete_in_solids();
@@ -658,13 +667,13 @@ void or_p2g(void *h) { static_cast<Oracle *>(h)->p2g(); }
 void or_apply_bc(void *h) { static_cast<Oracle *>(h)->apply_bc(); }
 void or_save_velocities(void *h) { static_cast<Oracle *>(h)->save_velocities(); }
 void or_delete_outside(void *h) { static_cast<Oracle *>(h)->delete_outside(); }
-void or_add_gravity(void *h, float dt) { static_cast<Oracle *>(h)->add_gravity(dt); }
+void or_add_gravity(void *h, Real dt) { static_cast<Oracle *>(h)->add_gravity(dt); }
 void or_divergence(void *h) { static_cast<Oracle *>(h)->divergence(); }
 // precision 0: float, 1: double.
 int or_solve(void *h, int precision, int stop_mode, double threshold, int max_iter,
              int *iters, double *rr) {
   Oracle *o = static_cast<Oracle *>(h);
-  int rc = precision == 0 ? o->solve<float>(stop_mode, threshold, max_iter)
+  int rc = precision == 0 ? o->solve<Real>(stop_mode, threshold, max_iter)
                           : o->solve<double>(stop_mode, threshold, max_iter);
   if (iters) *iters = o->iters;
   if (rr) *rr = o->rr_final;
@@ -699,7 +708,7 @@ void or_g2p(void *h) { static_cast<Oracle *>(h)->g2p(); }
 
 // FlipDriver::RunOneStep (flip_app_sim.cc:211-254) for one partition with no
 // sources. Returns the solver status.
-int or_run_one_step(void *h, float dt, int precision, int stop_mode, double threshold,
+int or_run_one_step(void *h, Real dt, int precision, int stop_mode, double threshold,
                     int max_iter, int *iters) {
   Oracle *o = static_cast<Oracle *>(h);
   o->step_in_grid(dt);
@@ -712,7 +721,7 @@ int or_run_one_step(void *h, float dt, int precision, int stop_mode, double thre
   o->add_gravity(dt);
   o->apply_bc();
   o->divergence();
-  int rc = precision == 0 ? o->solve<float>(stop_mode, threshold, max_iter)
+  int rc = precision == 0 ? o->solve<Real>(stop_mode, threshold, max_iter)
                           : o->solve<double>(stop_mode, threshold, max_iter);
   if (iters) *iters = o->iters;
   if (rc != 0) return rc;
@@ -724,10 +733,10 @@ int or_run_one_step(void *h, float dt, int precision, int stop_mode, double thre
 
 // WhileAndComputeDt's reduction and clamps (flip_app_sim.cc:374, 395-407) for
 // one partition. Returns dt and advances *step.
-float or_compute_dt(void *h, int cfl, float frame_step, float *step) {
+Real or_compute_dt(void *h, int cfl, Real frame_step, Real *step) {
   Oracle *o = static_cast<Oracle *>(h);
-  float local_dt = float(cfl) * o->cfl();
-  float global_dt = float(1.0 / double(frame_step));
+  Real local_dt = Real(cfl) * o->cfl();
+  Real global_dt = Real(1.0 / double(frame_step));
   global_dt = std::min(global_dt, local_dt);
   if (global_dt < frame_step / 10) global_dt = frame_step / 10;
   if (*step + global_dt > frame_step) global_dt = frame_step - *step;
@@ -737,7 +746,7 @@ float or_compute_dt(void *h, int cfl, float frame_step, float *step) {
 }
 
 // Box-format access. which: 0 grid_velocity, 1 grid_velocity_update, 2 weight.
-void or_get_vec(void *h, int which, float *out, uint8_t *active) {
+void or_get_vec(void *h, int which, Real *out, uint8_t *active) {
   Oracle *o = static_cast<Oracle *>(h);
   const DenseVec &g = which == 0 ? o->v : which == 1 ? o->vu : o->w;
   for (int i = -1; i <= o->n[0]; ++i) for (int j = -1; j <= o->n[1]; ++j) for (int k = -1; k <= o->n[2]; ++k) {
@@ -746,7 +755,7 @@ void or_get_vec(void *h, int which, float *out, uint8_t *active) {
     if (active) active[b] = g.isOn(i, j, k) ? 1 : 0;
   }
 }
-void or_set_vec(void *h, int which, const float *in, const uint8_t *active) {
+void or_set_vec(void *h, int which, const Real *in, const uint8_t *active) {
   Oracle *o = static_cast<Oracle *>(h);
   DenseVec &g = which == 0 ? o->v : which == 1 ? o->vu : o->w;
   g.clear();
@@ -756,9 +765,9 @@ void or_set_vec(void *h, int which, const float *in, const uint8_t *active) {
   }
 }
 // which: 0 divergence, 1 pressure.
-void or_get_scalar(void *h, int which, float *out) {
+void or_get_scalar(void *h, int which, Real *out) {
   Oracle *o = static_cast<Oracle *>(h);
-  const Dense<float> &g = which == 0 ? o->div : o->pressure;
+  const Dense<Real> &g = which == 0 ? o->div : o->pressure;
   for (int i = -1; i <= o->n[0]; ++i) for (int j = -1; j <= o->n[1]; ++j) for (int k = -1; k <= o->n[2]; ++k)
     out[box_idx(o->n, i, j, k)] = g.get(i, j, k);
 }
@@ -781,12 +790,12 @@ void or_set_marker(void *h, const int32_t *in) {
     if (m != AIR) o->marker.set(i, j, k, m);
   }
 }
-void or_set_scalar(void *h, int which, const float *in) {
+void or_set_scalar(void *h, int which, const Real *in) {
   Oracle *o = static_cast<Oracle *>(h);
-  Dense<float> &g = which == 0 ? o->div : o->pressure;
+  Dense<Real> &g = which == 0 ? o->div : o->pressure;
   g.clear();
   for (int i = -1; i <= o->n[0]; ++i) for (int j = -1; j <= o->n[1]; ++j) for (int k = -1; k <= o->n[2]; ++k) {
-    float x = in[box_idx(o->n, i, j, k)];
+    Real x = in[box_idx(o->n, i, j, k)];
     if (x != 0.0f) g.set(i, j, k, x);
   }
 }

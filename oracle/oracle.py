@@ -1,4 +1,4 @@
-"""ctypes wrapper of the CPU oracle (oracle/liboracle.so).
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so, FP64 build liboracle64.so).
 
 TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and
 bench.py's cpu_baseline leg, as the checker. The oracle restates the
@@ -14,6 +14,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
+LIB64_PATH = os.path.join(HERE, "liboracle64.so")
 
 
 def build() -> str:
@@ -21,16 +22,19 @@ def build() -> str:
     return LIB_PATH
 
 
-_lib = None
+_libs = {}
 
 
-def load():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
+def load(fp64=False):
+    """The FP32 restatement (liboracle.so), or with fp64 the same source built
+    with double grids, particles and scalars (liboracle64.so)."""
+    if fp64 not in _libs:
+        path = LIB64_PATH if fp64 else LIB_PATH
+        if not os.path.exists(path):
             build()
-        lib = C.CDLL(LIB_PATH)
-        P, I, F, D, I64 = C.c_void_p, C.c_int, C.c_float, C.c_double, C.c_int64
+        lib = C.CDLL(path)
+        P, I, D, I64 = C.c_void_p, C.c_int, C.c_double, C.c_int64
+        F = C.c_double if fp64 else C.c_float
         sig = {
             "or_create": ([I, I, I], P), "or_destroy": ([P], None),
             "or_set_params": ([P, F, F, F, F, F], None), "or_set_box_boundary": ([P, I], None),
@@ -57,8 +61,8 @@ def load():
             fn = getattr(lib, name)
             fn.argtypes = args
             fn.restype = res
-        _lib = lib
-    return _lib
+        _libs[fp64] = lib
+    return _libs[fp64]
 
 
 def _ptr(a):
@@ -68,8 +72,10 @@ def _ptr(a):
 class Oracle:
     """The reference's one-partition FLIP step on dense CPU grids."""
 
-    def __init__(self, dims):
-        self.lib = load()
+    def __init__(self, dims, fp64=False):
+        self.lib = load(fp64)
+        self.real = np.float64 if fp64 else np.float32
+        self.creal = C.c_double if fp64 else C.c_float
         self.dims = tuple(int(d) for d in dims)
         self.box = tuple(d + 2 for d in self.dims)
         self.h = self.lib.or_create(*self.dims)
@@ -89,9 +95,9 @@ class Oracle:
         self.lib.or_add_solid_box(self.h, _ptr(np.array(lo, np.int32)), _ptr(np.array(hi, np.int32)))
 
     def set_particles(self, pos, vel, gv=None):
-        pos = np.ascontiguousarray(pos, np.float32)
-        vel = np.ascontiguousarray(vel, np.float32)
-        gv = vel if gv is None else np.ascontiguousarray(gv, np.float32)
+        pos = np.ascontiguousarray(pos, self.real)
+        vel = np.ascontiguousarray(vel, self.real)
+        gv = vel if gv is None else np.ascontiguousarray(gv, self.real)
         self.lib.or_set_particles(self.h, pos.shape[0], _ptr(pos), _ptr(vel), _ptr(gv))
 
     def num_particles(self):
@@ -99,7 +105,7 @@ class Oracle:
 
     def get_particles(self):
         n = self.num_particles()
-        pos, vel, gv = (np.empty((n, 3), np.float32) for _ in range(3))
+        pos, vel, gv = (np.empty((n, 3), self.real) for _ in range(3))
         cell = np.empty((n, 3), np.int32)
         self.lib.or_get_particles(self.h, _ptr(pos), _ptr(vel), _ptr(gv), _ptr(cell))
         return pos, vel, gv, cell
@@ -108,7 +114,7 @@ class Oracle:
         return float(self.lib.or_cfl(self.h))
 
     def compute_dt(self, cfl, frame_step, step):
-        s = C.c_float(step)
+        s = self.creal(step)
         dt = self.lib.or_compute_dt(self.h, int(cfl), float(frame_step), C.byref(s))
         return float(dt), s.value
 
@@ -181,23 +187,23 @@ class Oracle:
         return rc, it.value
 
     def get_vec(self, which=0):
-        out = np.empty(self.box + (3,), np.float32)
+        out = np.empty(self.box + (3,), self.real)
         act = np.empty(self.box, np.uint8)
         self.lib.or_get_vec(self.h, which, _ptr(out), _ptr(act))
         return out, act
 
     def set_vec(self, which, values, active):
-        v = np.ascontiguousarray(values, np.float32)
+        v = np.ascontiguousarray(values, self.real)
         a = np.ascontiguousarray(active, np.uint8)
         self.lib.or_set_vec(self.h, which, _ptr(v), _ptr(a))
 
     def get_scalar(self, which):
-        out = np.empty(self.box, np.float32)
+        out = np.empty(self.box, self.real)
         self.lib.or_get_scalar(self.h, which, _ptr(out))
         return out
 
     def set_scalar(self, which, values):
-        self.lib.or_set_scalar(self.h, which, _ptr(np.ascontiguousarray(values, np.float32)))
+        self.lib.or_set_scalar(self.h, which, _ptr(np.ascontiguousarray(values, self.real)))
 
     def get_int(self, which):
         out = np.empty(self.box, np.int32)
