@@ -39,16 +39,24 @@ HBM_PEAK_GBS = 8000.0
 PASS_A_BYTES = {fsl.PCG_FP32: 17, fsl.PCG_MIXED: 17, fsl.PCG_FP64: 33}
 PASS_B_BYTES = {fsl.PCG_FP32: 29, fsl.PCG_MIXED: 29, fsl.PCG_FP64: 57}
 PREC_NAME = {fsl.PCG_FP32: "fp32", fsl.PCG_FP64: "fp64", fsl.PCG_MIXED: "mixed"}
+# Algorithmic bytes per particle: P2G reads position and velocity (24 B) and
+# writes 12 B of faces + 1 B of label per cell (8 particles per cell); G2P
+# reads position and velocity (24 B), writes velocity and grid velocity
+# (24 B) and reads v and dv faces (24 B per cell / 8).
+P2G_BYTES = 24 + 13 / 8
+WORKLOAD_TEXT = {
+    "dam_break_256": "fluid x<96 y<160",
+    "multi_dam_512": "pool y<32 plus four 96x192x96 corner blocks",
+    "sphere_drop_128": "pool y<32 plus sphere r=20 at (64,92,64)",
+    "sphere_drop_64": "pool y<16 plus sphere r=10 at (32,46,32)",
+    "dam_break_128": "fluid x<48 y<80",
+}
+G2P_BYTES = 48 + 24 / 8
 
 
-def sync():
-    try:
-        import torch
-
-        if torch.cuda.is_available():
-            torch.cuda.synchronize()
-    except Exception:
-        pass
+def sync(ctx):
+    """Waits for the context's stream (the library's own HIP stream)."""
+    ctx.synchronize()
 
 
 def setup(dims, tiles, mask, seed, ppc=8):
@@ -237,19 +245,19 @@ def main():
         one_step()
     # Timed region: K steps, no measurement hook (its per-kernel events slow
     # the launches down), bracketed by device synchronisations.
-    sync()
+    sync(ctx)
     t0 = time.perf_counter()
     stats = [one_step() for _ in range(a.steps)]
-    sync()
+    sync(ctx)
     elapsed = time.perf_counter() - t0
     steps_per_s = a.steps / elapsed
     # Kernel durations: the next K steps with the hook on, which records each
     # PCG pass's own start and end on the library's stream.
     ctx.set_timing(True)
-    sync()
+    sync(ctx)
     t1 = time.perf_counter()
     stats_t = [one_step() for _ in range(a.steps)]
-    sync()
+    sync(ctx)
     hooked_ms = 1e3 * (time.perf_counter() - t1) / a.steps
     tm = ctx.timing()
     ctx.set_timing(False)
@@ -264,6 +272,8 @@ def main():
     pcg_gbps = sum((PASS_A_BYTES[2] + PASS_B_BYTES[2]) * r * i for r, i in zip(rows_t, iters_t)) / pcg_time / 1e9
     stage_ms = {k: tm[k] / a.steps for k in ("move", "clear", "p2g", "bc_save", "gravity_bc",
                                              "divergence", "solve", "update", "g2p")}
+    p2g_rate = n_particles / (stage_ms["p2g"] * 1e-3)
+    g2p_rate = n_particles / (stage_ms["g2p"] * 1e-3)
     traffic = None
     tf = os.path.join(ROOT, "profiles", "pcg_pass_a_traffic.json")
     if os.path.exists(tf):
@@ -284,7 +294,9 @@ def main():
         cpu = cpu_baseline(int(round(float(np.mean(iters)))))
 
     line = {
-        "metric": "FLIP steps/s at 256^3 (mixed-precision PCG, rel. residual 1e-6)",
+        "metric": ("FLIP steps/s at 256^3 (mixed-precision PCG, rel. residual 1e-6)"
+                   if a.workload == "dam_break_256" else
+                   f"FLIP steps/s, {a.workload} (mixed-precision PCG, rel. residual 1e-6)"),
         "value": steps_per_s,
         "unit": "FLIP steps/s",
         "n_gpus": 1,
@@ -298,7 +310,8 @@ def main():
         "dtype": "f32 (stencil, axpys) + f64 (dot products, residual refresh)",
         "data": "synthetic (Philox-seeded dam break)",
         "config": {
-            "workload": f"{a.workload}: FLIP step, fluid x<96 y<160, 8 ppc, 8 tiles (2x2x2)",
+            "workload": f"{a.workload}: FLIP step, {WORKLOAD_TEXT.get(a.workload, '')}, 8 ppc, "
+                        f"tiles {'x'.join(str(t) for t in tiles)}",
             "dims": list(dims),
             "particles": n_particles,
             "fluid_rows_mean": rows_w,
@@ -307,6 +320,11 @@ def main():
         },
         "pcg_GBps": pcg_gbps,
         "pcg_frac_of_hbm_peak": pcg_gbps / HBM_PEAK_GBS,
+        "pcg_iterations_per_s": sum(iters_t) / pcg_time,
+        "p2g_particles_per_s": p2g_rate,
+        "p2g_frac_of_hbm_peak": p2g_rate * P2G_BYTES / (HBM_PEAK_GBS * 1e9),
+        "g2p_particles_per_s": g2p_rate,
+        "g2p_frac_of_hbm_peak": g2p_rate * G2P_BYTES / (HBM_PEAK_GBS * 1e9),
         "stage_ms_per_step": stage_ms,
         "roofline": {
             "kernel": "PCG pass A (p = z + beta p, 7-point SpMV from LDS, p.Ap)",

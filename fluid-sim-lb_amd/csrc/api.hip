@@ -202,6 +202,12 @@ int fsl_destroy(fsl_ctx *ctx) {
 
 const char *fsl_last_error(const fsl_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
+int fsl_synchronize(fsl_ctx *ctx) {
+  CHECK_CTX();
+  FSL_HIP(hipStreamSynchronize(ctx->stream));
+  return FSL_OK;
+}
+
 int fsl_set_params(fsl_ctx *ctx, float gravity_scaled, const float force[3], float flip) {
   CHECK_CTX();
   if (!force) return FSL_ERR_ARG;

@@ -98,6 +98,7 @@ SIGNATURES = {
     "fsl_get_timing": [_P, _DP, _I],
     "fsl_project_forward": [_P, C.POINTER(SolveParams), C.POINTER(SolveStats)],
     "fsl_project_backward": [_P, C.POINTER(SolveParams), _P, _P, C.POINTER(SolveStats)],
+    "fsl_synchronize": [_P],
     "fsl_rollout_begin": [_P, C.c_int],
     "fsl_rollout_step": [_P, C.c_float, C.POINTER(SolveParams), _P, _P, C.POINTER(SolveStats)],
     "fsl_rollout_replay": [_P, C.POINTER(SolveParams), _P, _P, _P, C.POINTER(SolveStats)],
@@ -288,6 +289,9 @@ class Context:
         st = SolveStats()
         self.call("fsl_project_backward", C.byref(params), _ptr(g), _ptr(out), C.byref(st))
         return out, st.as_dict()
+
+    def synchronize(self):
+        self.call("fsl_synchronize")
 
     # -- rollouts (gradients with the recorded particle positions held fixed)
     def rollout_begin(self, max_substeps: int):

@@ -92,6 +92,8 @@ int fsl_create(fsl_ctx **out, const int dims[3], const int tiles[3], int device,
                int64_t particle_capacity);
 int fsl_destroy(fsl_ctx *ctx);
 const char *fsl_last_error(const fsl_ctx *ctx);
+/* Waits until all work enqueued on the context's stream has finished. */
+int fsl_synchronize(fsl_ctx *ctx);
 /* Scaled gravity (flip_app_sim.cc:74), force = config force * scaled gravity
  * (flip_app_sim.cc:87, SetForces flip_simulation.cc:248-253), flip factor. */
 int fsl_set_params(fsl_ctx *ctx, float gravity_scaled, const float force[3], float flip);

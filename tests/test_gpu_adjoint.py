@@ -280,3 +280,24 @@ def test_rollout_gradient_sphere_drop_128():
         print(f"128^3 dL/du0 . e{t}: adjoint {ad:.8e} FD {fd:.8e} (scale {scale:.3e})")
         assert abs(ad - fd) <= 1e-4 * scale
     ctx.close()
+
+
+def test_autograd_projection_matches_c_abi():
+    import torch
+
+    from fsl.autograd import project
+
+    n = (32, 32, 32)
+    marker = forward_labels(n, scenes.sphere_drop_mask(n, 8, (16, 23, 16), 5), 8)
+    ctx = projection_ctx(n, marker)
+    rng = np.random.default_rng(9)
+    # Host tensors: the C ABI takes host arrays, and the library runs on its
+    # own HIP stream (torch's device runtime is not involved).
+    u = torch.tensor(rng.standard_normal(ctx.box + (3,)), dtype=torch.float32,
+                     requires_grad=True)
+    w = torch.tensor(rng.standard_normal(ctx.box + (3,)), dtype=torch.float32)
+    loss = (project(ctx, u, tight()) * w).sum()
+    loss.backward()
+    ref, _ = ctx.project_backward(tight(), w.cpu().numpy())
+    assert np.allclose(u.grad.cpu().numpy(), ref, rtol=0, atol=1e-6 * np.abs(ref).max())
+    ctx.close()
