@@ -8,6 +8,8 @@
 //   gravity=9.8 flip=0.95 frame_rate=30 cfl=1
 //   solver_max_iterations=1500 solver_threshold=5e-9 (times n^3)
 //   output=<prefix>   writes <prefix>NNNNNNNN binary particle frames
+//   grid_output=<prefix>  also writes text grid dumps <prefix>{velocity,
+//                     pressure,marker}_NNNNNNNN each frame
 // Prints one JSON object per frame.
 #include <chrono>
 #include <cstdio>
@@ -91,6 +93,14 @@ int main(int argc, char **argv) {
         char num[16];
         std::snprintf(num, sizeof(num), "%08d", driver.frame());
         sim.WriteParticles(out + num, false);
+      }
+      const std::string gout = get(kv, "grid_output", "");
+      if (!gout.empty()) {
+        char num[16];
+        std::snprintf(num, sizeof(num), "%08d", driver.frame());
+        sim.WriteGrid(gout + "velocity_" + num, FSL_GRID_VELOCITY);
+        sim.WriteGrid(gout + "pressure_" + num, 10 + FSL_GRID_PRESSURE);
+        sim.WriteGrid(gout + "marker_" + num, 10 + FSL_GRID_MARKER);
       }
       std::printf("{\"frame\": %d, \"steps\": %d, \"particles\": %zu, \"last_iterations\": %d, "
                   "\"seconds\": %.6f}\n",

@@ -63,6 +63,26 @@ __global__ void k_mask_to_count(const uint8_t *mask, int64_t n, uint32_t *cnt) {
 }
 
 // ------------------------------------------------------ advection and sort
+// Runs of equal keys within a wavefront. Particles arrive sorted by cell from
+// the previous step and move at most about one cell, so neighbouring lanes
+// mostly share a key: one atomic per run instead of one per particle.
+struct KeyRun {
+  int head;    // lane of the first particle of this lane's run
+  int count;   // run length (valid in the head lane)
+  int rank;    // position of this lane in its run
+};
+__device__ inline KeyRun key_run(uint32_t key) {
+  const int lane = __lane_id();
+  const uint32_t prev = __shfl_up(key, 1);
+  const bool head = lane == 0 || prev != key;
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long le = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+  const int h = 63 - __clzll(heads & le);
+  const unsigned long long gt = heads & ~le;
+  const int nxt = gt ? __ffsll((long long)gt) - 1 : 64;
+  return {h, nxt - h, lane - h};
+}
+
 // Particles::StepInGrid (particles.cc:504-543) + DeleteParticlesInSolids
 // (particles.cc:202-219). Particles outside the partition box were deleted by
 // DeleteOutsideParticles after the previous P2G; they are dropped here.
@@ -70,39 +90,49 @@ __global__ void k_advect(Dom d, int64_t np, Particles P, float dt, const uint8_t
                          uint32_t *key, uint32_t *vcount, int drop_outside) {
 #pragma clang fp contract(off)
   const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (q >= np) return;
-  float p[3] = {P.a[0][q], P.a[1][q], P.a[2][q]};
-  if (drop_outside) {
-    const int a = int(floorf(p[0])), b = int(floorf(p[1])), c = int(floorf(p[2]));
-    if (!in_bbox(d, a, b, c)) { key[q] = kDead; return; }
-  }
-  int id[3];
+  uint32_t k = kDead;
+  if (q < np) {
+    float p[3] = {P.a[0][q], P.a[1][q], P.a[2][q]};
+    bool alive = true;
+    if (drop_outside) {
+      const int a = int(floorf(p[0])), b = int(floorf(p[1])), c = int(floorf(p[2]));
+      if (!in_bbox(d, a, b, c)) alive = false;
+    }
+    if (alive) {
+      int id[3];
 #pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const float m = P.a[6 + a][q] * dt;
-    p[a] = p[a] + m;
-    id[a] = int(floorf(p[a]));
-    if (id[a] < 0) { id[a] = 0; p[a] = 0.0f; }
-  }
+      for (int a = 0; a < 3; ++a) {
+        const float m = P.a[6 + a][q] * dt;
+        p[a] = p[a] + m;
+        id[a] = int(floorf(p[a]));
+        if (id[a] < 0) { id[a] = 0; p[a] = 0.0f; }
+      }
 #pragma unroll
-  for (int a = 0; a < 3; ++a)
-    if (id[a] >= d.n[a]) { id[a] = d.n[a]; p[a] = float(d.n[a]); }
-  P.a[0][q] = p[0]; P.a[1][q] = p[1]; P.a[2][q] = p[2];
-  const int64_t v = vox_of(d, id[0], id[1], id[2]);
-  if (solid[v]) { key[q] = kDead; return; }
-  key[q] = uint32_t(v);
-  atomicAdd(&vcount[v], 1u);
+      for (int a = 0; a < 3; ++a)
+        if (id[a] >= d.n[a]) { id[a] = d.n[a]; p[a] = float(d.n[a]); }
+      P.a[0][q] = p[0]; P.a[1][q] = p[1]; P.a[2][q] = p[2];
+      const int64_t v = vox_of(d, id[0], id[1], id[2]);
+      if (!solid[v]) k = uint32_t(v);
+    }
+    key[q] = k;
+  }
+  const KeyRun r = key_run(k);
+  if (r.rank == 0 && k != kDead) atomicAdd(&vcount[k], uint32_t(r.count));
 }
 
 // Counting-sort scatter. vcount returns to zero, ready for the next step.
+// A run of equal keys takes one atomic; its lanes fill the slots the old
+// per-particle order would have filled (descending from the counter).
 __global__ void k_scatter(int64_t np, const uint32_t *key, const uint32_t *vstart,
                           uint32_t *vcount, uint32_t *perm) {
   const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (q >= np) return;
-  const uint32_t k = key[q];
+  const uint32_t k = q < np ? key[q] : kDead;
+  const KeyRun r = key_run(k);
+  uint32_t base = 0;
+  if (r.rank == 0 && k != kDead) base = atomicSub(&vcount[k], uint32_t(r.count));
+  base = __shfl(base, r.head);
   if (k == kDead) return;
-  const uint32_t slot = vstart[k] + atomicSub(&vcount[k], 1u) - 1u;
-  perm[slot] = uint32_t(q);
+  perm[vstart[k] + base - 1u - uint32_t(r.rank)] = uint32_t(q);
 }
 
 // Restore the previous order inside each cell (stable sort): the thread at a

@@ -173,6 +173,42 @@ __device__ inline double total(const A *partials, int n, int k) {
   }
 }
 
+// Totals of both values of the n published (stride 2) partials in one pass:
+// one load round and one combined workgroup reduction, in A's precision
+// (Neumaier-compensated for double), the same in every workgroup.
+template <typename A>
+__device__ inline void total2(const A *partials, int n, double out[2]) {
+  __shared__ A s_t[2][16];
+  A v0, v1;
+  if constexpr (sizeof(A) == 8) {
+    double a0 = 0.0, c0 = 0.0, a1 = 0.0, c1 = 0.0;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) {
+      neumaier_add(a0, c0, double(partials[2 * q]));
+      neumaier_add(a1, c1, double(partials[2 * q + 1]));
+    }
+    v0 = a0 + c0;
+    v1 = a1 + c1;
+  } else {
+    v0 = A(0);
+    v1 = A(0);
+    for (int q = threadIdx.x; q < n; q += blockDim.x) {
+      v0 += partials[2 * q];
+      v1 += partials[2 * q + 1];
+    }
+  }
+  v0 = wave_sum(v0);
+  v1 = wave_sum(v1);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nw = (blockDim.x + 63) >> 6;
+  if (lane == 0) { s_t[0][wave] = v0; s_t[1][wave] = v1; }
+  __syncthreads();
+  A t0 = A(0), t1 = A(0);
+  for (int w = 0; w < nw; ++w) { t0 += s_t[0][w]; t1 += s_t[1][w]; }
+  out[0] = double(t0);
+  out[1] = double(t1);
+}
+
+
 __device__ inline bool converged(const PcgState *st, double rr) {
   // Relative mode uses <=, so a zero right-hand side (x0 = 0) stops at once.
   return st->stop_mode == FSL_STOP_ABS_RR ? rr < st->thr : sqrt(rr) <= st->thr;
@@ -320,28 +356,6 @@ __global__ __launch_bounds__(kT) void k_pass_a(Dom d, const int32_t *flist, int6
   __shared__ V s_p[2][kGroups][1000];
   if (st->done) return;
   V beta = V(0);
-  if (k > 0) {
-    const double rr = total<A, 2>(part_b1, g_parts, 0);
-    const A rz = A(total<A, 2>(part_b1, g_parts, 1));
-    const A rz_old = A(st->rzh[(k - 1) & 1]);
-    const A b = rz / rz_old;
-    int status = 0;
-    if (isnan(rr)) status = FSL_ERR_NAN;
-    else if (!(double(b) <= 1e5)) status = FSL_ERR_BETA;  // CHECK_LE(beta, LARGE_FLOAT), solver.cc:361
-    const bool stop = status != 0 || converged(st, rr) || k >= st->max_iter;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      st->it = k;
-      st->rr = rr;
-      st->rz = double(rz);
-      st->rz_old = double(rz_old);
-      st->beta = double(b);
-      st->rzh[k & 1] = double(rz);
-      if (status) st->status = status;
-      if (stop) st->done = 1;
-    }
-    if (stop) return;
-    beta = V(b);
-  }
   const int g = threadIdx.x / kG, t = gtid();
   A acc = A(0);
   auto pval = [&](V zz, V pp) {
@@ -386,9 +400,35 @@ __global__ __launch_bounds__(kT) void k_pass_a(Dom d, const int32_t *flist, int6
       }
     }
   };
+  // The first leaf's loads do not depend on beta: issue them before the
+  // scalar reduction, so the reduction runs under their latency.
   Walk q = walk(nf);
   Pre cur;
   if (q.more()) load(q, cur);
+  if (k > 0) {
+    double tot[2];
+    total2<A>(part_b1, g_parts, tot);
+    const double rr = tot[0];
+    const A rz = A(tot[1]);
+    const A rz_old = A(st->rzh[(k - 1) & 1]);
+    const A b = rz / rz_old;
+    int status = 0;
+    if (isnan(rr)) status = FSL_ERR_NAN;
+    else if (!(double(b) <= 1e5)) status = FSL_ERR_BETA;  // CHECK_LE(beta, LARGE_FLOAT), solver.cc:361
+    const bool stop = status != 0 || converged(st, rr) || k >= st->max_iter;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->it = k;
+      st->rr = rr;
+      st->rz = double(rz);
+      st->rz_old = double(rz_old);
+      st->beta = double(b);
+      st->rzh[k & 1] = double(rz);
+      if (status) st->status = status;
+      if (stop) st->done = 1;
+    }
+    if (stop) return;
+    beta = V(b);
+  }
   for (int j = 0; q.more(); ++j) {
     V *tile = s_p[j & 1][g];
     if (cur.valid) {
@@ -432,21 +472,6 @@ __global__ __launch_bounds__(kT) void k_pass_b(const int32_t *flist, int64_t nf,
                                                int g_parts, int k, PcgState *st) {
 #pragma clang fp contract(off)
   if (st->done) return;
-  const A pAp = A(total<A, 1>(part_a, g_parts, 0));
-  const A rz = A(st->rzh[k & 1]);
-  const A alpha_a = rz / pAp;
-  if (!(double(alpha_a) <= 1e5)) {  // CHECK_LE(alpha, LARGE_FLOAT), solver.cc:333
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      st->status = FSL_ERR_ALPHA;
-      st->done = 1;
-    }
-    return;
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    st->pAp = double(pAp);
-    st->alpha = double(alpha_a);
-  }
-  const V alpha = V(alpha_a);
   A acc[2] = {A(0), A(0)};
   // Software pipeline: round j+1's loads are issued before round j computes.
   struct Pre {
@@ -469,10 +494,26 @@ __global__ __launch_bounds__(kT) void k_pass_b(const int32_t *flist, int64_t nf,
       c.r[s] = r[v];
     }
   };
+  // Issue the first leaf's loads, then reduce pAp under their latency.
   Walk q = walk(nf);
   Pre cur;
   cur.valid = false;
   if (q.more()) load(q, cur);
+  const A pAp = A(total<A, 1>(part_a, g_parts, 0));
+  const A rz = A(st->rzh[k & 1]);
+  const A alpha_a = rz / pAp;
+  if (!(double(alpha_a) <= 1e5)) {  // CHECK_LE(alpha, LARGE_FLOAT), solver.cc:333
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->status = FSL_ERR_ALPHA;
+      st->done = 1;
+    }
+    return;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->pAp = double(pAp);
+    st->alpha = double(alpha_a);
+  }
+  const V alpha = V(alpha_a);
   while (q.more()) {
     const Pre c = cur;
     q.next();

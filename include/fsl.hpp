@@ -25,6 +25,7 @@
 #ifndef FSL_HPP_
 #define FSL_HPP_
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -253,6 +254,67 @@ class FlipSimulation {
       }
     }
     std::fclose(fp);
+  }
+
+  /* VectorGrid::Write / ScalarGrid::Write, text mode (vector_grid.cc:133-160,
+   * scalar_grid.cc:134-160): start and end of the partition, then
+   * "i j k : values" for every active voxel in OpenVDB iteration order
+   * (64^3 internal nodes by origin, then 8^3 leaves, then voxels, each
+   * x-major), restricted to [start, end] (to end + 1 for the staggered
+   * velocity grids).
+   *   which = FSL_GRID_VELOCITY / FSL_GRID_VELOCITY_UPDATE: vector grids;
+   *   which = 10 + FSL_GRID_DIVERGENCE / FSL_GRID_PRESSURE / FSL_GRID_MARKER:
+   *   scalar grids (active where the reference sets them: FLUID cells for
+   *   divergence and pressure, FLUID or SOLID cells for the marker). */
+  void WriteGrid(const std::string &file_name, int which) {
+    const int by = dims_.y + 2, bz = dims_.z + 2;
+    const bool vec = which < 10;
+    std::vector<float> vals;
+    std::vector<uint8_t> act;
+    std::vector<int32_t> marker = GetInt(FSL_GRID_MARKER);
+    if (vec) {
+      vals = GetVelocity(which, &act);
+    } else if (which - 10 == FSL_GRID_MARKER) {
+      vals.assign(marker.begin(), marker.end());
+    } else {
+      vals = GetScalar(which - 10);
+    }
+    const int ext = vec ? 1 : 0;
+    std::vector<std::pair<uint64_t, size_t>> order;
+    for (int i = 0; i <= dims_.x - 1 + ext; ++i)
+      for (int j = 0; j <= dims_.y - 1 + ext; ++j)
+        for (int k = 0; k <= dims_.z - 1 + ext; ++k) {
+          const size_t q = (size_t(i + 1) * by + (j + 1)) * bz + (k + 1);
+          bool on;
+          if (vec) on = act[q] != 0;
+          else if (which - 10 == FSL_GRID_MARKER) on = marker[q] == 1 || marker[q] == 2;
+          else on = marker[q] == 1;
+          if (on) order.push_back({vdb_key(i, j, k), q});
+        }
+    std::sort(order.begin(), order.end());
+    FILE *fp = std::fopen(file_name.c_str(), "wt");
+    if (!fp) throw Error(FSL_ERR_ARG, "could not create " + file_name);
+    std::fprintf(fp, "%d %d %d\n%d %d %d\n", 0, 0, 0, dims_.x - 1, dims_.y - 1, dims_.z - 1);
+    for (const auto &e : order) {
+      const size_t q = e.second;
+      const int i = int(q / (size_t(by) * bz)) - 1, j = int((q / bz) % by) - 1,
+                k = int(q % bz) - 1;
+      if (vec)
+        std::fprintf(fp, "%d %d %d : %f %f %f\n", i, j, k, vals[3 * q], vals[3 * q + 1],
+                     vals[3 * q + 2]);
+      else
+        std::fprintf(fp, "%d %d %d : %f\n", i, j, k, vals[q]);
+    }
+    std::fclose(fp);
+  }
+
+  /* OpenVDB Tree3<T,3,3> value-iteration key of voxel (i, j, k). */
+  static uint64_t vdb_key(int i, int j, int k) {
+    const uint64_t ri = uint64_t((i >> 6) + 1024), rj = uint64_t((j >> 6) + 1024),
+                   rk = uint64_t((k >> 6) + 1024);
+    const uint64_t leaf = uint64_t(((i >> 3) & 7) << 6 | ((j >> 3) & 7) << 3 | ((k >> 3) & 7));
+    const uint64_t vox = uint64_t((i & 7) << 6 | (j & 7) << 3 | (k & 7));
+    return (((ri * 2048 + rj) * 2048 + rk) << 18) | (leaf << 9) | vox;
   }
 
   Coord global_dims() const { return dims_; }

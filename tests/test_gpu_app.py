@@ -58,3 +58,75 @@ def test_water_app_matches_python_path(tmp_path):
     assert steps >= 1
     assert pos.shape[0] == rec.shape[0]
     assert np.array_equal(pos, rec["pos"]) and np.array_equal(vel, rec["vel"])
+
+
+def vdb_key(i, j, k):
+    """OpenVDB Tree3<T,3,3> value-iteration order (root by origin, then leaf,
+    then voxel offsets, each x-major)."""
+    i, j, k = (np.asarray(a, np.int64) for a in (i, j, k))
+    r = (((i >> 6) + 1024) * 2048 + ((j >> 6) + 1024)) * 2048 + ((k >> 6) + 1024)
+    leaf = (((i >> 3) & 7) << 6) | (((j >> 3) & 7) << 3) | ((k >> 3) & 7)
+    vox = ((i & 7) << 6) | ((j & 7) << 3) | (k & 7)
+    return (r << 18) | (leaf << 9) | vox
+
+
+def read_grid(path, ncomp):
+    lines = open(path).read().splitlines()
+    start = [int(x) for x in lines[0].split()]
+    end = [int(x) for x in lines[1].split()]
+    ijk, vals = [], []
+    for ln in lines[2:]:
+        a, b = ln.split(":")
+        ijk.append([int(x) for x in a.split()])
+        vals.append([float(x) for x in b.split()])
+    return start, end, np.array(ijk, np.int64).reshape(-1, 3), np.array(vals).reshape(-1, ncomp)
+
+
+def test_water_app_grid_dumps(tmp_path):
+    """Text grid dumps (vector_grid.cc:133-160, scalar_grid.cc:134-160): active
+    voxels in OpenVDB iteration order, values printed with %f."""
+    n = 32
+    gout = str(tmp_path / "grid_")
+    r = subprocess.run([APP, f"n={n}", "scene=sphere_drop", "frames=1", "seed=1",
+                        f"grid_output={gout}"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    dims = (n, n, n)
+    mask = scenes.sphere_drop_mask(dims, 8, (16, 23, 16), 5)
+    ctx = fsl.Context(dims, capacity=int(mask.sum()) * 8 + 64)
+    ctx.set_params(9.8, (0.0, 9.8, 0.0), 0.95)
+    ctx.set_box_boundary()
+    ctx.seed_particles(mask, 8, 1)
+    prm = fsl.solve_params(dims=dims)
+    frame = np.float32(1.0 / 30)
+    step = np.float32(0.0)
+    while np.float32(step + np.float32(1e-5)) < frame:
+        dt, step = ctx.compute_dt(1, frame, step)
+        step = np.float32(step)
+        ctx.step(dt, prm)
+    v, act = ctx.get_vec(fsl.GRID_VELOCITY)
+    marker = ctx.get_int(fsl.GRID_MARKER)
+    pres = ctx.get_scalar(fsl.GRID_PRESSURE)
+    ctx.close()
+    # velocity: active faces over [0, n] per axis
+    start, end, ijk, vals = read_grid(gout + "velocity_00000001", 3)
+    assert start == [0, 0, 0] and end == [n - 1] * 3
+    sub = act[1:n + 2, 1:n + 2, 1:n + 2]
+    want = np.argwhere(sub)
+    order = np.argsort(vdb_key(want[:, 0], want[:, 1], want[:, 2]), kind="stable")
+    want = want[order]
+    assert np.array_equal(ijk, want)
+    ref = v[want[:, 0] + 1, want[:, 1] + 1, want[:, 2] + 1]
+    assert np.allclose(vals, ref.astype(np.float64), rtol=0, atol=5e-7)  # printed with %f
+    # pressure: FLUID cells
+    _, _, ijk, vals = read_grid(gout + "pressure_00000001", 1)
+    fl = np.argwhere(marker[1:n + 1, 1:n + 1, 1:n + 1] == 1)
+    fl = fl[np.argsort(vdb_key(fl[:, 0], fl[:, 1], fl[:, 2]), kind="stable")]
+    assert np.array_equal(ijk, fl)
+    assert np.allclose(vals[:, 0], pres[fl[:, 0] + 1, fl[:, 1] + 1, fl[:, 2] + 1], atol=5e-7)
+    # marker: FLUID or SOLID cells
+    _, _, ijk, vals = read_grid(gout + "marker_00000001", 1)
+    ms = marker[1:n + 1, 1:n + 1, 1:n + 1]
+    on = np.argwhere((ms == 1) | (ms == 2))
+    on = on[np.argsort(vdb_key(on[:, 0], on[:, 1], on[:, 2]), kind="stable")]
+    assert np.array_equal(ijk, on)
+    assert np.array_equal(vals[:, 0], ms[on[:, 0], on[:, 1], on[:, 2]].astype(np.float64))
