@@ -34,6 +34,7 @@
 #include <hip/hip_ext.h>
 
 #include <algorithm>
+#include <cmath>
 #include <type_traits>
 
 #include "device_util.h"
@@ -850,6 +851,13 @@ int pcg_run(fsl_ctx *ctx, const fsl_solve_params *prm, fsl_solve_stats *st) {
   int rc = pcg_prepare(ctx, prm);
   if (rc) return rc;
   const int batch = prm->batch > 0 ? prm->batch : 16;
+  // Batch sizing: after each host check, the next batch covers most of the
+  // iterations the observed residual decay predicts are still needed, so a
+  // solve takes a few host round trips instead of one per `batch`.
+  // Iterations launched past convergence exit at once (the done flag).
+  const bool adaptive = prm->batch <= 0;
+  double rr_prev = ctx->pcg_host->rr;
+  int k_prev = 0;
   hipEvent_t e0, e1;
   FSL_HIP(hipEventCreate(&e0));
   FSL_HIP(hipEventCreate(&e1));
@@ -857,7 +865,8 @@ int pcg_run(fsl_ctx *ctx, const fsl_solve_params *prm, fsl_solve_stats *st) {
   const bool mixed = ctx->pcg_precision == FSL_PCG_MIXED;
   // With the measurement hook on, each pass records its own start and end
   // events (at most 50 iterations per batch, four events per iteration).
-  const int nb = ctx->timing ? std::min(batch, 50) : batch;
+  const int nb_cap = ctx->timing ? 50 : 1024;
+  int nb = std::min(batch, nb_cap);
   hipEvent_t *pev = ctx->tev + 32;
   int k = 0;  // next iteration to launch
   while (!ctx->pcg_host->done) {
@@ -895,6 +904,16 @@ int pcg_run(fsl_ctx *ctx, const fsl_solve_params *prm, fsl_solve_stats *st) {
     }
     if (h->status) break;
     if (!h->done) {
+      if (adaptive && k > k_prev && h->rr > 0.0 && rr_prev > 0.0) {
+        const double rate = std::log(h->rr / rr_prev) / double(k - k_prev);
+        const double target = h->stop_mode == FSL_STOP_REL ? h->thr * h->thr : h->thr;
+        if (rate < 0.0 && target > 0.0) {
+          const double need = std::log(target / h->rr) / rate;
+          nb = int(std::min(double(nb_cap), std::max(16.0, 0.8 * need)));
+        }
+      }
+      rr_prev = h->rr;
+      k_prev = k;
       // A(k) ran for real: the next batch starts at B(k).
       dispatch(ctx->pcg_precision, [&](auto K) { decltype(K)::pass_b(ctx, k); });
       if (mixed && (k + 1) % ctx->pcg_refresh == 0) launch_refresh(ctx, k, 0);

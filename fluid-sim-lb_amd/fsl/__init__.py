@@ -137,7 +137,7 @@ def _ptr(a: np.ndarray | None):
 
 
 def solve_params(precision=PCG_FP32, stop_mode=STOP_ABS_RR, threshold=None, dims=None,
-                 max_iterations=1500, refresh_every=32, batch=16) -> SolveParams:
+                 max_iterations=1500, refresh_every=32, batch=0) -> SolveParams:
     """Solver parameters; by default the reference's rule r.r < N^3 * 0.5e-8
     (flip_app.cc:326-328, flip_app.h:52-53)."""
     if threshold is None:

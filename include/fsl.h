@@ -68,7 +68,8 @@ typedef struct fsl_solve_params {
   double threshold;   /* absolute r.r bound, or relative ||r||/||b|| bound */
   int max_iterations; /* reference default 1500 (flip_app.h:52) */
   int refresh_every;  /* MIXED: FP64 residual refresh period (0 = 32) */
-  int batch;          /* iterations launched between host checks (0 = 16) */
+  int batch;          /* iterations launched between host checks; 0: sized from
+                         the observed residual decay (first batch 16) */
   int warm_start;     /* 0: x0 = 0 (the cleared pressure grid, solver.cc:86-88);
                          1: x0 = the current pressure grid */
 } fsl_solve_params;
