@@ -77,6 +77,30 @@ __device__ inline void halo_cell(int h, int &x, int &y, int &z) {
   else { x = a; y = b; z = s; }
 }
 
+// Halo slot s of thread t in a 256-thread leaf group: slot 0 covers the
+// x and y faces (one face per wave), slot 1 the two z faces spread over the
+// first 32 lanes of every wave, so each wave carries the same halo work.
+// Returns -1 when the slot is idle.
+__device__ inline int halo_of(int t, int s) {
+  if (s == 0) return t;
+  const int lane = t & 63;
+  return lane < 32 ? 256 + (t >> 6) * 32 + lane : -1;
+}
+// Voxel of halo cell h of `leaf`: the neighbour leaf across face h / 64 and
+// the mirrored layer inside it.
+__device__ inline int64_t halo_voxel(const Dom &d, int64_t leaf, int h) {
+  const int f = h >> 6, a = (h >> 3) & 7, b = h & 7;
+  const int64_t l12 = int64_t(d.L[1]) * d.L[2];
+  const int64_t step = f < 2 ? l12 : (f < 4 ? int64_t(d.L[2]) : 1);
+  const int64_t nb = (f & 1) ? leaf + step : leaf - step;
+  const int layer = (f & 1) ? 0 : 7;
+  int off;
+  if (f < 2) off = layer << 6 | a << 3 | b;
+  else if (f < 4) off = a << 6 | layer << 3 | b;
+  else off = a << 6 | b << 3 | layer;
+  return (nb << 9) | off;
+}
+
 // Apply the operator at LDS index c.
 template <typename V, bool ORD>
 __device__ inline V stencil(const V *sp, const uint8_t *sf, int c, int diag) {
@@ -387,14 +411,14 @@ __global__ __launch_bounds__(kT) void k_pass_a(Dom d, const int32_t *flist, int6
       x.ol[s] = lab[v];
       x.oz[s] = zv[v];
       x.op[s] = k ? pin[v] : V(0);
-      const int h = t + s * kG;
+      const int h = halo_of(t, s);
       x.hz[s] = V(0);
       x.hp[s] = V(0);
-      x.hix[s] = 0;
-      if (h < 384) {
+      x.hix[s] = -1;
+      if (h >= 0) {
         int hx, hy, hz_;
         halo_cell(h, hx, hy, hz_);
-        const int64_t hv = vox_of(d, o[0] + hx, o[1] + hy, o[2] + hz_);
+        const int64_t hv = halo_voxel(d, x.leaf, h);
         x.hz[s] = zv[hv];
         x.hp[s] = k ? pin[hv] : V(0);
         x.hix[s] = tix(hx, hy, hz_);
@@ -436,7 +460,7 @@ __global__ __launch_bounds__(kT) void k_pass_a(Dom d, const int32_t *flist, int6
 #pragma unroll
       for (int s = 0; s < 2; ++s) {
         tile[tix_of_local(t + s * kG)] = pval(cur.oz[s], cur.op[s]);
-        if (t + s * kG < 384) tile[cur.hix[s]] = pval(cur.hz[s], cur.hp[s]);
+        if (cur.hix[s] >= 0) tile[cur.hix[s]] = pval(cur.hz[s], cur.hp[s]);
       }
     }
     const bool valid = cur.valid;
