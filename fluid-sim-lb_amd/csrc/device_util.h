@@ -22,6 +22,12 @@ __device__ inline int64_t xcd_work(int64_t nwork) {
   return int64_t(blockIdx.x % kXcd) * chunk + blockIdx.x / kXcd;
 }
 
+// Fused multiply-add in the operands' own precision (__builtin_fma is the
+// double builtin: float operands would be promoted, multiplied and added in
+// double, then rounded back).
+__device__ inline float fmav(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ inline double fmav(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 template <typename T>
 __device__ inline T wave_sum(T v) {
 #pragma unroll

@@ -69,6 +69,7 @@ struct PcgState {
   double rz, rz_old, rr, pAp, alpha, beta;
   double bnorm2, thr, rr_true;
   double rzh[2];  // rz of iteration k in slot k & 1
+  double alph[2]; // alpha of iteration k in slot k & 1 (single-reduction CG)
   int it, done, status, max_iter;
   int stop_mode, precision, refresh_due, pad;
   unsigned cnt_a, cnt_b, cnt_r, cnt_x;

@@ -49,17 +49,20 @@ constexpr int kGroups = 4;          // leaves in flight per workgroup
 constexpr int kT = kG * kGroups;    // threads per workgroup
 constexpr int kBlocksPerXcd = 64;   // 2 workgroups per CU, 32 CUs per XCD
 
+// 1 / diag for diag in 0..6 (0 for 0). JacobiPreconditioner stores
+// float(1.0 / diag) for float vectors. The value is selected in registers: a
+// table indexed at run time would live in global memory, and each lookup
+// would wait for every load in flight, including the next leaf's prefetch.
 template <typename V>
-__device__ inline V dinv(int diag);
-template <>
-__device__ inline float dinv<float>(int diag) {
-  // JacobiPreconditioner stores float(1.0 / diag); diag is 1..6.
-  constexpr float t[8] = {0.0f, 1.0f, 0.5f, float(1.0 / 3.0), 0.25f, 0.2f, float(1.0 / 6.0), 0.0f};
-  return t[diag & 7];
-}
-template <>
-__device__ inline double dinv<double>(int diag) {
-  return diag ? 1.0 / double(diag) : 0.0;
+__device__ inline V dinv(int diag) {
+  const V r3 = V(1.0 / 3.0), r5 = V(1.0 / 5.0), r6 = V(1.0 / 6.0);
+  V v = diag == 1 ? V(1) : V(0);
+  v = diag == 2 ? V(0.5) : v;
+  v = diag == 3 ? r3 : v;
+  v = diag == 4 ? V(0.25) : v;
+  v = diag == 5 ? r5 : v;
+  v = diag == 6 ? r6 : v;
+  return v;
 }
 
 __device__ inline bool is_fluid(uint8_t lab) { return (lab & 3) == kFluid; }
@@ -125,7 +128,7 @@ __device__ inline V stencil(const V *sp, const uint8_t *sf, int c, int diag) {
     nb += sf[c + 10] ? sp[c + 10] : V(0);
     nb += sf[c - 1] ? sp[c - 1] : V(0);
     nb += sf[c + 1] ? sp[c + 1] : V(0);
-    return __builtin_fma(V(diag), sp[c], -nb);
+    return fmav(V(diag), sp[c], -nb);
   }
 }
 
@@ -147,25 +150,27 @@ __device__ inline V stencil0(const V *sp, int c, int diag) {
     return s;
   } else {
     const V nb = ((sp[c - 100] + sp[c + 100]) + (sp[c - 10] + sp[c + 10])) + (sp[c - 1] + sp[c + 1]);
-    return __builtin_fma(V(diag), sp[c], -nb);
+    return fmav(V(diag), sp[c], -nb);
   }
 }
 
 // Persistent walk over the leaf list: XCD x = blockIdx % 8 owns the list
-// chunk x; its workgroups stride through it, kGroups leaves per round (leaf
-// group g = threadIdx / kG takes every kGroups-th leaf of the workgroup).
+// chunk x; its workgroups stride through it, blockDim / kG leaves per round
+// (leaf group g = threadIdx / kG takes every (blockDim / kG)-th leaf of the
+// workgroup).
 struct Walk {
-  int64_t w, end, step;
+  int64_t w, end, step, ng;
   __device__ bool more() const { return w < end; }
   __device__ int64_t mine() const { return w + int64_t(threadIdx.x / kG) * step; }
   __device__ bool valid() const { return mine() < end; }
-  __device__ void next() { w += int64_t(kGroups) * step; }
+  __device__ void next() { w += ng * step; }
 };
 __device__ inline Walk walk(int64_t nf) {
   const int64_t per = gridDim.x / kXcd;
   const int64_t chunk = (nf + kXcd - 1) / kXcd;
   const int64_t beg = int64_t(blockIdx.x % kXcd) * chunk;
-  return {beg + blockIdx.x / kXcd, beg + chunk < nf ? beg + chunk : nf, per};
+  return {beg + blockIdx.x / kXcd, beg + chunk < nf ? beg + chunk : nf, per,
+          int64_t(blockDim.x / kG)};
 }
 __device__ inline int gtid() { return threadIdx.x % kG; }
 
@@ -198,41 +203,47 @@ __device__ inline double total(const A *partials, int n, int k) {
   }
 }
 
-// Totals of both values of the n published (stride 2) partials in one pass:
+// Totals of all K values of the n published (stride K) partials in one pass:
 // one load round and one combined workgroup reduction, in A's precision
 // (Neumaier-compensated for double), the same in every workgroup.
-template <typename A>
-__device__ inline void total2(const A *partials, int n, double out[2]) {
-  __shared__ A s_t[2][16];
-  A v0, v1;
+template <typename A, int K>
+__device__ inline void total_all(const A *partials, int n, double out[K]) {
+  __shared__ A s_t[K][16];
+  A v[K];
   if constexpr (sizeof(A) == 8) {
-    double a0 = 0.0, c0 = 0.0, a1 = 0.0, c1 = 0.0;
+    double a[K], c[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) { a[j] = 0.0; c[j] = 0.0; }
     for (int q = threadIdx.x; q < n; q += blockDim.x) {
-      neumaier_add(a0, c0, double(partials[2 * q]));
-      neumaier_add(a1, c1, double(partials[2 * q + 1]));
+#pragma unroll
+      for (int j = 0; j < K; ++j) neumaier_add(a[j], c[j], double(partials[K * q + j]));
     }
-    v0 = a0 + c0;
-    v1 = a1 + c1;
+#pragma unroll
+    for (int j = 0; j < K; ++j) v[j] = a[j] + c[j];
   } else {
-    v0 = A(0);
-    v1 = A(0);
+#pragma unroll
+    for (int j = 0; j < K; ++j) v[j] = A(0);
     for (int q = threadIdx.x; q < n; q += blockDim.x) {
-      v0 += partials[2 * q];
-      v1 += partials[2 * q + 1];
+#pragma unroll
+      for (int j = 0; j < K; ++j) v[j] += partials[K * q + j];
     }
   }
-  v0 = wave_sum(v0);
-  v1 = wave_sum(v1);
+#pragma unroll
+  for (int j = 0; j < K; ++j) v[j] = wave_sum(v[j]);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nw = (blockDim.x + 63) >> 6;
-  if (lane == 0) { s_t[0][wave] = v0; s_t[1][wave] = v1; }
+  if (lane == 0) {
+#pragma unroll
+    for (int j = 0; j < K; ++j) s_t[j][wave] = v[j];
+  }
   __syncthreads();
-  A t0 = A(0), t1 = A(0);
-  for (int w = 0; w < nw; ++w) { t0 += s_t[0][w]; t1 += s_t[1][w]; }
-  out[0] = double(t0);
-  out[1] = double(t1);
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    A t = A(0);
+    for (int w = 0; w < nw; ++w) t += s_t[j][w];
+    out[j] = double(t);
+  }
 }
-
 
 __device__ inline bool converged(const PcgState *st, double rr) {
   // Relative mode uses <=, so a zero right-hand side (x0 = 0) stops at once.
@@ -321,7 +332,7 @@ __global__ __launch_bounds__(kT) void k_prepare(Dom d, const int32_t *flist, int
       else x[v] = V(x0);
       r[v] = rv;
       const V z = rv * dinv<V>(ol[s] >> 2);
-      zv[v] = z;
+      if (zv) zv[v] = z;
       acc[0] += A(rv) * A(rv);
       acc[1] += A(rv) * A(z);
       acc[2] += A(b) * A(b);
@@ -432,7 +443,7 @@ __global__ __launch_bounds__(kT) void k_pass_a(Dom d, const int32_t *flist, int6
   if (q.more()) load(q, cur);
   if (k > 0) {
     double tot[2];
-    total2<A>(part_b1, g_parts, tot);
+    total_all<A, 2>(part_b1, g_parts, tot);
     const double rr = tot[0];
     const A rz = A(tot[1]);
     const A rz_old = A(st->rzh[(k - 1) & 1]);
@@ -556,8 +567,8 @@ __global__ __launch_bounds__(kT) void k_pass_b(const int32_t *flist, int64_t nf,
         const V m2 = (-alpha) * c.a[s];
         rn = m2 + c.r[s];
       } else {
-        xn = __builtin_fma(alpha, c.p[s], c.x[s]);
-        rn = __builtin_fma(-alpha, c.a[s], c.r[s]);
+        xn = fmav(alpha, c.p[s], c.x[s]);
+        rn = fmav(-alpha, c.a[s], c.r[s]);
       }
       x[v] = xn;
       r[v] = rn;
@@ -619,13 +630,429 @@ __global__ __launch_bounds__(kT) void k_refresh_r(Dom d, const int32_t *flist, i
       const float rf = float(rt);
       const float zf = rf * dinv<float>(ol[s] >> 2);
       r[v] = rf;
-      zv[v] = zf;
+      if (zv) zv[v] = zf;
       acc[0] += double(rf) * double(rf);
       acc[1] += double(rf) * double(zf);
     }
     __syncthreads();
   }
-  publish<double, 2>(acc, part_b);
+  if (part_b) publish<double, 2>(acc, part_b);
+}
+
+// ------------------------------------------------------------------------
+// Single-reduction iteration. One kernel per iteration k, given r_k, p_{k-1},
+// x_k and the totals (gamma_k, delta_k, r.r_k) published by the previous
+// kernel, with gamma = r.u, delta = u.Au, u = M^-1 r:
+//   beta_k  = gamma_k / gamma_{k-1}
+//   alpha_k = gamma_k / (delta_k - beta_k gamma_k / alpha_{k-1})
+//   p_k = u_k + beta_k p_{k-1};  x_{k+1} = x_k + alpha_k p_k;
+//   r_{k+1} = r_k - alpha_k A p_k;  u_{k+1} = M^-1 r_{k+1};
+//   publish gamma_{k+1}, delta_{k+1} = u_{k+1}.A u_{k+1}, r.r_{k+1}.
+// These are the iterates of the two-pass PCG (solver.cc:296-366) in exact
+// arithmetic: p.Ap = delta_k - beta_k gamma_k / alpha_{k-1} follows from the
+// M-orthogonality of the residuals (Chronopoulos and Gear, J. Comput. Appl.
+// Math. 25, 1989). A p_k is computed directly (no recurrence for A p), so the
+// residual recurrence is the two-pass one.
+//
+// To apply A to u_{k+1} on a leaf, u_{k+1} is needed on the leaf and its six
+// face layers, hence r_{k+1}, hence A p_k there, hence p_k on the leaf, its
+// face layers at distance 1 and 2, and the twelve edge rows at distance
+// (1,1): 1376 cells, staged in a 12^3 LDS tile. r_k and p_{k-1} are read on
+// those cells and p_k recomputed (the halo cells are recomputed, not
+// exchanged). r and p are double-buffered by iteration parity, as other
+// workgroups read the halo values of the iteration's inputs.
+
+// 12^3 tile over [-2, 10)^3 of the leaf and 10^3 tile over [-1, 9)^3.
+__device__ inline int t12(int x, int y, int z) { return ((x + 2) * 12 + (y + 2)) * 12 + (z + 2); }
+__device__ inline int t12_of_local(int c) { return t12(c >> 6, (c >> 3) & 7, c & 7); }
+
+// Halo cell m of the fused pass, relative to the leaf origin. m in [0, 384):
+// face layers at distance 1 (face m/64: x-, x+, y-, y+, z-, z+); m in
+// [384, 768): face layers at distance 2; m in [768, 864): edge rows at
+// distance (1, 1) (axis pairs xy, xz, yz, each with signs --, -+, +-, ++;
+// the free axis runs over 0..7).
+__device__ inline void fhalo_cell(int m, int &x, int &y, int &z) {
+  if (m < 768) {
+    const int near = m < 384;
+    const int h = near ? m : m - 384;
+    const int f = h >> 6, a = (h >> 3) & 7, b = h & 7;
+    const int s = (f & 1) ? (near ? 8 : 9) : (near ? -1 : -2);
+    if (f < 2) { x = s; y = a; z = b; }
+    else if (f < 4) { x = a; y = s; z = b; }
+    else { x = a; y = b; z = s; }
+  } else {
+    const int e = (m - 768) >> 3, t = m & 7;
+    const int pair = e >> 2, s1 = (e & 2) ? 8 : -1, s2 = (e & 1) ? 8 : -1;
+    if (pair == 0) { x = s1; y = s2; z = t; }
+    else if (pair == 1) { x = s1; y = t; z = s2; }
+    else { x = t; y = s1; z = s2; }
+  }
+}
+// Global voxel of the cell (x, y, z) (each in [-2, 10)) relative to `leaf`.
+__device__ inline int64_t rel_voxel(const Dom &d, int64_t leaf, int x, int y, int z) {
+  const int64_t l12 = int64_t(d.L[1]) * d.L[2];
+  const int dx = x < 0 ? -1 : (x > 7 ? 1 : 0), dy = y < 0 ? -1 : (y > 7 ? 1 : 0),
+            dz = z < 0 ? -1 : (z > 7 ? 1 : 0);
+  const int64_t nb = leaf + dx * l12 + dy * int64_t(d.L[2]) + dz;
+  return (nb << 9) | ((x & 7) << 6) | ((y & 7) << 3) | (z & 7);
+}
+// Halo slot s (0..3) of thread t of a 256-thread leaf group, -1 when idle.
+// Every wave gets 96 distance-1 cells (slot 0, and slot 1 on lanes < 32) and
+// 120 other cells, so the waves of a group carry equal work.
+__device__ inline int fhalo_of(int t, int s) {
+  const int w = t >> 6, l = t & 63;
+  if (s == 0) return w * 64 + l;
+  if (s == 1) return l < 32 ? 256 + w * 32 + l : 384 + w * 32 + (l - 32);
+  if (s == 2) return 512 + w * 64 + l;
+  return l < 24 ? 768 + w * 24 + l : -1;
+}
+// The cell a slot loads: its own cell, or for an idle slot some edge cell
+// whose values are loaded and discarded. Every load of the prefetch is then
+// unconditional, so the compiler issues them all together instead of sinking
+// them, each with its own wait, into the branches that use them.
+__device__ inline int fhalo_load_of(int t, int s) {
+  const int m = fhalo_of(t, s);
+  return m >= 0 ? m : 768 + ((t & 63) - 24);
+}
+
+// Zero-padded stencil with tile strides SX, SY, 1.
+template <typename V, bool ORD, int SX, int SY>
+__device__ inline V stencil_s(const V *sp, int c, int diag) {
+#pragma clang fp contract(off)
+  if (ORD) {
+    V s = V(0);
+    s = s + (-sp[c - SX]);
+    s = s + (-sp[c - SY]);
+    s = s + (-sp[c - 1]);
+    { const V m = V(diag) * sp[c]; s = s + m; }
+    s = s + (-sp[c + 1]);
+    s = s + (-sp[c + SY]);
+    s = s + (-sp[c + SX]);
+    return s;
+  } else {
+    const V nb = ((sp[c - SX] + sp[c + SX]) + (sp[c - SY] + sp[c + SY])) + (sp[c - 1] + sp[c + 1]);
+    return fmav(V(diag), sp[c], -nb);
+  }
+}
+
+// GR leaf groups per workgroup: the groups of a workgroup share its
+// barriers, so small workgroups (GR = 1) let the leaves of one CU proceed
+// independently and overlap each other's memory latency.
+// Element i of p. With 32-bit offsets the byte offset is formed in 32 bits,
+// so the access is a scalar base plus a 32-bit vector offset (no 64-bit
+// address arithmetic per access); the caller guarantees it fits.
+template <typename T>
+__device__ inline T &elem(T *p, uint32_t i) {
+  using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+  return *reinterpret_cast<T *>(reinterpret_cast<B *>(p) + uint32_t(i * uint32_t(sizeof(T))));
+}
+template <typename T>
+__device__ inline T &elem(T *p, int64_t i) { return p[i]; }
+
+// OFF32: every array the kernel indexes is under 4 GiB, so voxel offsets
+// are 32-bit and each access is a scalar base plus a vector offset.
+template <typename V, typename A, bool ORD, int GR, bool OFF32>
+__global__ __launch_bounds__(kG * GR) void k_cg(Dom d, const int32_t *flist, int64_t nf,
+                                                const uint8_t *lab, const V *rin, V *rout,
+                                                const V *pin, V *pout, V *x, const A *part_in,
+                                                A *part_out, int g_parts, int k, PcgState *st) {
+#pragma clang fp contract(off)
+  using Off = typename std::conditional<OFF32, uint32_t, int64_t>::type;
+  __shared__ V s_p[GR][1728];
+  __shared__ V s_u[GR][1000];
+  // Leaf schedule: entry j is the list entry of round j of this group
+  // (the first list entry for rounds past the end, which are discarded).
+  constexpr int kSched = kG;
+  __shared__ int32_t s_leaf[GR][kSched];
+  // 1 / diag by label (label = FLUID | diag << 2; 0 for every non-row label).
+  __shared__ V s_dinv[32];
+  if (st->done) return;
+  const int g = threadIdx.x / kG, t = gtid();
+  if (threadIdx.x < 32) {
+    const int l = threadIdx.x;
+    s_dinv[l] = is_fluid(uint8_t(l)) ? dinv<V>(l >> 2) : V(0);
+  }
+  const bool r1b = (t & 63) < 32;   // halo slot 1 is a distance-1 cell
+  const bool h3 = (t & 63) < 24;    // halo slot 3 is in use
+  // Per-thread constants: for each halo slot, the voxel offset from the
+  // leaf's first voxel and the tile positions; for own cells, tile positions.
+  int32_t hoff[4];
+  int hp12[4], hu10[2];
+  const int64_t l12 = int64_t(d.L[1]) * d.L[2];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    int hx, hy, hz;
+    fhalo_cell(fhalo_load_of(t, s), hx, hy, hz);
+    const int dx = hx < 0 ? -1 : (hx > 7 ? 1 : 0), dy = hy < 0 ? -1 : (hy > 7 ? 1 : 0),
+              dz = hz < 0 ? -1 : (hz > 7 ? 1 : 0);
+    hoff[s] = int32_t((dx * l12 + dy * int64_t(d.L[2]) + dz) * kLeaf) |
+              ((hx & 7) << 6) | ((hy & 7) << 3) | (hz & 7);
+    hp12[s] = t12(hx, hy, hz);
+    if (s < 2) hu10[s] = tix(hx, hy, hz);
+  }
+  const int op12[2] = {t12_of_local(t), t12_of_local(t + kG)};
+  const int ou10[2] = {tix_of_local(t), tix_of_local(t + kG)};
+
+  struct Pre {
+    V orr[2], op[2], ox[2], hr[4], hp[4];
+    uint8_t ol[2], hl[4];
+    Off base;
+    bool valid;
+  };
+  // Prefetch of one leaf. All loads are unconditional (a round past the end
+  // of the list re-reads the list's first leaf, and at k = 0 the unused p
+  // buffer is read; such values are discarded), so the compiler issues them
+  // together rather than sinking each, with a wait, into a branch.
+  auto load = [&](bool valid, int32_t leaf, Pre &c) {
+    c.valid = valid;
+    c.base = Off(leaf) << 9;
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const Off v = c.base + Off(t + s * kG);
+      c.ol[s] = elem(lab, v);
+      c.orr[s] = elem(rin, v);
+      c.op[s] = elem(pin, v);
+      c.ox[s] = elem(x, v);
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const Off hv = c.base + Off(hoff[s]);
+      c.hl[s] = elem(lab, hv);
+      c.hr[s] = elem(rin, hv);
+      c.hp[s] = elem(pin, hv);
+    }
+  };
+  const Walk q0 = walk(nf);
+  Walk q = q0;
+  auto refill = [&](int j0) {
+    const int64_t pos = q0.mine() + int64_t(j0 + t) * q0.ng * q0.step;
+    s_leaf[g][t] = flist[pos < q0.end ? pos : 0];
+  };
+  int j = 0;  // round of q
+  refill(0);
+  __syncthreads();
+  Pre cur;
+  cur.valid = false;
+  if (q.more()) load(q.valid(), s_leaf[g][0], cur);
+  // Scalars of iteration k, reduced under the latency of the first loads.
+  double tot[3];
+  total_all<A, 3>(part_in, g_parts, tot);
+  const A gam = A(tot[0]), del = A(tot[1]);
+  const double rr = tot[2];
+  A beta = A(0), pAp = del;
+  if (k > 0) {
+    beta = gam / A(st->rzh[(k - 1) & 1]);
+    const A m = beta * gam;
+    pAp = del - m / A(st->alph[(k - 1) & 1]);
+  }
+  const A alpha_a = gam / pAp;
+  int status = 0;
+  if (isnan(rr)) status = FSL_ERR_NAN;
+  else if (!(double(beta) <= 1e5)) status = FSL_ERR_BETA;  // solver.cc:361
+  bool stop = status != 0 || converged(st, rr) || k >= st->max_iter;
+  if (!stop && !(double(pAp) > 0.0 && double(alpha_a) <= 1e5)) {  // solver.cc:333
+    status = FSL_ERR_ALPHA;
+    stop = true;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->it = k;
+    st->rr = rr;
+    st->rz = double(gam);
+    st->beta = double(beta);
+    st->pAp = double(pAp);
+    st->alpha = double(alpha_a);
+    st->rzh[k & 1] = double(gam);
+    st->alph[k & 1] = double(alpha_a);
+    if (status) st->status = status;
+    if (stop) st->done = 1;
+  }
+  if (stop) return;
+  const V alpha = V(alpha_a), bv = V(beta);
+  // p = u + beta p_old with u = dinv * r; dinv is 0 off the FLUID rows, and
+  // the select drops whatever p_old holds there.
+  auto pnew = [&](V r, V p, V di, uint8_t l) {
+    const V u = r * di;
+    const V m = bv * p;
+    const V pv = k == 0 ? u : u + m;
+    return is_fluid(l) ? pv : V(0);
+  };
+  A acc[3] = {A(0), A(0), A(0)};
+  V *tp = s_p[g];
+  V *tu = s_u[g];
+  while (q.more()) {
+    // Phase 1: p_k on the 1376 cells into the 12^3 tile.
+    const bool valid = cur.valid;
+    const Off base = cur.base;
+    V wr[2], wp[2], wx[2], wd[2], hr1[2], hd1[2];
+    uint8_t wl[2], hl1[2];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      wd[s] = s_dinv[cur.ol[s] & 31];
+      wp[s] = pnew(cur.orr[s], cur.op[s], wd[s], cur.ol[s]);
+      wr[s] = cur.orr[s];
+      wx[s] = cur.ox[s];
+      wl[s] = cur.ol[s];
+    }
+    V hpv[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const V di = s_dinv[cur.hl[s] & 31];
+      if (s < 2) hd1[s] = di;
+      hpv[s] = pnew(cur.hr[s], cur.hp[s], di, cur.hl[s]);
+    }
+    hr1[0] = cur.hr[0];
+    hr1[1] = cur.hr[1];
+    hl1[0] = cur.hl[0];
+    hl1[1] = cur.hl[1];
+    if (valid) {
+#pragma unroll
+      for (int s = 0; s < 2; ++s) tp[op12[s]] = wp[s];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        if (s == 3 && !h3) continue;
+        tp[hp12[s]] = hpv[s];
+      }
+    }
+    q.next();
+    ++j;
+    cur.valid = false;
+    if (q.more()) {
+      if (j % kSched == 0) {
+        __syncthreads();
+        refill(j);
+        __syncthreads();
+      }
+      load(q.valid(), s_leaf[g][j % kSched], cur);
+    }
+    __syncthreads();
+    // Phase 2: A p_k on the leaf and its distance-1 layers; r_{k+1} and
+    // u_{k+1} there; x, r and p of the leaf's own cells are written.
+    V rn[2], un[2];
+    if (valid) {
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        rn[s] = V(0);
+        un[s] = V(0);
+        if (is_fluid(wl[s])) {
+          const V ap = stencil_s<V, ORD, 144, 12>(tp, op12[s], wl[s] >> 2);
+          const Off v = base + Off(t + s * kG);
+          V xn;
+          if (ORD) {
+            const V m = alpha * wp[s];
+            xn = m + wx[s];
+            const V m2 = (-alpha) * ap;
+            rn[s] = m2 + wr[s];
+          } else {
+            xn = fmav(alpha, wp[s], wx[s]);
+            rn[s] = fmav(-alpha, ap, wr[s]);
+          }
+          un[s] = rn[s] * wd[s];
+          elem(x, v) = xn;
+          elem(rout, v) = rn[s];
+          elem(pout, v) = wp[s];
+        }
+        tu[ou10[s]] = un[s];
+      }
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        if (s == 1 && !r1b) continue;
+        // A non-row cell has dinv 0, so its u is 0 whatever its stencil.
+        const V ap = stencil_s<V, ORD, 144, 12>(tp, hp12[s], hl1[s] >> 2);
+        V rh;
+        if (ORD) {
+          const V m2 = (-alpha) * ap;
+          rh = m2 + hr1[s];
+        } else {
+          rh = fmav(-alpha, ap, hr1[s]);
+        }
+        tu[hu10[s]] = hd1[s] == V(0) ? V(0) : rh * hd1[s];
+      }
+    }
+    __syncthreads();
+    // Phase 3: A u_{k+1} on the leaf; gamma, delta and r.r of iteration k+1.
+    if (valid) {
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        if (!is_fluid(wl[s])) continue;
+        const V au = stencil_s<V, ORD, 100, 10>(tu, ou10[s], wl[s] >> 2);
+        acc[0] += A(rn[s]) * A(un[s]);
+        acc[1] += A(un[s]) * A(au);
+        acc[2] += A(rn[s]) * A(rn[s]);
+      }
+    }
+  }
+  publish<A, 3>(acc, part_out);
+}
+
+// gamma = r.u, delta = u.Au and r.r for a residual r (u = M^-1 r), from a
+// leaf tile with its face halos: the totals the single-reduction kernel
+// starts from, after preparation and after each residual refresh.
+template <typename V, typename A, bool ORD, int GR>
+__global__ __launch_bounds__(kG * GR) void k_gamma_delta(Dom d, const int32_t *flist,
+                                                         int64_t nf, const uint8_t *lab,
+                                                         const V *r, A *part,
+                                                         const PcgState *st, int force) {
+#pragma clang fp contract(off)
+  __shared__ V s_u[GR][1000];
+  __shared__ uint8_t s_f[GR][1000];
+  if ((!force && st->done) || st->status) return;
+  const int g = threadIdx.x / kG;
+  A acc[3] = {A(0), A(0), A(0)};
+  for (Walk q = walk(nf); q.more(); q.next()) {
+    const bool valid = q.valid();
+    const int64_t leaf = valid ? flist[q.mine()] : 0;
+    int o[3];
+    leaf_origin(d, leaf, o);
+    uint8_t ol[2] = {0, 0};
+    if (valid)
+      fill_tile<V>(d, lab, leaf, o, s_u[g], s_f[g], ol,
+                   [&](int64_t v, uint8_t l) { return r[v] * dinv<V>(l >> 2); });
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int c = gtid() + s * kG;
+      if (!valid || !is_fluid(ol[s])) continue;
+      const int ti = tix_of_local(c);
+      const V u = s_u[g][ti];
+      const V rv = r[(leaf << 9) | c];
+      const V au = stencil_s<V, ORD, 100, 10>(s_u[g], ti, ol[s] >> 2);
+      acc[0] += A(rv) * A(u);
+      acc[1] += A(u) * A(au);
+      acc[2] += A(rv) * A(rv);
+    }
+    __syncthreads();
+  }
+  publish<A, 3>(acc, part);
+}
+
+// Scalars after preparation (single-reduction path).
+template <typename A>
+__global__ __launch_bounds__(1024) void k_prepare_scalars_sr(const A *part_f, int g_f,
+                                                             const A *part_bb, int g_bb,
+                                                             PcgState *st) {
+  double tot[3];
+  total_all<A, 3>(part_f, g_f, tot);
+  const double bb = total<A, 1>(part_bb, g_bb, 0);
+  if (threadIdx.x != 0) return;
+  st->rr = tot[2];
+  st->rz = tot[0];
+  st->bnorm2 = bb;
+  if (st->stop_mode == FSL_STOP_REL) st->thr = st->thr * sqrt(bb);
+  st->it = 0;
+  st->status = isnan(tot[2]) ? FSL_ERR_NAN : 0;
+  st->done = (st->status != 0 || converged(st, tot[2]) || st->max_iter <= 0) ? 1 : 0;
+}
+
+// After a forced refresh (single-reduction path): the true r.r decides.
+__global__ __launch_bounds__(1024) void k_refresh_scalars_sr(const double *part_f, int g,
+                                                             PcgState *st) {
+  double tot[3];
+  total_all<double, 3>(part_f, g, tot);
+  if (threadIdx.x != 0) return;
+  st->rr_true = tot[2];
+  st->rr = tot[2];
+  if (isnan(tot[2])) st->status = FSL_ERR_NAN;
+  st->done = (st->status != 0 || converged(st, tot[2]) || st->it >= st->max_iter) ? 1 : 0;
 }
 
 // Stage API: after B(k), the scalars and the stopping test of iteration k+1
@@ -719,21 +1146,37 @@ __global__ __launch_bounds__(1024) void k_total_scalar(const double *part, int g
   if (threadIdx.x == 0) st->rr_true = t;
 }
 
-// Workgroups for a list of n leaves: up to kBlocksPerXcd per XCD.
-inline int grid_for(int64_t n) {
+// Workgroups for a list of n leaves: up to `cap` per XCD.
+inline int grid_for(int64_t n, int cap = kBlocksPerXcd) {
   const int64_t chunk = (n + kXcd - 1) / kXcd;
   const int64_t per = (chunk + kGroups - 1) / kGroups;
-  return int(kXcd * std::max<int64_t>(1, std::min<int64_t>(per, kBlocksPerXcd)));
+  return int(kXcd * std::max<int64_t>(1, std::min<int64_t>(per, cap)));
+}
+// k_cg runs one leaf group (256 threads) per workgroup; its register use
+// allows five such workgroups per CU with float vectors (160 per XCD) and
+// three with double vectors (96 per XCD).
+constexpr int kCgGroups = 1;
+inline int grid_cg(int64_t n, int vbytes) {
+  const int64_t cap = vbytes == 8 ? 96 : 160;
+  const int64_t chunk = (n + kXcd - 1) / kXcd;
+  return int(kXcd * std::max<int64_t>(1, std::min<int64_t>(chunk, cap)));
 }
 
-// Partial-sum buffers, carved from ctx->partials (sized for 8 x 1024 x 3 doubles).
+// Partial-sum buffers, carved from ctx->partials: seven slots of up to four
+// doubles per workgroup.
+constexpr int kPartSlots = 7;
+constexpr size_t kPartSlot = size_t(kXcd) * 256 * 4 * sizeof(double);
 struct Parts {
-  void *a, *b[2], *bb, *r;
+  void *a, *b[2], *bb, *r, *f[2];
 };
 inline Parts parts(fsl_ctx *c) {
   char *base = static_cast<char *>(c->partials);
-  const size_t slot = size_t(kXcd) * kBlocksPerXcd * 2 * sizeof(double);
-  return {base, {base + slot, base + 2 * slot}, base + 3 * slot, base + 4 * slot};
+  const size_t slot = kPartSlot;
+  return {base,
+          {base + slot, base + 2 * slot},
+          base + 3 * slot,
+          base + 4 * slot,
+          {base + 5 * slot, base + 6 * slot}};
 }
 
 template <typename V, typename A, bool ORD, bool MIXED>
@@ -776,6 +1219,33 @@ struct Kernels {
     k_finish<V, MIXED><<<grid_for(c->n_act), kT, 0, c->stream>>>(
         c->act_list, c->n_act, c->lab, (const V *)c->x, c->x64, c->pres);
   }
+  // Single-reduction path: r_k lives in rb(k & 1), p_k in p[k & 1], and the
+  // totals (gamma_k, delta_k, r.r_k) in f[k & 1].
+  static V *rb(fsl_ctx *c, int k) { return (V *)(k & 1 ? c->z : c->r); }
+  static void sr_prepare(fsl_ctx *c, int warm) {
+    const int gp = grid_for(c->n_fluid), g = grid_cg(c->n_fluid, sizeof(V));
+    Parts P = parts(c);
+    k_prepare<V, A, ORD, MIXED><<<gp, kT, 0, c->stream>>>(
+        c->dom, c->fluid_list, c->n_fluid, c->lab, c->div, c->pres, warm, (V *)c->x, rb(c, 0),
+        nullptr, c->x64, (A *)P.b[1], (A *)P.bb);
+    k_gamma_delta<V, A, ORD, kCgGroups><<<g, kG * kCgGroups, 0, c->stream>>>(
+        c->dom, c->fluid_list, c->n_fluid, c->lab, rb(c, 0), (A *)P.f[0], c->pcg, 1);
+    k_prepare_scalars_sr<A><<<1, 1024, 0, c->stream>>>((const A *)P.f[0], g, (const A *)P.bb, gp,
+                                                       c->pcg);
+  }
+  static void sr_step(fsl_ctx *c, int k, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+    const int g = grid_cg(c->n_fluid, sizeof(V));
+    Parts P = parts(c);
+    // 32-bit voxel offsets when every indexed array is under 4 GiB.
+    auto kern = uint64_t(c->nvox) * sizeof(V) < (uint64_t(1) << 32)
+                    ? k_cg<V, A, ORD, kCgGroups, true>
+                    : k_cg<V, A, ORD, kCgGroups, false>;
+    hipExtLaunchKernelGGL(kern, dim3(g), dim3(kG * kCgGroups), 0, c->stream, e0, e1, 0, c->dom,
+                          (const int32_t *)c->fluid_list, c->n_fluid, (const uint8_t *)c->lab,
+                          (const V *)rb(c, k), rb(c, k + 1), (const V *)c->p[(k + 1) & 1],
+                          (V *)c->p[k & 1], (V *)c->x, (const A *)P.f[k & 1],
+                          (A *)P.f[(k + 1) & 1], g, k, c->pcg);
+  }
   static void true_residual(fsl_ctx *c) {
     const int g = grid_for(c->n_fluid);
     Parts P = parts(c);
@@ -796,6 +1266,21 @@ void launch_refresh(fsl_ctx *c, int k, int force) {
                                        c->x64, (float *)c->r, (float *)c->z, (double *)P.b[k & 1],
                                        c->pcg, force);
   if (force) k_refresh_scalars<<<1, 1024, 0, c->stream>>>((const double *)P.b[k & 1], g, c->pcg);
+}
+
+// Single-reduction path, MIXED: replace r_kn (input of iteration kn) by the
+// FP64 true residual, and its totals by those of the true residual.
+void launch_refresh_sr(fsl_ctx *c, int kn, int force) {
+  const int g = grid_cg(c->n_fluid, sizeof(float));
+  Parts P = parts(c);
+  float *rbuf = (float *)(kn & 1 ? c->z : c->r);
+  k_refresh_x<<<g, kT, 0, c->stream>>>(c->fluid_list, c->n_fluid, c->lab, (float *)c->x, c->x64,
+                                       c->pcg, force);
+  k_refresh_r<<<g, kT, 0, c->stream>>>(c->dom, c->fluid_list, c->n_fluid, c->lab, c->div,
+                                       c->x64, rbuf, nullptr, nullptr, c->pcg, force);
+  k_gamma_delta<float, double, false, kCgGroups><<<g, kG * kCgGroups, 0, c->stream>>>(
+      c->dom, c->fluid_list, c->n_fluid, c->lab, rbuf, (double *)P.f[kn & 1], c->pcg, force);
+  if (force) k_refresh_scalars_sr<<<1, 1024, 0, c->stream>>>((const double *)P.f[kn & 1], g, c->pcg);
 }
 
 template <typename F>
@@ -821,10 +1306,18 @@ int clear_done(fsl_ctx *ctx) {
 
 }  // namespace
 
-size_t pcg_partials_bytes() { return size_t(5) * kXcd * kBlocksPerXcd * 2 * sizeof(double); }
+size_t pcg_partials_bytes() { return kPartSlots * kPartSlot; }
 
-int pcg_prepare(fsl_ctx *ctx, const fsl_solve_params *prm) {
-  if (prm->precision < 0 || prm->precision > 2 || prm->stop_mode < 0 || prm->stop_mode > 1) {
+bool pcg_single_reduction(const fsl_solve_params *prm) {
+  if (prm->variant == FSL_PCG_SINGLE_REDUCTION) return true;
+  if (prm->variant == FSL_PCG_TWO_PASS) return false;
+  return prm->precision != FSL_PCG_FP32;
+}
+
+// Host-side solver state of a new solve; uploads it to the device.
+static int pcg_setup(fsl_ctx *ctx, const fsl_solve_params *prm) {
+  if (prm->precision < 0 || prm->precision > 2 || prm->stop_mode < 0 || prm->stop_mode > 1 ||
+      prm->variant < 0 || prm->variant > 2) {
     ctx->err = "invalid solver parameters";
     return FSL_ERR_ARG;
   }
@@ -842,12 +1335,18 @@ int pcg_prepare(fsl_ctx *ctx, const fsl_solve_params *prm) {
                : prm->threshold;
   if (ctx->n_fluid == 0) h->done = 1;
   FSL_HIP(hipMemcpyAsync(ctx->pcg, h, sizeof(PcgState), hipMemcpyHostToDevice, ctx->stream));
+  return FSL_OK;
+}
+
+int pcg_prepare(fsl_ctx *ctx, const fsl_solve_params *prm) {
+  int rc = pcg_setup(ctx, prm);
+  if (rc) return rc;
   if (ctx->n_fluid > 0) {
     dispatch(ctx->pcg_precision,
              [&](auto k) { decltype(k)::prepare(ctx, prm->warm_start ? 1 : 0); });
     FSL_HIP(hipGetLastError());
   }
-  int rc = sync_state(ctx);
+  rc = sync_state(ctx);
   if (rc) return rc;
   ctx->pcg_ready = true;
   return ctx->pcg_host->status;
@@ -871,7 +1370,96 @@ int pcg_iterate_once(fsl_ctx *ctx) {
   return ctx->pcg_host->status;
 }
 
+// The single-reduction solve: one k_cg launch per iteration. Batches of
+// iterations run between host checks of the device state; launches past
+// convergence exit at once (the done flag).
+static int pcg_run_sr(fsl_ctx *ctx, const fsl_solve_params *prm, fsl_solve_stats *st) {
+  int rc = pcg_setup(ctx, prm);
+  if (rc) return rc;
+  if (ctx->n_fluid > 0) {
+    dispatch(ctx->pcg_precision,
+             [&](auto K) { decltype(K)::sr_prepare(ctx, prm->warm_start ? 1 : 0); });
+    FSL_HIP(hipGetLastError());
+  }
+  rc = sync_state(ctx);
+  if (rc) return rc;
+  ctx->pcg_ready = true;
+  if (ctx->pcg_host->status) return ctx->pcg_host->status;
+  const bool adaptive = prm->batch <= 0;
+  const int batch = prm->batch > 0 ? prm->batch : 16;
+  const bool mixed = ctx->pcg_precision == FSL_PCG_MIXED;
+  const int nb_cap = ctx->timing ? 100 : 1024;
+  int nb = std::min(batch, nb_cap);
+  double rr_prev = ctx->pcg_host->rr;
+  int it_prev = 0;
+  hipEvent_t e0, e1;
+  FSL_HIP(hipEventCreate(&e0));
+  FSL_HIP(hipEventCreate(&e1));
+  FSL_HIP(hipEventRecord(e0, ctx->stream));
+  hipEvent_t *pev = ctx->tev + 32;
+  int k = 0;  // next iteration to launch
+  while (!ctx->pcg_host->done) {
+    const int k0 = k;
+    for (int b = 0; b < nb; ++b, ++k) {
+      hipEvent_t *e = pev + 2 * b;
+      const bool t = ctx->timing;
+      dispatch(ctx->pcg_precision, [&](auto K) {
+        decltype(K)::sr_step(ctx, k, t ? e[0] : nullptr, t ? e[1] : nullptr);
+      });
+      if (mixed && (k + 1) % ctx->pcg_refresh == 0) launch_refresh_sr(ctx, k + 1, 0);
+    }
+    FSL_HIP(hipGetLastError());
+    rc = sync_state(ctx);
+    if (rc) return rc;
+    const PcgState *h = ctx->pcg_host;
+    if (ctx->timing) {
+      // Launches k0..k-1 ran; those before the stopping one did the work.
+      const int worked = std::min(h->done ? h->it - k0 : nb, nb);
+      for (int b = 0; b < worked; ++b) {
+        float ms = 0.0f;
+        FSL_HIP(hipEventElapsedTime(&ms, pev[2 * b], pev[2 * b + 1]));
+        ctx->tacc[FSL_T_FUSED] += ms;
+        ctx->tacc[FSL_T_FUSED_N] += 1;
+      }
+    }
+    if (h->status) break;
+    if (!h->done) {
+      if (adaptive && h->it > it_prev && h->rr > 0.0 && rr_prev > 0.0) {
+        const double rate = std::log(h->rr / rr_prev) / double(h->it - it_prev);
+        const double target = h->stop_mode == FSL_STOP_REL ? h->thr * h->thr : h->thr;
+        if (rate < 0.0 && target > 0.0) {
+          const double need = std::log(target / h->rr) / rate;
+          nb = int(std::min(double(nb_cap), std::max(16.0, 0.8 * need)));
+        }
+      }
+      rr_prev = h->rr;
+      it_prev = h->it;
+      continue;
+    }
+    if (mixed && h->it > 0 && h->it < h->max_iter) {
+      // Converged on the recursive residual: confirm on the true residual of
+      // x_it, and continue from iteration it if it does not hold.
+      const int last = h->it;
+      launch_refresh_sr(ctx, last, 1);
+      FSL_HIP(hipGetLastError());
+      rc = sync_state(ctx);
+      if (rc) return rc;
+      if (!ctx->pcg_host->done) k = last;
+    }
+  }
+  FSL_HIP(hipEventRecord(e1, ctx->stream));
+  FSL_HIP(hipEventSynchronize(e1));
+  float ms = 0.0f;
+  FSL_HIP(hipEventElapsedTime(&ms, e0, e1));
+  FSL_HIP(hipEventDestroy(e0));
+  FSL_HIP(hipEventDestroy(e1));
+  if (st) st->seconds = double(ms) * 1e-3;
+  if (ctx->pcg_host->status) return ctx->pcg_host->status;
+  return pcg_finish(ctx, st);
+}
+
 int pcg_run(fsl_ctx *ctx, const fsl_solve_params *prm, fsl_solve_stats *st) {
+  if (pcg_single_reduction(prm)) return pcg_run_sr(ctx, prm, st);
   int rc = pcg_prepare(ctx, prm);
   if (rc) return rc;
   const int batch = prm->batch > 0 ? prm->batch : 16;

@@ -46,6 +46,14 @@ extern "C" {
 #define FSL_PCG_FP64 1  /* FP64 vectors, stencil and dots */
 #define FSL_PCG_MIXED 2 /* FP32 stencil/axpys, FP64 dots, FP64 true-residual refresh */
 
+/* PCG iteration structure. */
+#define FSL_PCG_AUTO 0     /* FP32: two-pass; FP64 and MIXED: single-reduction */
+#define FSL_PCG_TWO_PASS 1 /* the reference's stage order (solver_driver.cc:318-343): Ap and
+                              p.Ap, then the axpys with r.r and r.z; two reductions */
+#define FSL_PCG_SINGLE_REDUCTION 2 /* one pass per iteration: p, Ap, x, r and A(M^-1 r) from
+                              a two-cell halo, alpha from the Chronopoulos-Gear recurrence;
+                              one reduction */
+
 /* PCG stopping rule. */
 #define FSL_STOP_ABS_RR 0 /* stop when r.r < threshold (solver_driver.cc:330) */
 #define FSL_STOP_REL 1    /* stop when ||r|| < threshold * ||b|| */
@@ -72,6 +80,7 @@ typedef struct fsl_solve_params {
                          the observed residual decay (first batch 16) */
   int warm_start;     /* 0: x0 = 0 (the cleared pressure grid, solver.cc:86-88);
                          1: x0 = the current pressure grid */
+  int variant;        /* FSL_PCG_AUTO / TWO_PASS / SINGLE_REDUCTION */
 } fsl_solve_params;
 
 typedef struct fsl_solve_stats {
@@ -199,7 +208,9 @@ int fsl_pcg_finish(fsl_ctx *ctx, fsl_solve_stats *stats);
 #define FSL_T_PASS_B_N 12
 #define FSL_T_REDUCE 13   /* PCG scalar reductions (one workgroup each), summed */
 #define FSL_T_STEPS 14    /* fsl_step calls timed */
-#define FSL_T_COUNT 15
+#define FSL_T_FUSED 15    /* single-reduction PCG pass, summed */
+#define FSL_T_FUSED_N 16  /* single-reduction passes that did work */
+#define FSL_T_COUNT 17
 int fsl_set_timing(fsl_ctx *ctx, int enable);
 int fsl_get_timing(fsl_ctx *ctx, double *out, int n);
 
