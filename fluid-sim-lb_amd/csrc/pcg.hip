@@ -662,9 +662,17 @@ __global__ __launch_bounds__(kT) void k_refresh_r(Dom d, const int32_t *flist, i
 // exchanged). r and p are double-buffered by iteration parity, as other
 // workgroups read the halo values of the iteration's inputs.
 
-// 12^3 tile over [-2, 10)^3 of the leaf and 10^3 tile over [-1, 9)^3.
-__device__ inline int t12(int x, int y, int z) { return ((x + 2) * 12 + (y + 2)) * 12 + (z + 2); }
+// Tile P over [-2, 10)^3 of the leaf and tile U over [-1, 9)^3, with row
+// strides padded so that the wave access patterns of the stencil and of the
+// halo faces spread over the LDS banks (x strides 151 and 137 words instead
+// of 144 and 100: about a third fewer bank-conflict cycles for 32 or 64
+// banks).
+constexpr int kPX = 151, kPY = 12, kPSize = 11 * kPX + 11 * kPY + 12;
+constexpr int kUX = 137, kUY = 12, kUSize = 9 * kUX + 9 * kUY + 10;
+__device__ inline int t12(int x, int y, int z) { return (x + 2) * kPX + (y + 2) * kPY + (z + 2); }
 __device__ inline int t12_of_local(int c) { return t12(c >> 6, (c >> 3) & 7, c & 7); }
+__device__ inline int u10(int x, int y, int z) { return (x + 1) * kUX + (y + 1) * kUY + (z + 1); }
+__device__ inline int u10_of_local(int c) { return u10(c >> 6, (c >> 3) & 7, c & 7); }
 
 // Halo cell m of the fused pass, relative to the leaf origin. m in [0, 384):
 // face layers at distance 1 (face m/64: x-, x+, y-, y+, z-, z+); m in
@@ -696,24 +704,72 @@ __device__ inline int64_t rel_voxel(const Dom &d, int64_t leaf, int x, int y, in
   const int64_t nb = leaf + dx * l12 + dy * int64_t(d.L[2]) + dz;
   return (nb << 9) | ((x & 7) << 6) | ((y & 7) << 3) | (z & 7);
 }
-// Halo slot s (0..3) of thread t of a 256-thread leaf group, -1 when idle.
-// Every wave gets 96 distance-1 cells (slot 0, and slot 1 on lanes < 32) and
-// 120 other cells, so the waves of a group carry equal work.
-__device__ inline int fhalo_of(int t, int s) {
-  const int w = t >> 6, l = t & 63;
-  if (s == 0) return w * 64 + l;
-  if (s == 1) return l < 32 ? 256 + w * 32 + l : 384 + w * 32 + (l - 32);
-  if (s == 2) return 512 + w * 64 + l;
-  return l < 24 ? 768 + w * 24 + l : -1;
+// Halo runs of the single-reduction pass. The 864 halo cells of a leaf (face
+// layers at distance 1 and 2, edge rows at distance (1, 1)) are loaded as
+// runs of consecutive cells along z, 4, 2 or 1 cells per lane, so a leaf's
+// halo takes at most two load instructions per wave and array:
+//   slot 0, wave 0: x faces, layers x = -2, -1, 8, 9: 64 runs of 4;
+//   slot 0, wave 1: y faces, layers y = -2, -1, 8, 9: 64 runs of 4;
+//   slot 0, waves 2, 3: z faces, z = (-2, -1) and (8, 9): 128 runs of 2;
+//   slot 1, wave 0, lanes 0-7: xy edge rows: 8 runs of 4;
+//   slot 1, waves 1, 2, lanes 0-31: xz and yz edge rows: 64 single cells.
+// Idle lanes of a loading wave load a run of the same slot and discard it.
+struct Run {
+  int x, y, z;  // first cell, relative to the leaf origin
+  int n;        // cells in the run (0: the lane's slot is idle)
+};
+__device__ inline int run_width(int w, int s) {
+  if (s == 0) return w <= 1 ? 4 : 2;
+  return w == 0 ? 4 : (w <= 2 ? 1 : 0);
 }
-// The cell a slot loads: its own cell, or for an idle slot some edge cell
-// whose values are loaded and discarded. Every load of the prefetch is then
-// unconditional, so the compiler issues them all together instead of sinking
-// them, each with its own wait, into the branches that use them.
-__device__ inline int fhalo_load_of(int t, int s) {
-  const int m = fhalo_of(t, s);
-  return m >= 0 ? m : 768 + ((t & 63) - 24);
+__device__ inline Run halo_run(int w, int l, int s) {
+  if (s == 0) {
+    if (w <= 1) {
+      const int side = l >> 5, i = l & 31;
+      const int a = i >> 2, layer = (i >> 1) & 1, z = (i & 1) * 4;
+      const int c = side ? 8 + layer : layer - 2;
+      return w == 0 ? Run{c, a, z, 4} : Run{a, c, z, 4};
+    }
+    const int u = (w - 2) * 64 + l, side = u >> 6, i = u & 63;
+    return Run{i >> 3, i & 7, side ? 8 : -2, 2};
+  }
+  if (w == 0) {
+    const int e = (l >> 1) & 3;  // lanes 8-63 repeat lanes 0-7
+    return Run{(e & 2) ? 8 : -1, (e & 1) ? 8 : -1, (l & 1) * 4, l < 8 ? 4 : 0};
+  }
+  if (w <= 2) {
+    const int e = (l >> 3) & 3, f = l & 7, a = (e & 2) ? 8 : -1, b = (e & 1) ? 8 : -1;
+    const int n = l < 32 ? 1 : 0;  // lanes 32-63 repeat lanes 0-31
+    return w == 1 ? Run{a, f, b, n} : Run{f, a, b, n};
+  }
+  return Run{-1, -1, 0, 0};
 }
+// Distance-1 cells of a run: bit e of the mask is set when cell e of the run
+// is distance-1 cell base + e (index f * 64 + a * 8 + b of fhalo_cell).
+__device__ inline void run_r1(const Run &r, int s, int &mask, int &base) {
+  mask = 0;
+  base = 0;
+  if (s != 0) return;
+  if (r.n == 4) {
+    const bool xf = r.x < 0 || r.x > 7;
+    const int c = xf ? r.x : r.y;
+    if (c == -1 || c == 8) {
+      mask = 0xF;
+      base = (xf ? (c < 0 ? 0 : 64) : (c < 0 ? 128 : 192)) + (xf ? r.y : r.x) * 8 + r.z;
+    }
+  } else {
+    const bool lo = r.z < 0;
+    mask = lo ? 2 : 1;
+    base = (lo ? 256 : 320) + r.x * 8 + r.y - (lo ? 1 : 0);
+  }
+}
+
+// N consecutive elements from element i (i aligned to N), one vector access.
+template <int N, typename T, typename Off>
+__device__ inline void load_run(const T *p, Off i, T out[4]);
+// N consecutive labels from voxel i, packed little-endian.
+template <int N, typename Off>
+__device__ inline uint32_t load_labels(const uint8_t *lab, Off i);
 
 // Zero-padded stencil with tile strides SX, SY, 1.
 template <typename V, bool ORD, int SX, int SY>
@@ -749,8 +805,41 @@ __device__ inline T &elem(T *p, uint32_t i) {
 template <typename T>
 __device__ inline T &elem(T *p, int64_t i) { return p[i]; }
 
+template <int N, typename T, typename Off>
+__device__ inline void load_run(const T *p, Off i, T out[4]) {
+  if constexpr (N == 1) {
+    out[0] = elem(p, i);
+  } else {
+    typedef T VT __attribute__((ext_vector_type(N)));
+    const VT v = *reinterpret_cast<const VT *>(&elem(p, i));
+#pragma unroll
+    for (int e = 0; e < N; ++e) out[e] = v[e];
+  }
+}
+template <int N, typename Off>
+__device__ inline uint32_t load_labels(const uint8_t *lab, Off i) {
+  if constexpr (N == 4) return *reinterpret_cast<const uint32_t *>(&elem(lab, i));
+  else if constexpr (N == 2) return *reinterpret_cast<const uint16_t *>(&elem(lab, i));
+  else return elem(lab, i);
+}
+template <typename T, typename Off>
+__device__ inline void store_pair(T *p, Off i, T a, T b) {
+  typedef T VT __attribute__((ext_vector_type(2)));
+  VT v;
+  v[0] = a;
+  v[1] = b;
+  *reinterpret_cast<VT *>(&elem(p, i)) = v;
+}
+
 // OFF32: every array the kernel indexes is under 4 GiB, so voxel offsets
 // are 32-bit and each access is a scalar base plus a vector offset.
+//
+// Per leaf group (256 threads): thread t owns cells 2t and 2t+1 (pairs along
+// z, loaded and stored as vectors) and up to two halo runs. Phase 1 forms
+// p_k on the leaf and its halo in the P tile and hands the distance-1 cells'
+// r and labels to a balanced mapping (thread t: distance-1 cells t and, on
+// lanes 0-31, 256 + wave * 32 + lane) through LDS; phase 2 applies A to p_k
+// there and on the own cells; phase 3 applies A to u_{k+1} on the own cells.
 template <typename V, typename A, bool ORD, int GR, bool OFF32>
 __global__ __launch_bounds__(kG * GR) void k_cg(Dom d, const int32_t *flist, int64_t nf,
                                                 const uint8_t *lab, const V *rin, V *rout,
@@ -758,8 +847,10 @@ __global__ __launch_bounds__(kG * GR) void k_cg(Dom d, const int32_t *flist, int
                                                 A *part_out, int g_parts, int k, PcgState *st) {
 #pragma clang fp contract(off)
   using Off = typename std::conditional<OFF32, uint32_t, int64_t>::type;
-  __shared__ V s_p[GR][1728];
-  __shared__ V s_u[GR][1000];
+  __shared__ V s_p[GR][kPSize];
+  __shared__ V s_u[GR][kUSize];
+  __shared__ V s_r1[GR][384];        // r of the distance-1 cells
+  __shared__ uint8_t s_l1[GR][384];  // their labels
   // Leaf schedule: entry j is the list entry of round j of this group
   // (the first list entry for rounds past the end, which are discarded).
   constexpr int kSched = kG;
@@ -768,58 +859,76 @@ __global__ __launch_bounds__(kG * GR) void k_cg(Dom d, const int32_t *flist, int
   __shared__ V s_dinv[32];
   if (st->done) return;
   const int g = threadIdx.x / kG, t = gtid();
+  const int w = __builtin_amdgcn_readfirstlane(t >> 6), l = t & 63;
   if (threadIdx.x < 32) {
-    const int l = threadIdx.x;
-    s_dinv[l] = is_fluid(uint8_t(l)) ? dinv<V>(l >> 2) : V(0);
+    const int lb = threadIdx.x;
+    s_dinv[lb] = is_fluid(uint8_t(lb)) ? dinv<V>(lb >> 2) : V(0);
   }
-  const bool r1b = (t & 63) < 32;   // halo slot 1 is a distance-1 cell
-  const bool h3 = (t & 63) < 24;    // halo slot 3 is in use
-  // Per-thread constants: for each halo slot, the voxel offset from the
-  // leaf's first voxel and the tile positions; for own cells, tile positions.
-  int32_t hoff[4];
-  int hp12[4], hu10[2];
+  // Per-thread constants: the two halo runs (voxel offset from the leaf's
+  // first voxel, P tile position, distance-1 cells), the two distance-1
+  // cells of phase 2 (tile positions) and the own pair (tile positions).
   const int64_t l12 = int64_t(d.L[1]) * d.L[2];
+  const int wid0 = run_width(w, 0), wid1 = run_width(w, 1);
+  int32_t hoff[2];
+  int hp0[2], rmask[2], rbase[2], hn[2];
 #pragma unroll
-  for (int s = 0; s < 4; ++s) {
-    int hx, hy, hz;
-    fhalo_cell(fhalo_load_of(t, s), hx, hy, hz);
-    const int dx = hx < 0 ? -1 : (hx > 7 ? 1 : 0), dy = hy < 0 ? -1 : (hy > 7 ? 1 : 0),
-              dz = hz < 0 ? -1 : (hz > 7 ? 1 : 0);
+  for (int s = 0; s < 2; ++s) {
+    const Run r = halo_run(w, l, s);
+    const int dx = r.x < 0 ? -1 : (r.x > 7 ? 1 : 0), dy = r.y < 0 ? -1 : (r.y > 7 ? 1 : 0),
+              dz = r.z < 0 ? -1 : (r.z > 7 ? 1 : 0);
     hoff[s] = int32_t((dx * l12 + dy * int64_t(d.L[2]) + dz) * kLeaf) |
-              ((hx & 7) << 6) | ((hy & 7) << 3) | (hz & 7);
-    hp12[s] = t12(hx, hy, hz);
-    if (s < 2) hu10[s] = tix(hx, hy, hz);
+              ((r.x & 7) << 6) | ((r.y & 7) << 3) | (r.z & 7);
+    hp0[s] = t12(r.x, r.y, r.z);
+    hn[s] = r.n;
+    run_r1(r, s, rmask[s], rbase[s]);
   }
-  const int op12[2] = {t12_of_local(t), t12_of_local(t + kG)};
-  const int ou10[2] = {tix_of_local(t), tix_of_local(t + kG)};
+  const bool m1ok = l < 32;
+  const int m1[2] = {t, 256 + w * 32 + (l & 31)};
+  int r1p[2], r1u[2];
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    int hx, hy, hz;
+    fhalo_cell(m1[s], hx, hy, hz);
+    r1p[s] = t12(hx, hy, hz);
+    r1u[s] = u10(hx, hy, hz);
+  }
+  const int op12 = t12_of_local(2 * t), ou10 = u10_of_local(2 * t);  // cell 2t+1 is +1
 
   struct Pre {
-    V orr[2], op[2], ox[2], hr[4], hp[4];
-    uint8_t ol[2], hl[4];
+    V orr[2], op[2], ox[2], hr[2][4], hp[2][4];
+    uint32_t ol, hl[2];
     Off base;
     bool valid;
   };
-  // Prefetch of one leaf. All loads are unconditional (a round past the end
-  // of the list re-reads the list's first leaf, and at k = 0 the unused p
-  // buffer is read; such values are discarded), so the compiler issues them
-  // together rather than sinking each, with a wait, into a branch.
+  // Prefetch of one leaf. All loads of a wave are unconditional (a round
+  // past the end of the list re-reads the list's first leaf, at k = 0 the
+  // unused p buffer is read, idle lanes repeat a run; such values are
+  // discarded), so the compiler issues them together.
   auto load = [&](bool valid, int32_t leaf, Pre &c) {
     c.valid = valid;
     c.base = Off(leaf) << 9;
+    const Off v = c.base + Off(2 * t);
+    c.ol = load_labels<2>(lab, v);
+    load_run<2>(rin, v, c.orr);
+    load_run<2>(pin, v, c.op);
+    load_run<2>(x, v, c.ox);
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
-      const Off v = c.base + Off(t + s * kG);
-      c.ol[s] = elem(lab, v);
-      c.orr[s] = elem(rin, v);
-      c.op[s] = elem(pin, v);
-      c.ox[s] = elem(x, v);
-    }
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
       const Off hv = c.base + Off(hoff[s]);
-      c.hl[s] = elem(lab, hv);
-      c.hr[s] = elem(rin, hv);
-      c.hp[s] = elem(pin, hv);
+      const int wd = s ? wid1 : wid0;
+      if (wd == 4) {
+        c.hl[s] = load_labels<4>(lab, hv);
+        load_run<4>(rin, hv, c.hr[s]);
+        load_run<4>(pin, hv, c.hp[s]);
+      } else if (wd == 2) {
+        c.hl[s] = load_labels<2>(lab, hv);
+        load_run<2>(rin, hv, c.hr[s]);
+        load_run<2>(pin, hv, c.hp[s]);
+      } else if (wd == 1) {
+        c.hl[s] = load_labels<1>(lab, hv);
+        load_run<1>(rin, hv, c.hr[s]);
+        load_run<1>(pin, hv, c.hp[s]);
+      }
     }
   };
   const Walk q0 = walk(nf);
@@ -868,49 +977,53 @@ __global__ __launch_bounds__(kG * GR) void k_cg(Dom d, const int32_t *flist, int
   }
   if (stop) return;
   const V alpha = V(alpha_a), bv = V(beta);
-  // p = u + beta p_old with u = dinv * r; dinv is 0 off the FLUID rows, and
-  // the select drops whatever p_old holds there.
-  auto pnew = [&](V r, V p, V di, uint8_t l) {
+  // p = u + beta p_old with u = dinv * r; the select drops whatever a
+  // non-row cell holds.
+  auto pnew = [&](V r, V p, uint32_t lb, V &di) {
+    const uint8_t lc = uint8_t(lb);
+    di = s_dinv[lc & 31];
     const V u = r * di;
     const V m = bv * p;
     const V pv = k == 0 ? u : u + m;
-    return is_fluid(l) ? pv : V(0);
+    return is_fluid(lc) ? pv : V(0);
   };
   A acc[3] = {A(0), A(0), A(0)};
   V *tp = s_p[g];
   V *tu = s_u[g];
+  V *tr1 = s_r1[g];
+  uint8_t *tl1 = s_l1[g];
   while (q.more()) {
-    // Phase 1: p_k on the 1376 cells into the 12^3 tile.
+    // Phase 1: p_k on the 1376 cells into the P tile; distance-1 cells' r
+    // and labels to LDS.
     const bool valid = cur.valid;
     const Off base = cur.base;
-    V wr[2], wp[2], wx[2], wd[2], hr1[2], hd1[2];
-    uint8_t wl[2], hl1[2];
+    V wr[2], wp[2], wx[2], wd[2];
+    uint8_t wl[2];
 #pragma unroll
-    for (int s = 0; s < 2; ++s) {
-      wd[s] = s_dinv[cur.ol[s] & 31];
-      wp[s] = pnew(cur.orr[s], cur.op[s], wd[s], cur.ol[s]);
-      wr[s] = cur.orr[s];
-      wx[s] = cur.ox[s];
-      wl[s] = cur.ol[s];
+    for (int i = 0; i < 2; ++i) {
+      wl[i] = uint8_t(cur.ol >> (8 * i));
+      wp[i] = pnew(cur.orr[i], cur.op[i], wl[i], wd[i]);
+      wr[i] = cur.orr[i];
+      wx[i] = cur.ox[i];
     }
-    V hpv[4];
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      const V di = s_dinv[cur.hl[s] & 31];
-      if (s < 2) hd1[s] = di;
-      hpv[s] = pnew(cur.hr[s], cur.hp[s], di, cur.hl[s]);
-    }
-    hr1[0] = cur.hr[0];
-    hr1[1] = cur.hr[1];
-    hl1[0] = cur.hl[0];
-    hl1[1] = cur.hl[1];
     if (valid) {
+      tp[op12] = wp[0];
+      tp[op12 + 1] = wp[1];
 #pragma unroll
-      for (int s = 0; s < 2; ++s) tp[op12[s]] = wp[s];
+      for (int s = 0; s < 2; ++s) {
+        if (hn[s] == 0) continue;
+        const int wd_s = s ? wid1 : wid0;
 #pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        if (s == 3 && !h3) continue;
-        tp[hp12[s]] = hpv[s];
+        for (int e = 0; e < 4; ++e) {
+          if (e >= wd_s) break;
+          const uint32_t lb = cur.hl[s] >> (8 * e);
+          V di;
+          tp[hp0[s] + e] = pnew(cur.hr[s][e], cur.hp[s][e], lb, di);
+          if ((rmask[s] >> e) & 1) {
+            tr1[rbase[s] + e] = cur.hr[s][e];
+            tl1[rbase[s] + e] = uint8_t(lb);
+          }
+        }
       }
     }
     q.next();
@@ -925,59 +1038,63 @@ __global__ __launch_bounds__(kG * GR) void k_cg(Dom d, const int32_t *flist, int
       load(q.valid(), s_leaf[g][j % kSched], cur);
     }
     __syncthreads();
-    // Phase 2: A p_k on the leaf and its distance-1 layers; r_{k+1} and
-    // u_{k+1} there; x, r and p of the leaf's own cells are written.
+    // Phase 2: A p_k on the own pair and the distance-1 cells; r_{k+1} and
+    // u_{k+1} there; x, r and p of the own pair are written.
     V rn[2], un[2];
     if (valid) {
+      V xn[2];
 #pragma unroll
-      for (int s = 0; s < 2; ++s) {
-        rn[s] = V(0);
-        un[s] = V(0);
-        if (is_fluid(wl[s])) {
-          const V ap = stencil_s<V, ORD, 144, 12>(tp, op12[s], wl[s] >> 2);
-          const Off v = base + Off(t + s * kG);
-          V xn;
+      for (int i = 0; i < 2; ++i) {
+        rn[i] = V(0);
+        un[i] = V(0);
+        xn[i] = wx[i];
+        if (is_fluid(wl[i])) {
+          const V ap = stencil_s<V, ORD, kPX, kPY>(tp, op12 + i, wl[i] >> 2);
           if (ORD) {
-            const V m = alpha * wp[s];
-            xn = m + wx[s];
+            const V m = alpha * wp[i];
+            xn[i] = m + wx[i];
             const V m2 = (-alpha) * ap;
-            rn[s] = m2 + wr[s];
+            rn[i] = m2 + wr[i];
           } else {
-            xn = fmav(alpha, wp[s], wx[s]);
-            rn[s] = fmav(-alpha, ap, wr[s]);
+            xn[i] = fmav(alpha, wp[i], wx[i]);
+            rn[i] = fmav(-alpha, ap, wr[i]);
           }
-          un[s] = rn[s] * wd[s];
-          elem(x, v) = xn;
-          elem(rout, v) = rn[s];
-          elem(pout, v) = wp[s];
+          un[i] = rn[i] * wd[i];
         }
-        tu[ou10[s]] = un[s];
+        tu[ou10 + i] = un[i];
       }
+      const Off v = base + Off(2 * t);
+      store_pair(x, v, xn[0], xn[1]);
+      store_pair(rout, v, rn[0], rn[1]);
+      store_pair(pout, v, wp[0], wp[1]);
 #pragma unroll
       for (int s = 0; s < 2; ++s) {
-        if (s == 1 && !r1b) continue;
-        // A non-row cell has dinv 0, so its u is 0 whatever its stencil.
-        const V ap = stencil_s<V, ORD, 144, 12>(tp, hp12[s], hl1[s] >> 2);
+        if (s == 1 && !m1ok) continue;
+        const uint8_t lb = tl1[m1[s]];
+        const V di = s_dinv[lb & 31];
+        // A non-row cell has dinv 0: its u is 0 whatever its stencil.
+        const V ap = stencil_s<V, ORD, kPX, kPY>(tp, r1p[s], lb >> 2);
+        const V rh0 = tr1[m1[s]];
         V rh;
         if (ORD) {
           const V m2 = (-alpha) * ap;
-          rh = m2 + hr1[s];
+          rh = m2 + rh0;
         } else {
-          rh = fmav(-alpha, ap, hr1[s]);
+          rh = fmav(-alpha, ap, rh0);
         }
-        tu[hu10[s]] = hd1[s] == V(0) ? V(0) : rh * hd1[s];
+        tu[r1u[s]] = di == V(0) ? V(0) : rh * di;
       }
     }
     __syncthreads();
-    // Phase 3: A u_{k+1} on the leaf; gamma, delta and r.r of iteration k+1.
+    // Phase 3: A u_{k+1} on the own pair; gamma, delta and r.r of k+1.
     if (valid) {
 #pragma unroll
-      for (int s = 0; s < 2; ++s) {
-        if (!is_fluid(wl[s])) continue;
-        const V au = stencil_s<V, ORD, 100, 10>(tu, ou10[s], wl[s] >> 2);
-        acc[0] += A(rn[s]) * A(un[s]);
-        acc[1] += A(un[s]) * A(au);
-        acc[2] += A(rn[s]) * A(rn[s]);
+      for (int i = 0; i < 2; ++i) {
+        if (!is_fluid(wl[i])) continue;
+        const V au = stencil_s<V, ORD, kUX, kUY>(tu, ou10 + i, wl[i] >> 2);
+        acc[0] += A(rn[i]) * A(un[i]);
+        acc[1] += A(un[i]) * A(au);
+        acc[2] += A(rn[i]) * A(rn[i]);
       }
     }
   }
@@ -1152,12 +1269,27 @@ inline int grid_for(int64_t n, int cap = kBlocksPerXcd) {
   const int64_t per = (chunk + kGroups - 1) / kGroups;
   return int(kXcd * std::max<int64_t>(1, std::min<int64_t>(per, cap)));
 }
-// k_cg runs one leaf group (256 threads) per workgroup; its register use
-// allows five such workgroups per CU with float vectors (160 per XCD) and
-// three with double vectors (96 per XCD).
+// k_cg and k_gamma_delta run one leaf group (256 threads) per workgroup, as
+// many workgroups per CU as the kernel's registers and LDS allow (the
+// runtime's occupancy figure), all resident at once.
 constexpr int kCgGroups = 1;
-inline int grid_cg(int64_t n, int vbytes) {
-  const int64_t cap = vbytes == 8 ? 96 : 160;
+template <typename K>
+int resident_per_xcd(K kernel, int threads) {
+  static int cached = 0;  // one per kernel instantiation
+  if (!cached) {
+    int per_cu = 0, dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess ||
+        per_cu <= 0)
+      return 32;
+    cached = std::max(1, per_cu * std::max(1, ncu / kXcd));
+  }
+  return cached;
+}
+template <typename K>
+inline int grid_cg(int64_t n, K kernel) {
+  const int64_t cap = std::min(resident_per_xcd(kernel, kG * kCgGroups), 256);
   const int64_t chunk = (n + kXcd - 1) / kXcd;
   return int(kXcd * std::max<int64_t>(1, std::min<int64_t>(chunk, cap)));
 }
@@ -1222,8 +1354,18 @@ struct Kernels {
   // Single-reduction path: r_k lives in rb(k & 1), p_k in p[k & 1], and the
   // totals (gamma_k, delta_k, r.r_k) in f[k & 1].
   static V *rb(fsl_ctx *c, int k) { return (V *)(k & 1 ? c->z : c->r); }
+  // The k_cg variant for this context: 32-bit voxel offsets when every
+  // indexed array is under 4 GiB.
+  static auto cg_kernel(fsl_ctx *c) {
+    return uint64_t(c->nvox) * sizeof(V) < (uint64_t(1) << 32)
+               ? k_cg<V, A, ORD, kCgGroups, true>
+               : k_cg<V, A, ORD, kCgGroups, false>;
+  }
+  // Workgroups of every single-reduction kernel (they publish partials for
+  // one another, so they share the count).
+  static int sr_grid(fsl_ctx *c) { return grid_cg(c->n_fluid, k_cg<V, A, ORD, kCgGroups, true>); }
   static void sr_prepare(fsl_ctx *c, int warm) {
-    const int gp = grid_for(c->n_fluid), g = grid_cg(c->n_fluid, sizeof(V));
+    const int gp = grid_for(c->n_fluid), g = sr_grid(c);
     Parts P = parts(c);
     k_prepare<V, A, ORD, MIXED><<<gp, kT, 0, c->stream>>>(
         c->dom, c->fluid_list, c->n_fluid, c->lab, c->div, c->pres, warm, (V *)c->x, rb(c, 0),
@@ -1234,13 +1376,9 @@ struct Kernels {
                                                        c->pcg);
   }
   static void sr_step(fsl_ctx *c, int k, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
-    const int g = grid_cg(c->n_fluid, sizeof(V));
+    const int g = sr_grid(c);
     Parts P = parts(c);
-    // 32-bit voxel offsets when every indexed array is under 4 GiB.
-    auto kern = uint64_t(c->nvox) * sizeof(V) < (uint64_t(1) << 32)
-                    ? k_cg<V, A, ORD, kCgGroups, true>
-                    : k_cg<V, A, ORD, kCgGroups, false>;
-    hipExtLaunchKernelGGL(kern, dim3(g), dim3(kG * kCgGroups), 0, c->stream, e0, e1, 0, c->dom,
+    hipExtLaunchKernelGGL(cg_kernel(c), dim3(g), dim3(kG * kCgGroups), 0, c->stream, e0, e1, 0, c->dom,
                           (const int32_t *)c->fluid_list, c->n_fluid, (const uint8_t *)c->lab,
                           (const V *)rb(c, k), rb(c, k + 1), (const V *)c->p[(k + 1) & 1],
                           (V *)c->p[k & 1], (V *)c->x, (const A *)P.f[k & 1],
@@ -1271,7 +1409,7 @@ void launch_refresh(fsl_ctx *c, int k, int force) {
 // Single-reduction path, MIXED: replace r_kn (input of iteration kn) by the
 // FP64 true residual, and its totals by those of the true residual.
 void launch_refresh_sr(fsl_ctx *c, int kn, int force) {
-  const int g = grid_cg(c->n_fluid, sizeof(float));
+  const int g = Kernels<float, double, false, true>::sr_grid(c);
   Parts P = parts(c);
   float *rbuf = (float *)(kn & 1 ? c->z : c->r);
   k_refresh_x<<<g, kT, 0, c->stream>>>(c->fluid_list, c->n_fluid, c->lab, (float *)c->x, c->x64,
