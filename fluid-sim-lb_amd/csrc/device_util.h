@@ -28,11 +28,48 @@ __device__ inline int64_t xcd_work(int64_t nwork) {
 __device__ inline float fmav(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ inline double fmav(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
+// DPP move of a 32- or 64-bit value (64-bit values move as two halves).
+// Lanes whose row is off in ROW_MASK receive 0.
+template <int CTRL, int ROW_MASK = 0xF, typename T>
+__device__ inline T dpp_move(T v) {
+  static_assert(sizeof(T) == 4 || sizeof(T) == 8, "32- or 64-bit values");
+  if constexpr (sizeof(T) == 4) {
+    const int r = __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, ROW_MASK, 0xF,
+                                              false);
+    return __builtin_bit_cast(T, r);
+  } else {
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, int(b), CTRL, ROW_MASK, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, int(b >> 32), CTRL, ROW_MASK, 0xF, false);
+    return __builtin_bit_cast(T, (long long)(unsigned(lo)) | ((long long)hi << 32));
+  }
+}
+template <typename T>
+__device__ inline T read_lane(T v, int lane) {
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+  } else {
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_readlane(int(b), lane);
+    const int hi = __builtin_amdgcn_readlane(int(b >> 32), lane);
+    return __builtin_bit_cast(T, (long long)(unsigned(lo)) | ((long long)hi << 32));
+  }
+}
+
+// Sum over a 64-lane wavefront, returned to every lane. DPP steps: swaps
+// within quads, rotations within rows of 16, then row 15 / lane 31
+// broadcasts into the upper rows; lane 63 holds the total. The order of the
+// additions is fixed, so the result is deterministic. Lanes must all be
+// active.
 template <typename T>
 __device__ inline T wave_sum(T v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+  v += dpp_move<0xB1>(v);        // quad_perm [1,0,3,2]
+  v += dpp_move<0x4E>(v);        // quad_perm [2,3,0,1]
+  v += dpp_move<0x124>(v);       // row_ror:4
+  v += dpp_move<0x128>(v);       // row_ror:8
+  v += dpp_move<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_move<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+  return read_lane(v, 63);
 }
 __device__ inline float wave_max(float v) {
 #pragma unroll

@@ -174,13 +174,27 @@ __device__ inline Walk walk(int64_t nf) {
 }
 __device__ inline int gtid() { return threadIdx.x % kG; }
 
-// Block-reduce K values and publish this workgroup's partial sums.
+// Block-reduce K values and publish this workgroup's partial sums: wave
+// sums, one barrier, then thread 0 adds the waves' sums in order.
 template <typename A, int K>
 __device__ inline void publish(A vals[K], A *partials) {
   __shared__ A s_red[K][16];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nw = (blockDim.x + 63) >> 6;
 #pragma unroll
-  for (int k = 0; k < K; ++k) vals[k] = block_sum<A>(vals[k], s_red[k]);
+  for (int k = 0; k < K; ++k) vals[k] = wave_sum(vals[k]);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) s_red[k][wave] = vals[k];
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      A t = A(0);
+      for (int w = 0; w < nw; ++w) t += s_red[k][w];
+      vals[k] = t;
+    }
 #pragma unroll
     for (int k = 0; k < K; ++k) partials[int64_t(blockIdx.x) * K + k] = vals[k];
   }
@@ -704,60 +718,57 @@ __device__ inline int64_t rel_voxel(const Dom &d, int64_t leaf, int x, int y, in
   const int64_t nb = leaf + dx * l12 + dy * int64_t(d.L[2]) + dz;
   return (nb << 9) | ((x & 7) << 6) | ((y & 7) << 3) | (z & 7);
 }
-// Halo runs of the single-reduction pass. The 864 halo cells of a leaf (face
-// layers at distance 1 and 2, edge rows at distance (1, 1)) are loaded as
-// runs of consecutive cells along z, 4, 2 or 1 cells per lane, so a leaf's
-// halo takes at most two load instructions per wave and array:
-//   slot 0, wave 0: x faces, layers x = -2, -1, 8, 9: 64 runs of 4;
-//   slot 0, wave 1: y faces, layers y = -2, -1, 8, 9: 64 runs of 4;
-//   slot 0, waves 2, 3: z faces, z = (-2, -1) and (8, 9): 128 runs of 2;
-//   slot 1, wave 0, lanes 0-7: xy edge rows: 8 runs of 4;
-//   slot 1, waves 1, 2, lanes 0-31: xz and yz edge rows: 64 single cells.
-// Idle lanes of a loading wave load a run of the same slot and discard it.
+// Halo pairs of the single-reduction pass. The 864 halo cells of a leaf
+// (face layers at distance 1 and 2, edge rows at distance (1, 1)) are loaded
+// as 464 pairs of z-neighbours (two values per lane, one vector access):
+//   [0, 128)   x faces, x = -2, -1, 8, 9;
+//   [128, 256) y faces, y = -2, -1, 8, 9;
+//   [256, 384) z faces, z = (-2, -1) and (8, 9);
+//   [384, 400) xy edge rows (x, y = -1 or 8), z pairs;
+//   [400, 432) xz edge rows (x = -1 or 8, z = -1 or 8), loaded with the cell
+//              beyond the edge in z (the pair (-2, -1) or (8, 9));
+//   [432, 464) yz edge rows, likewise.
+// Pair slot 0 of thread t is pair t; slot 1 is pair 256 + wave * 52 + lane
+// on lanes 0-51 (every wave carries the same work); idle lanes load pair 0
+// and discard it.
 struct Run {
-  int x, y, z;  // first cell, relative to the leaf origin
-  int n;        // cells in the run (0: the lane's slot is idle)
+  int x, y, z;  // first cell of the pair, relative to the leaf origin
 };
-__device__ inline int run_width(int w, int s) {
-  if (s == 0) return w <= 1 ? 4 : 2;
-  return w == 0 ? 4 : (w <= 2 ? 1 : 0);
+__device__ inline Run halo_pair(int u) {
+  if (u < 256) {
+    const int side = (u >> 6) & 1, i = u & 63, layer = i >> 5;
+    const int c = side ? 8 + layer : layer - 2;
+    if (u < 128) return Run{c, (i >> 2) & 7, (i & 3) * 2};
+    return Run{i >> 3, (i >> 2) & 1 ? (side ? 9 : -1) : (side ? 8 : -2), (i & 3) * 2};
+  }
+  if (u < 384) {
+    const int side = (u >> 6) & 1, i = u & 63;
+    return Run{i >> 3, i & 7, side ? 8 : -2};
+  }
+  if (u < 400) {
+    const int e = (u - 384) >> 2;
+    return Run{(e & 2) ? 8 : -1, (e & 1) ? 8 : -1, (u & 3) * 2};
+  }
+  const int e = ((u - 400) >> 3) & 3, f = u & 7, a = (e & 2) ? 8 : -1, b = (e & 1) ? 8 : -2;
+  return u < 432 ? Run{a, f, b} : Run{f, a, b};
 }
-__device__ inline Run halo_run(int w, int l, int s) {
-  if (s == 0) {
-    if (w <= 1) {
-      const int side = l >> 5, i = l & 31;
-      const int a = i >> 2, layer = (i >> 1) & 1, z = (i & 1) * 4;
-      const int c = side ? 8 + layer : layer - 2;
-      return w == 0 ? Run{c, a, z, 4} : Run{a, c, z, 4};
-    }
-    const int u = (w - 2) * 64 + l, side = u >> 6, i = u & 63;
-    return Run{i >> 3, i & 7, side ? 8 : -2, 2};
-  }
-  if (w == 0) {
-    const int e = (l >> 1) & 3;  // lanes 8-63 repeat lanes 0-7
-    return Run{(e & 2) ? 8 : -1, (e & 1) ? 8 : -1, (l & 1) * 4, l < 8 ? 4 : 0};
-  }
-  if (w <= 2) {
-    const int e = (l >> 3) & 3, f = l & 7, a = (e & 2) ? 8 : -1, b = (e & 1) ? 8 : -1;
-    const int n = l < 32 ? 1 : 0;  // lanes 32-63 repeat lanes 0-31
-    return w == 1 ? Run{a, f, b, n} : Run{f, a, b, n};
-  }
-  return Run{-1, -1, 0, 0};
+__device__ inline int pair_of(int t, int s) {
+  const int w = t >> 6, l = t & 63;
+  return s == 0 ? t : (l < 52 ? 256 + w * 52 + l : -1);
 }
-// Distance-1 cells of a run: bit e of the mask is set when cell e of the run
-// is distance-1 cell base + e (index f * 64 + a * 8 + b of fhalo_cell).
-__device__ inline void run_r1(const Run &r, int s, int &mask, int &base) {
+// Distance-1 cells of a pair: bit e of the mask is set when cell e of the
+// pair is distance-1 cell base + e (index f * 64 + a * 8 + b of fhalo_cell).
+__device__ inline void pair_r1(int u, const Run &r, int &mask, int &base) {
   mask = 0;
   base = 0;
-  if (s != 0) return;
-  if (r.n == 4) {
-    const bool xf = r.x < 0 || r.x > 7;
+  if (u < 256) {
+    const bool xf = u < 128;
     const int c = xf ? r.x : r.y;
     if (c == -1 || c == 8) {
-      mask = 0xF;
+      mask = 3;
       base = (xf ? (c < 0 ? 0 : 64) : (c < 0 ? 128 : 192)) + (xf ? r.y : r.x) * 8 + r.z;
     }
-  } else {
+  } else if (u < 384) {
     const bool lo = r.z < 0;
     mask = lo ? 2 : 1;
     base = (lo ? 256 : 320) + r.x * 8 + r.y - (lo ? 1 : 0);
@@ -766,7 +777,7 @@ __device__ inline void run_r1(const Run &r, int s, int &mask, int &base) {
 
 // N consecutive elements from element i (i aligned to N), one vector access.
 template <int N, typename T, typename Off>
-__device__ inline void load_run(const T *p, Off i, T out[4]);
+__device__ inline void load_run(const T *p, Off i, T *out);
 // N consecutive labels from voxel i, packed little-endian.
 template <int N, typename Off>
 __device__ inline uint32_t load_labels(const uint8_t *lab, Off i);
@@ -806,7 +817,7 @@ template <typename T>
 __device__ inline T &elem(T *p, int64_t i) { return p[i]; }
 
 template <int N, typename T, typename Off>
-__device__ inline void load_run(const T *p, Off i, T out[4]) {
+__device__ inline void load_run(const T *p, Off i, T *out) {
   if constexpr (N == 1) {
     out[0] = elem(p, i);
   } else {
@@ -844,7 +855,8 @@ template <typename V, typename A, bool ORD, int GR, bool OFF32>
 __global__ __launch_bounds__(kG * GR) void k_cg(Dom d, const int32_t *flist, int64_t nf,
                                                 const uint8_t *lab, const V *rin, V *rout,
                                                 const V *pin, V *pout, V *x, const A *part_in,
-                                                A *part_out, int g_parts, int k, PcgState *st) {
+                                                A *part_out, int g_parts, int k, int mode,
+                                                PcgState *st) {
 #pragma clang fp contract(off)
   using Off = typename std::conditional<OFF32, uint32_t, int64_t>::type;
   __shared__ V s_p[GR][kPSize];
@@ -868,19 +880,20 @@ __global__ __launch_bounds__(kG * GR) void k_cg(Dom d, const int32_t *flist, int
   // first voxel, P tile position, distance-1 cells), the two distance-1
   // cells of phase 2 (tile positions) and the own pair (tile positions).
   const int64_t l12 = int64_t(d.L[1]) * d.L[2];
-  const int wid0 = run_width(w, 0), wid1 = run_width(w, 1);
   int32_t hoff[2];
-  int hp0[2], rmask[2], rbase[2], hn[2];
+  int hp0[2], rmask[2], rbase[2];
+  bool hon[2];
 #pragma unroll
   for (int s = 0; s < 2; ++s) {
-    const Run r = halo_run(w, l, s);
+    const int u = pair_of(t, s);
+    hon[s] = u >= 0;
+    const Run r = halo_pair(hon[s] ? u : 0);
     const int dx = r.x < 0 ? -1 : (r.x > 7 ? 1 : 0), dy = r.y < 0 ? -1 : (r.y > 7 ? 1 : 0),
               dz = r.z < 0 ? -1 : (r.z > 7 ? 1 : 0);
     hoff[s] = int32_t((dx * l12 + dy * int64_t(d.L[2]) + dz) * kLeaf) |
               ((r.x & 7) << 6) | ((r.y & 7) << 3) | (r.z & 7);
     hp0[s] = t12(r.x, r.y, r.z);
-    hn[s] = r.n;
-    run_r1(r, s, rmask[s], rbase[s]);
+    pair_r1(hon[s] ? u : 512, r, rmask[s], rbase[s]);
   }
   const bool m1ok = l < 32;
   const int m1[2] = {t, 256 + w * 32 + (l & 31)};
@@ -895,7 +908,7 @@ __global__ __launch_bounds__(kG * GR) void k_cg(Dom d, const int32_t *flist, int
   const int op12 = t12_of_local(2 * t), ou10 = u10_of_local(2 * t);  // cell 2t+1 is +1
 
   struct Pre {
-    V orr[2], op[2], ox[2], hr[2][4], hp[2][4];
+    V orr[2], op[2], ox[2], hr[2][2], hp[2][2];
     uint32_t ol, hl[2];
     Off base;
     bool valid;
@@ -911,24 +924,13 @@ __global__ __launch_bounds__(kG * GR) void k_cg(Dom d, const int32_t *flist, int
     c.ol = load_labels<2>(lab, v);
     load_run<2>(rin, v, c.orr);
     load_run<2>(pin, v, c.op);
-    load_run<2>(x, v, c.ox);
+    if (mode & 1) load_run<2>(x, v, c.ox);
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
       const Off hv = c.base + Off(hoff[s]);
-      const int wd = s ? wid1 : wid0;
-      if (wd == 4) {
-        c.hl[s] = load_labels<4>(lab, hv);
-        load_run<4>(rin, hv, c.hr[s]);
-        load_run<4>(pin, hv, c.hp[s]);
-      } else if (wd == 2) {
-        c.hl[s] = load_labels<2>(lab, hv);
-        load_run<2>(rin, hv, c.hr[s]);
-        load_run<2>(pin, hv, c.hp[s]);
-      } else if (wd == 1) {
-        c.hl[s] = load_labels<1>(lab, hv);
-        load_run<1>(rin, hv, c.hr[s]);
-        load_run<1>(pin, hv, c.hp[s]);
-      }
+      c.hl[s] = load_labels<2>(lab, hv);
+      load_run<2>(rin, hv, c.hr[s]);
+      load_run<2>(pin, hv, c.hp[s]);
     }
   };
   const Walk q0 = walk(nf);
@@ -975,7 +977,34 @@ __global__ __launch_bounds__(kG * GR) void k_cg(Dom d, const int32_t *flist, int
     if (status) st->status = status;
     if (stop) st->done = 1;
   }
-  if (stop) return;
+  // alpha_{k-1}: x lacks alpha_{k-1} p_{k-1} when bit 1 of mode is set.
+  const V alpha_prev = V(st->alph[(k + 1) & 1]);
+  if (stop) {
+    if (status == 0 && (mode & 2)) {
+      // Stopping at x_k: apply the pending alpha_{k-1} p_{k-1} (p_{k-1} is
+      // the input direction).
+      for (Walk w2 = q0; w2.more(); w2.next()) {
+        if (!w2.valid()) continue;
+        const Off v = (Off(flist[w2.mine()]) << 9) + Off(2 * t);
+        const uint32_t lb = load_labels<2>(lab, v);
+        V pp[2], xx[2];
+        load_run<2>(pin, v, pp);
+        load_run<2>(x, v, xx);
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          if (!is_fluid(uint8_t(lb >> (8 * i)))) continue;
+          if (ORD) {
+            const V m = alpha_prev * pp[i];
+            xx[i] = m + xx[i];
+          } else {
+            xx[i] = fmav(alpha_prev, pp[i], xx[i]);
+          }
+        }
+        store_pair(x, v, xx[0], xx[1]);
+      }
+    }
+    return;
+  }
   const V alpha = V(alpha_a), bv = V(beta);
   // p = u + beta p_old with u = dinv * r; the select drops whatever a
   // non-row cell holds.
@@ -997,7 +1026,7 @@ __global__ __launch_bounds__(kG * GR) void k_cg(Dom d, const int32_t *flist, int
     // and labels to LDS.
     const bool valid = cur.valid;
     const Off base = cur.base;
-    V wr[2], wp[2], wx[2], wd[2];
+    V wr[2], wp[2], wx[2], wd[2], wo[2];
     uint8_t wl[2];
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
@@ -1005,17 +1034,16 @@ __global__ __launch_bounds__(kG * GR) void k_cg(Dom d, const int32_t *flist, int
       wp[i] = pnew(cur.orr[i], cur.op[i], wl[i], wd[i]);
       wr[i] = cur.orr[i];
       wx[i] = cur.ox[i];
+      wo[i] = cur.op[i];
     }
     if (valid) {
       tp[op12] = wp[0];
       tp[op12 + 1] = wp[1];
 #pragma unroll
       for (int s = 0; s < 2; ++s) {
-        if (hn[s] == 0) continue;
-        const int wd_s = s ? wid1 : wid0;
+        if (!hon[s]) continue;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          if (e >= wd_s) break;
+        for (int e = 0; e < 2; ++e) {
           const uint32_t lb = cur.hl[s] >> (8 * e);
           V di;
           tp[hp0[s] + e] = pnew(cur.hr[s][e], cur.hp[s][e], lb, di);
@@ -1050,13 +1078,20 @@ __global__ __launch_bounds__(kG * GR) void k_cg(Dom d, const int32_t *flist, int
         xn[i] = wx[i];
         if (is_fluid(wl[i])) {
           const V ap = stencil_s<V, ORD, kPX, kPY>(tp, op12 + i, wl[i] >> 2);
+          // x: alpha_{k-1} p_{k-1} (when pending) then alpha_k p_k, the two
+          // axpys of consecutive iterations in order.
           if (ORD) {
+            if (mode & 2) {
+              const V m0 = alpha_prev * wo[i];
+              xn[i] = m0 + xn[i];
+            }
             const V m = alpha * wp[i];
-            xn[i] = m + wx[i];
+            xn[i] = m + xn[i];
             const V m2 = (-alpha) * ap;
             rn[i] = m2 + wr[i];
           } else {
-            xn[i] = fmav(alpha, wp[i], wx[i]);
+            if (mode & 2) xn[i] = fmav(alpha_prev, wo[i], xn[i]);
+            xn[i] = fmav(alpha, wp[i], xn[i]);
             rn[i] = fmav(-alpha, ap, wr[i]);
           }
           un[i] = rn[i] * wd[i];
@@ -1064,7 +1099,7 @@ __global__ __launch_bounds__(kG * GR) void k_cg(Dom d, const int32_t *flist, int
         tu[ou10 + i] = un[i];
       }
       const Off v = base + Off(2 * t);
-      store_pair(x, v, xn[0], xn[1]);
+      if (mode & 1) store_pair(x, v, xn[0], xn[1]);
       store_pair(rout, v, rn[0], rn[1]);
       store_pair(pout, v, wp[0], wp[1]);
 #pragma unroll
@@ -1375,14 +1410,17 @@ struct Kernels {
     k_prepare_scalars_sr<A><<<1, 1024, 0, c->stream>>>((const A *)P.f[0], g, (const A *)P.bb, gp,
                                                        c->pcg);
   }
-  static void sr_step(fsl_ctx *c, int k, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+  // mode: bit 0, x is updated in iteration k (odd k); bit 1, x lacks
+  // alpha_{k-1} p_{k-1} (deferred by iteration k-1).
+  static void sr_step(fsl_ctx *c, int k, int mode, hipEvent_t e0 = nullptr,
+                      hipEvent_t e1 = nullptr) {
     const int g = sr_grid(c);
     Parts P = parts(c);
     hipExtLaunchKernelGGL(cg_kernel(c), dim3(g), dim3(kG * kCgGroups), 0, c->stream, e0, e1, 0, c->dom,
                           (const int32_t *)c->fluid_list, c->n_fluid, (const uint8_t *)c->lab,
                           (const V *)rb(c, k), rb(c, k + 1), (const V *)c->p[(k + 1) & 1],
                           (V *)c->p[k & 1], (V *)c->x, (const A *)P.f[k & 1],
-                          (A *)P.f[(k + 1) & 1], g, k, c->pcg);
+                          (A *)P.f[(k + 1) & 1], g, k, mode, c->pcg);
   }
   static void true_residual(fsl_ctx *c) {
     const int g = grid_for(c->n_fluid);
@@ -1461,7 +1499,9 @@ static int pcg_setup(fsl_ctx *ctx, const fsl_solve_params *prm) {
   }
   ctx->pcg_precision = prm->precision;
   ctx->pcg_stop = prm->stop_mode;
-  ctx->pcg_refresh = prm->refresh_every > 0 ? prm->refresh_every : 32;
+  // An even period: the single-reduction path's x is complete after odd
+  // iterations only.
+  ctx->pcg_refresh = prm->refresh_every > 0 ? prm->refresh_every + (prm->refresh_every & 1) : 32;
   PcgState *h = ctx->pcg_host;
   *h = PcgState{};
   h->max_iter = prm->max_iterations;
@@ -1536,13 +1576,18 @@ static int pcg_run_sr(fsl_ctx *ctx, const fsl_solve_params *prm, fsl_solve_stats
   FSL_HIP(hipEventRecord(e0, ctx->stream));
   hipEvent_t *pev = ctx->tev + 32;
   int k = 0;  // next iteration to launch
+  // x is updated every other iteration: iteration k odd applies
+  // alpha_{k-1} p_{k-1} + alpha_k p_k; a kernel that stops at an odd k applies
+  // the pending term. After a forced refresh x is complete (resume_k).
+  int resume_k = -1;
   while (!ctx->pcg_host->done) {
     const int k0 = k;
     for (int b = 0; b < nb; ++b, ++k) {
       hipEvent_t *e = pev + 2 * b;
       const bool t = ctx->timing;
+      const int mode = (k & 1) ? (k == resume_k ? 1 : 3) : 0;
       dispatch(ctx->pcg_precision, [&](auto K) {
-        decltype(K)::sr_step(ctx, k, t ? e[0] : nullptr, t ? e[1] : nullptr);
+        decltype(K)::sr_step(ctx, k, mode, t ? e[0] : nullptr, t ? e[1] : nullptr);
       });
       if (mixed && (k + 1) % ctx->pcg_refresh == 0) launch_refresh_sr(ctx, k + 1, 0);
     }
@@ -1582,7 +1627,10 @@ static int pcg_run_sr(fsl_ctx *ctx, const fsl_solve_params *prm, fsl_solve_stats
       FSL_HIP(hipGetLastError());
       rc = sync_state(ctx);
       if (rc) return rc;
-      if (!ctx->pcg_host->done) k = last;
+      if (!ctx->pcg_host->done) {
+        k = last;
+        resume_k = last;
+      }
     }
   }
   FSL_HIP(hipEventRecord(e1, ctx->stream));
