@@ -38,9 +38,11 @@ HBM_PEAK_GBS = 8000.0
 # r, Ap, label and writes x, r, z.
 PASS_A_BYTES = {fsl.PCG_FP32: 17, fsl.PCG_MIXED: 17, fsl.PCG_FP64: 33}
 PASS_B_BYTES = {fsl.PCG_FP32: 29, fsl.PCG_MIXED: 29, fsl.PCG_FP64: 57}
-# Single-reduction pass (one kernel per iteration): reads r, p, x, label and
-# writes r, p, x (the two-cell halo re-reads are not counted).
-SR_BYTES = {fsl.PCG_FP32: 25, fsl.PCG_MIXED: 25, fsl.PCG_FP64: 49}
+# Single-reduction pass (one kernel per iteration): reads r, p, label and
+# writes r, p every iteration; reads and writes x every other iteration (the
+# two-cell halo re-reads are not counted): 17 + 8/2 = 21 B per row with float
+# vectors, 33 + 16/2 = 41 with double vectors.
+SR_BYTES = {fsl.PCG_FP32: 21, fsl.PCG_MIXED: 21, fsl.PCG_FP64: 41}
 PREC_NAME = {fsl.PCG_FP32: "fp32", fsl.PCG_FP64: "fp64", fsl.PCG_MIXED: "mixed"}
 # Algorithmic bytes per particle: P2G reads position and velocity (24 B) and
 # writes 12 B of faces + 1 B of label per cell (8 particles per cell); G2P

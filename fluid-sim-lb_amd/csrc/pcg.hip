@@ -218,29 +218,19 @@ __device__ inline double total(const A *partials, int n, int k) {
 }
 
 // Totals of all K values of the n published (stride K) partials in one pass:
-// one load round and one combined workgroup reduction, in A's precision
-// (Neumaier-compensated for double), the same in every workgroup.
+// one load round and one combined workgroup reduction, in A's precision, the
+// same in every workgroup. A thread adds at most a few partials (n is at most
+// a few thousand workgroups), so plain sums in a fixed order suffice: the
+// rounding error is a few units of 1e-16 relative to the sum of magnitudes.
 template <typename A, int K>
 __device__ inline void total_all(const A *partials, int n, double out[K]) {
   __shared__ A s_t[K][16];
   A v[K];
-  if constexpr (sizeof(A) == 8) {
-    double a[K], c[K];
 #pragma unroll
-    for (int j = 0; j < K; ++j) { a[j] = 0.0; c[j] = 0.0; }
-    for (int q = threadIdx.x; q < n; q += blockDim.x) {
+  for (int j = 0; j < K; ++j) v[j] = A(0);
+  for (int q = threadIdx.x; q < n; q += blockDim.x) {
 #pragma unroll
-      for (int j = 0; j < K; ++j) neumaier_add(a[j], c[j], double(partials[K * q + j]));
-    }
-#pragma unroll
-    for (int j = 0; j < K; ++j) v[j] = a[j] + c[j];
-  } else {
-#pragma unroll
-    for (int j = 0; j < K; ++j) v[j] = A(0);
-    for (int q = threadIdx.x; q < n; q += blockDim.x) {
-#pragma unroll
-      for (int j = 0; j < K; ++j) v[j] += partials[K * q + j];
-    }
+    for (int j = 0; j < K; ++j) v[j] += partials[K * q + j];
   }
 #pragma unroll
   for (int j = 0; j < K; ++j) v[j] = wave_sum(v[j]);
@@ -1127,9 +1117,11 @@ __global__ __launch_bounds__(kG * GR) void k_cg(Dom d, const int32_t *flist, int
       for (int i = 0; i < 2; ++i) {
         if (!is_fluid(wl[i])) continue;
         const V au = stencil_s<V, ORD, kUX, kUY>(tu, ou10 + i, wl[i] >> 2);
-        acc[0] += A(rn[i]) * A(un[i]);
-        acc[1] += A(un[i]) * A(au);
-        acc[2] += A(rn[i]) * A(rn[i]);
+        // FP64 accumulation with one rounding per term (a float product is
+        // exact in double, so the fma only rounds the sum).
+        acc[0] = fmav(A(rn[i]), A(un[i]), acc[0]);
+        acc[1] = fmav(A(un[i]), A(au), acc[1]);
+        acc[2] = fmav(A(rn[i]), A(rn[i]), acc[2]);
       }
     }
   }
