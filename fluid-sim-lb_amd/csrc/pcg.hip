@@ -844,9 +844,9 @@ __device__ inline void store_pair(T *p, Off i, T a, T b) {
 template <typename V, typename A, bool ORD, int GR, bool OFF32>
 __global__ __launch_bounds__(kG * GR) void k_cg(Dom d, const int32_t *flist, int64_t nf,
                                                 const uint8_t *lab, const V *rin, V *rout,
-                                                const V *pin, V *pout, V *x, const A *part_in,
-                                                A *part_out, int g_parts, int k, int mode,
-                                                PcgState *st) {
+                                                const V *pin, V *pout, V *x, double *x64,
+                                                const A *part_in, A *part_out, int g_parts,
+                                                int k, int mode, PcgState *st) {
 #pragma clang fp contract(off)
   using Off = typename std::conditional<OFF32, uint32_t, int64_t>::type;
   __shared__ V s_p[GR][kPSize];
@@ -1089,7 +1089,20 @@ __global__ __launch_bounds__(kG * GR) void k_cg(Dom d, const int32_t *flist, int
         tu[ou10 + i] = un[i];
       }
       const Off v = base + Off(2 * t);
-      if (mode & 1) store_pair(x, v, xn[0], xn[1]);
+      if (mode & 4) {
+        // Fold before a residual refresh: x64 += x_{k+1}, x = 0 (x and x64
+        // are read only at a thread's own cells, so no other workgroup sees
+        // the intermediate state).
+        typedef double D2 __attribute__((ext_vector_type(2)));
+        D2 *xp = reinterpret_cast<D2 *>(&elem(x64, v));
+        D2 acc64 = *xp;
+        if (is_fluid(wl[0])) acc64[0] += double(xn[0]);
+        if (is_fluid(wl[1])) acc64[1] += double(xn[1]);
+        *xp = acc64;
+        store_pair(x, v, V(0), V(0));
+      } else if (mode & 1) {
+        store_pair(x, v, xn[0], xn[1]);
+      }
       store_pair(rout, v, rn[0], rn[1]);
       store_pair(pout, v, wp[0], wp[1]);
 #pragma unroll
@@ -1126,6 +1139,134 @@ __global__ __launch_bounds__(kG * GR) void k_cg(Dom d, const int32_t *flist, int
     }
   }
   publish<A, 3>(acc, part_out);
+}
+
+// MIXED residual refresh (single-reduction path): r = b - A x64 in FP64 on
+// the leaf and its distance-1 cells from x64 staged on the two-cell halo
+// (the layout of k_cg: own z-pairs, 464 halo pairs, distance-1 cells through
+// LDS), stored as float on the leaf; then u = M^-1 r, A u on the leaf, and
+// the partials (gamma, delta, r.r) of the refreshed residual. x has been
+// folded into x64 before (by the k_cg launch before the refresh, or by
+// k_refresh_x).
+template <bool OFF32>
+__global__ __launch_bounds__(kG) void k_refresh_cg(Dom d, const int32_t *flist, int64_t nf,
+                                                   const uint8_t *lab, const float *div,
+                                                   const double *x64, float *rout,
+                                                   double *part_out, const PcgState *st,
+                                                   int force) {
+#pragma clang fp contract(off)
+  using Off = typename std::conditional<OFF32, uint32_t, int64_t>::type;
+  __shared__ double s_p[kPSize];
+  __shared__ float s_u[kUSize];
+  __shared__ float s_b1[384];
+  __shared__ uint8_t s_l1[384];
+  __shared__ float s_dinv[32];
+  if ((!force && st->done) || st->status) return;
+  const int t = threadIdx.x;
+  const int w = __builtin_amdgcn_readfirstlane(t >> 6), l = t & 63;
+  if (t < 32) s_dinv[t] = is_fluid(uint8_t(t)) ? dinv<float>(t >> 2) : 0.0f;
+  const int64_t l12 = int64_t(d.L[1]) * d.L[2];
+  int32_t hoff[2];
+  int hp0[2], rmask[2], rbase[2];
+  bool hon[2];
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const int u = pair_of(t, s);
+    hon[s] = u >= 0;
+    const Run r = halo_pair(hon[s] ? u : 0);
+    const int dx = r.x < 0 ? -1 : (r.x > 7 ? 1 : 0), dy = r.y < 0 ? -1 : (r.y > 7 ? 1 : 0),
+              dz = r.z < 0 ? -1 : (r.z > 7 ? 1 : 0);
+    hoff[s] = int32_t((dx * l12 + dy * int64_t(d.L[2]) + dz) * kLeaf) |
+              ((r.x & 7) << 6) | ((r.y & 7) << 3) | (r.z & 7);
+    hp0[s] = t12(r.x, r.y, r.z);
+    pair_r1(hon[s] ? u : 512, r, rmask[s], rbase[s]);
+  }
+  const bool m1ok = l < 32;
+  const int m1[2] = {t, 256 + w * 32 + (l & 31)};
+  int r1p[2], r1u[2];
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    int hx, hy, hz;
+    fhalo_cell(m1[s], hx, hy, hz);
+    r1p[s] = t12(hx, hy, hz);
+    r1u[s] = u10(hx, hy, hz);
+  }
+  const int op12 = t12_of_local(2 * t), ou10 = u10_of_local(2 * t);
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (Walk q = walk(nf); q.more(); q.next()) {
+    // Every round has a valid leaf: one group per workgroup.
+    const Off base = Off(flist[q.mine()]) << 9;
+    const Off v = base + Off(2 * t);
+    const uint32_t ol = load_labels<2>(lab, v);
+    double ox[2];
+    float ob[2];
+    load_run<2>(x64, v, ox);
+    load_run<2>(div, v, ob);
+    double hx[2][2];
+    float hb[2][2];
+    uint32_t hl[2];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const Off hv = base + Off(hoff[s]);
+      hl[s] = load_labels<2>(lab, hv);
+      load_run<2>(x64, hv, hx[s]);
+      load_run<2>(div, hv, hb[s]);
+    }
+    // Phase 1: x64 on the 1376 cells (0 off the rows) into the tile.
+    uint8_t wl[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      wl[i] = uint8_t(ol >> (8 * i));
+      s_p[op12 + i] = is_fluid(wl[i]) ? ox[i] : 0.0;
+    }
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      if (!hon[s]) continue;
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        const uint8_t lb = uint8_t(hl[s] >> (8 * e));
+        s_p[hp0[s] + e] = is_fluid(lb) ? hx[s][e] : 0.0;
+        if ((rmask[s] >> e) & 1) {
+          s_b1[rbase[s] + e] = hb[s][e];
+          s_l1[rbase[s] + e] = lb;
+        }
+      }
+    }
+    __syncthreads();
+    // Phase 2: r = b - A x64 (FP64) on the own pair and the distance-1 cells.
+    float rn[2], un[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      rn[i] = 0.0f;
+      if (is_fluid(wl[i]))
+        rn[i] = float(double(ob[i]) - stencil_s<double, false, kPX, kPY>(s_p, op12 + i, wl[i] >> 2));
+      un[i] = rn[i] * s_dinv[wl[i] & 31];
+      s_u[ou10 + i] = un[i];
+    }
+    store_pair(rout, v, rn[0], rn[1]);
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      if (s == 1 && !m1ok) continue;
+      const uint8_t lb = s_l1[m1[s]];
+      const float di = s_dinv[lb & 31];
+      float uh = 0.0f;
+      if (di != 0.0f)
+        uh = float(double(s_b1[m1[s]]) - stencil_s<double, false, kPX, kPY>(s_p, r1p[s], lb >> 2)) * di;
+      s_u[r1u[s]] = uh;
+    }
+    __syncthreads();
+    // Phase 3: A u on the own pair; gamma, delta, r.r in FP64.
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      if (!is_fluid(wl[i])) continue;
+      const float au = stencil_s<float, false, kUX, kUY>(s_u, ou10 + i, wl[i] >> 2);
+      acc[0] = fmav(double(rn[i]), double(un[i]), acc[0]);
+      acc[1] = fmav(double(un[i]), double(au), acc[1]);
+      acc[2] = fmav(double(rn[i]), double(rn[i]), acc[2]);
+    }
+    __syncthreads();
+  }
+  publish<double, 3>(acc, part_out);
 }
 
 // gamma = r.u, delta = u.Au and r.r for a residual r (u = M^-1 r), from a
@@ -1384,7 +1525,8 @@ struct Kernels {
   // The k_cg variant for this context: 32-bit voxel offsets when every
   // indexed array is under 4 GiB.
   static auto cg_kernel(fsl_ctx *c) {
-    return uint64_t(c->nvox) * sizeof(V) < (uint64_t(1) << 32)
+    // x64 (double) is indexed too: 8 bytes per voxel.
+    return uint64_t(c->nvox) * 8 < (uint64_t(1) << 32)
                ? k_cg<V, A, ORD, kCgGroups, true>
                : k_cg<V, A, ORD, kCgGroups, false>;
   }
@@ -1411,7 +1553,7 @@ struct Kernels {
     hipExtLaunchKernelGGL(cg_kernel(c), dim3(g), dim3(kG * kCgGroups), 0, c->stream, e0, e1, 0, c->dom,
                           (const int32_t *)c->fluid_list, c->n_fluid, (const uint8_t *)c->lab,
                           (const V *)rb(c, k), rb(c, k + 1), (const V *)c->p[(k + 1) & 1],
-                          (V *)c->p[k & 1], (V *)c->x, (const A *)P.f[k & 1],
+                          (V *)c->p[k & 1], (V *)c->x, c->x64, (const A *)P.f[k & 1],
                           (A *)P.f[(k + 1) & 1], g, k, mode, c->pcg);
   }
   static void true_residual(fsl_ctx *c) {
@@ -1442,12 +1584,14 @@ void launch_refresh_sr(fsl_ctx *c, int kn, int force) {
   const int g = Kernels<float, double, false, true>::sr_grid(c);
   Parts P = parts(c);
   float *rbuf = (float *)(kn & 1 ? c->z : c->r);
-  k_refresh_x<<<g, kT, 0, c->stream>>>(c->fluid_list, c->n_fluid, c->lab, (float *)c->x, c->x64,
-                                       c->pcg, force);
-  k_refresh_r<<<g, kT, 0, c->stream>>>(c->dom, c->fluid_list, c->n_fluid, c->lab, c->div,
-                                       c->x64, rbuf, nullptr, nullptr, c->pcg, force);
-  k_gamma_delta<float, double, false, kCgGroups><<<g, kG * kCgGroups, 0, c->stream>>>(
-      c->dom, c->fluid_list, c->n_fluid, c->lab, rbuf, (double *)P.f[kn & 1], c->pcg, force);
+  // Before a scheduled refresh the k_cg launch has folded x into x64; a
+  // forced refresh (convergence check) folds here.
+  if (force)
+    k_refresh_x<<<grid_for(c->n_fluid), kT, 0, c->stream>>>(c->fluid_list, c->n_fluid, c->lab,
+                                                          (float *)c->x, c->x64, c->pcg, force);
+  auto kern = uint64_t(c->nvox) * 8 < (uint64_t(1) << 32) ? k_refresh_cg<true> : k_refresh_cg<false>;
+  kern<<<g, kG, 0, c->stream>>>(c->dom, c->fluid_list, c->n_fluid, c->lab, c->div, c->x64, rbuf,
+                                (double *)P.f[kn & 1], c->pcg, force);
   if (force) k_refresh_scalars_sr<<<1, 1024, 0, c->stream>>>((const double *)P.f[kn & 1], g, c->pcg);
 }
 
@@ -1577,7 +1721,8 @@ static int pcg_run_sr(fsl_ctx *ctx, const fsl_solve_params *prm, fsl_solve_stats
     for (int b = 0; b < nb; ++b, ++k) {
       hipEvent_t *e = pev + 2 * b;
       const bool t = ctx->timing;
-      const int mode = (k & 1) ? (k == resume_k ? 1 : 3) : 0;
+      const int mode = ((k & 1) ? (k == resume_k ? 1 : 3) : 0) |
+                       (mixed && (k + 1) % ctx->pcg_refresh == 0 ? 4 : 0);
       dispatch(ctx->pcg_precision, [&](auto K) {
         decltype(K)::sr_step(ctx, k, mode, t ? e[0] : nullptr, t ? e[1] : nullptr);
       });
