@@ -213,56 +213,55 @@ template <> struct Axes<0> { static constexpr int a = 1, b = 2; };
 template <> struct Axes<1> { static constexpr int a = 0, b = 2; };
 template <> struct Axes<2> { static constexpr int a = 0, b = 1; };
 
-// Weight of corner (cD, ca, cb) for component D, or 0 when out of range.
-template <int D>
-__device__ inline float corner(const float wt[2][2][2], int cD, int ca, int cb) {
-  if (ca < 0 || ca > 1 || cb < 0 || cb > 1) return 0.0f;
-  int c[3];
-  c[D] = cD; c[Axes<D>::a] = ca; c[Axes<D>::b] = cb;
-  return wt[c[0]][c[1]][c[2]];
-}
-
 // Accumulate one particle into the 18 sample slots (slot = sd*9 + sa*3 + sb;
 // sample offset from the cell: +sd along D, sa-1 along a, sb-1 along b).
-// Weights follow the reference's factored form (flip_simulation.cc:977-990).
+// The trilinear weights of ScatterWeight (flip_simulation.cc:936-1009) are
+// formed as products of per-axis factors: along D, (1 - t, t) on the slots
+// sd = 0, 1; along a (and b), (1 - t, t) on the slots (0, 1) when the
+// particle lies below the cell's centre line (the lower sample is in cell
+// c - 1) or on the slots (1, 2) otherwise. The reference's factored form
+// (flip_simulation.cc:977-990) is the same polynomial; the two roundings
+// differ in the last bits of a weight.
 template <int D>
 __device__ inline void p2g_particle(const float pos[3], const int c[3], float vel, float avw[18],
                                     float aw[18], uint32_t &cls) {
 #pragma clang fp contract(off)
-  float lower[3] = {float(c[0]), float(c[1]), float(c[2])};
-  int h[3] = {0, 0, 0};
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    if (a == D) continue;
-    lower[a] = lower[a] + 0.5f;
-    if (pos[a] < lower[a]) lower[a] = lower[a] - 1.0f;
-    else h[a] = 1;
+  constexpr int A = Axes<D>::a, B = Axes<D>::b;
+  const float td = pos[D] - float(c[D]);
+  const float fd[2] = {1.0f - td, td};
+  float fa[3], fb[3];
+  int ha, hb;
+  {
+    const float mid = float(c[A]) + 0.5f;
+    ha = pos[A] < mid ? 0 : 1;
+    const float t = pos[A] - (ha ? mid : mid - 1.0f), u = 1.0f - t;
+    fa[0] = ha ? 0.0f : u;
+    fa[1] = ha ? u : t;
+    fa[2] = ha ? t : 0.0f;
   }
-  const float x = pos[0] - lower[0], y = pos[1] - lower[1], z = pos[2] - lower[2];
-  const float xy = x * y, yz = y * z, xz = z * x, xyz = xy * z;
-  float wt[2][2][2];
-  wt[1][1][1] = xyz;
-  wt[1][1][0] = xy - xyz;
-  wt[1][0][1] = xz - xyz;
-  wt[1][0][0] = (x - xz) - wt[1][1][0];
-  wt[0][1][1] = yz - xyz;
-  wt[0][1][0] = (y - yz) - wt[1][1][0];
-  wt[0][0][1] = (z - yz) - wt[1][0][1];
-  wt[0][0][0] = ((1.0f - x) - y + xy) - wt[0][0][1];
-  const int ha = h[Axes<D>::a], hb = h[Axes<D>::b];
+  {
+    const float mid = float(c[B]) + 0.5f;
+    hb = pos[B] < mid ? 0 : 1;
+    const float t = pos[B] - (hb ? mid : mid - 1.0f), u = 1.0f - t;
+    fb[0] = hb ? 0.0f : u;
+    fb[1] = hb ? u : t;
+    fb[2] = hb ? t : 0.0f;
+  }
   cls |= 1u << (ha * 2 + hb);
+  float fab[3][3];
+#pragma unroll
+  for (int sa = 0; sa < 3; ++sa)
+#pragma unroll
+    for (int sb = 0; sb < 3; ++sb) fab[sa][sb] = fa[sa] * fb[sb];
 #pragma unroll
   for (int sd = 0; sd < 2; ++sd)
 #pragma unroll
     for (int sa = 0; sa < 3; ++sa)
 #pragma unroll
       for (int sb = 0; sb < 3; ++sb) {
-        const float w00 = corner<D>(wt, sd, sa, sb), w01 = corner<D>(wt, sd, sa, sb - 1),
-                    w10 = corner<D>(wt, sd, sa - 1, sb), w11 = corner<D>(wt, sd, sa - 1, sb - 1);
-        const float w = ha ? (hb ? w11 : w10) : (hb ? w01 : w00);
+        const float w = fd[sd] * fab[sa][sb];
         const int s = sd * 9 + sa * 3 + sb;
-        const float m = w * vel;
-        avw[s] = avw[s] + m;
+        avw[s] = fmaf(w, vel, avw[s]);
         aw[s] = aw[s] + w;
       }
 }
