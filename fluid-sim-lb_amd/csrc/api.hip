@@ -159,6 +159,7 @@ int fsl_create(fsl_ctx **out, const int dims[3], const int tiles[3], int device,
   for (int a = 0; a < 9 && !rc; ++a) rc = dev_alloc(ctx, &ctx->Q.a[a], size_t(ctx->cap));
   if (!rc) rc = dev_alloc(ctx, &ctx->key, size_t(ctx->cap));
   if (!rc) rc = dev_alloc(ctx, &ctx->perm, size_t(ctx->cap));
+  if (!rc) rc = dev_alloc(ctx, &ctx->skey, size_t(ctx->cap));
   if (!rc) rc = dev_alloc(ctx, &ctx->vcount, nv);
   if (!rc) rc = dev_alloc(ctx, &ctx->vstart, nv + 1);
   const int64_t ncell = int64_t(dims[0]) * dims[1] * dims[2];
@@ -184,7 +185,7 @@ int fsl_destroy(fsl_ctx *ctx) {
                   ctx->marker, ctx->solid, ctx->lab, ctx->div, ctx->pres, ctx->x, ctx->r, ctx->z,
                   ctx->p[0], ctx->p[1], ctx->Ap, ctx->x64, ctx->partials, ctx->pcg, ctx->lhas,
                   ctx->lact, ctx->lfluid, ctx->lscan, ctx->act_list, ctx->fluid_list, ctx->morton,
-                  ctx->lflag_m, ctx->key,
+                  ctx->lflag_m, ctx->key, ctx->skey,
                   ctx->perm, ctx->vcount, ctx->vstart, ctx->scan_tmp, ctx->flags};
   for (void *b : bufs) if (b) (void)hipFree(b);
   for (int a = 0; a < 9; ++a) {

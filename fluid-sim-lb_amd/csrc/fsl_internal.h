@@ -125,6 +125,7 @@ struct fsl_ctx {
   int64_t cap = 0, np = 0;
   fsl::Particles P{}, Q{};
   uint32_t *key = nullptr, *perm = nullptr;
+  uint32_t *skey = nullptr;  // cell of each sorted slot
   uint32_t *vcount = nullptr, *vstart = nullptr;  // per voxel; vstart has nvox + 1
   uint32_t *scan_tmp = nullptr;
   int64_t scan_tmp_len = 0;

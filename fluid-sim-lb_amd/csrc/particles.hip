@@ -121,29 +121,39 @@ __global__ void k_advect(Dom d, int64_t np, Particles P, float dt, const uint8_t
 }
 
 // Counting-sort scatter. vcount returns to zero, ready for the next step.
-// A run of equal keys takes one atomic; its lanes fill the slots the old
-// per-particle order would have filled (descending from the counter).
+// A run of equal keys takes one atomic and fills consecutive slots in lane
+// order, ascending from the cell's next free slot, so a cell whose particles
+// arrive in one run is already in its previous order. skey[slot] records the
+// slot's cell.
 __global__ void k_scatter(int64_t np, const uint32_t *key, const uint32_t *vstart,
-                          uint32_t *vcount, uint32_t *perm) {
+                          uint32_t *vcount, uint32_t *perm, uint32_t *skey) {
   const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const uint32_t k = q < np ? key[q] : kDead;
   const KeyRun r = key_run(k);
-  uint32_t base = 0;
-  if (r.rank == 0 && k != kDead) base = atomicSub(&vcount[k], uint32_t(r.count));
-  base = __shfl(base, r.head);
+  uint32_t left = 0;  // slots of the cell still free before this run
+  if (r.rank == 0 && k != kDead) left = atomicSub(&vcount[k], uint32_t(r.count));
+  left = __shfl(left, r.head);
   if (k == kDead) return;
-  perm[vstart[k] + base - 1u - uint32_t(r.rank)] = uint32_t(q);
+  const uint32_t b = vstart[k], total = vstart[k + 1] - b;
+  const uint32_t slot = b + (total - left) + uint32_t(r.rank);
+  perm[slot] = uint32_t(q);
+  skey[slot] = k;
 }
 
-// Restore the previous order inside each cell (stable sort): the thread at a
-// cell's first slot insertion-sorts the cell's old indices.
-__global__ void k_cell_sort(uint32_t n, const uint32_t *key, const uint32_t *vstart,
+// Restore the previous order inside each cell (stable sort). The thread at
+// a cell's first slot checks the cell's old indices and insertion-sorts them
+// when runs arrived out of order (a cell filled by several runs).
+__global__ void k_cell_sort(uint32_t n, const uint32_t *skey, const uint32_t *vstart,
                             uint32_t *perm) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
-  const uint32_t k = key[perm[s]];
-  const uint32_t b = vstart[k], e = vstart[k + 1];
-  if (s != b || e - b < 2) return;
+  const uint32_t k = skey[s];
+  if (s > 0 && skey[s - 1] == k) return;
+  const uint32_t b = s, e = vstart[k + 1];
+  if (e - b < 2) return;
+  bool sorted = true;
+  for (uint32_t x = b + 1; x < e && sorted; ++x) sorted = perm[x - 1] < perm[x];
+  if (sorted) return;
   for (uint32_t x = b + 1; x < e; ++x) {
     const uint32_t val = perm[x];
     uint32_t y = x;
@@ -285,13 +295,29 @@ __device__ inline void p2g_component(const Dom &d, const Particles &P, const int
   for (int s = 0; s < 2; ++s)
 #pragma unroll
     for (int q = 0; q < 18; ++q) { avw[s][q] = 0.0f; aw[s][q] = 0.0f; }
+  // Four particles per trip, their loads issued together (indices clamped
+  // to the cell's last particle; the clamped copies are not accumulated), so
+  // a thread has four particles in flight; the sums run in particle order.
 #pragma unroll
   for (int s = 0; s < 2; ++s) {
     if (!has[s]) continue;
     const int c[3] = {rc[s][0], rc[s][1], rc[s][2]};
-    for (uint32_t q = beg[s]; q < end[s]; ++q) {
-      const float pos[3] = {P.a[0][q], P.a[1][q], P.a[2][q]};
-      p2g_particle<D>(pos, c, P.a[3 + D][q], avw[s], aw[s], cls[s]);
+    for (uint32_t q = beg[s]; q < end[s]; q += 4) {
+      float px[4], py[4], pz[4], pv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t qq = q + u < end[s] ? q + u : end[s] - 1;
+        px[u] = P.a[0][qq];
+        py[u] = P.a[1][qq];
+        pz[u] = P.a[2][qq];
+        pv[u] = P.a[3 + D][qq];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (q + u >= end[s]) break;
+        const float pos[3] = {px[u], py[u], pz[u]};
+        p2g_particle<D>(pos, c, pv[u], avw[s], aw[s], cls[s]);
+      }
     }
   }
   // rc holds region coordinates in the ring-extended leaf: own samples are
@@ -552,11 +578,11 @@ hipError_t launch_advect(fsl_ctx *c, float dt, bool drop_outside) {
 // restore in-cell order, gather into the other buffer and swap.
 hipError_t launch_sort_tail(fsl_ctx *c, uint32_t n_alive) {
   if (c->np > 0) {
-    k_scatter<<<blocks_for(c->np, kThreads), kThreads, 0, c->stream>>>(c->np, c->key, c->vstart,
-                                                                        c->vcount, c->perm);
+    k_scatter<<<blocks_for(c->np, kThreads), kThreads, 0, c->stream>>>(
+        c->np, c->key, c->vstart, c->vcount, c->perm, c->skey);
   }
   if (n_alive > 0) {
-    k_cell_sort<<<blocks_for(n_alive, kThreads), kThreads, 0, c->stream>>>(n_alive, c->key,
+    k_cell_sort<<<blocks_for(n_alive, kThreads), kThreads, 0, c->stream>>>(n_alive, c->skey,
                                                                             c->vstart, c->perm);
     k_gather<<<blocks_for(n_alive, kThreads), kThreads, 0, c->stream>>>(n_alive, c->perm, c->P,
                                                                          c->Q);
