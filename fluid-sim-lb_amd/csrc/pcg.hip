@@ -658,13 +658,15 @@ __global__ __launch_bounds__(kT) void k_refresh_r(Dom d, const int32_t *flist, i
 // Math. 25, 1989). A p_k is computed directly (no recurrence for A p), so the
 // residual recurrence is the two-pass one.
 //
-// To apply A to u_{k+1} on a leaf, u_{k+1} is needed on the leaf and its six
-// face layers, hence r_{k+1}, hence A p_k there, hence p_k on the leaf, its
-// face layers at distance 1 and 2, and the twelve edge rows at distance
-// (1,1): 1376 cells, staged in a 12^3 LDS tile. r_k and p_{k-1} are read on
-// those cells and p_k recomputed (the halo cells are recomputed, not
-// exchanged). r and p are double-buffered by iteration parity, as other
-// workgroups read the halo values of the iteration's inputs.
+// delta_{k+1} is taken in edge form: sum over rows c of diag_c u_c^2 minus
+// 2 u_c u_n over the faces (c, n) between rows, each face counted at its
+// lower cell, so a leaf needs u_{k+1} on itself and on its three upper face
+// layers, hence r_{k+1} and A p_k there, hence p_k on the leaf, its six face
+// layers at distance 1, the three upper layers at distance 2 and nine edge
+// rows: 1160 cells, staged in an LDS tile. r_k and p_{k-1} are read on those
+// cells and p_k recomputed (halo cells are recomputed, not exchanged). r and
+// p are double-buffered by iteration parity, as other workgroups read the
+// halo values of the iteration's inputs.
 
 // Tile P over [-2, 10)^3 of the leaf and tile U over [-1, 9)^3, with row
 // strides padded so that the wave access patterns of the stencil and of the
@@ -708,61 +710,66 @@ __device__ inline int64_t rel_voxel(const Dom &d, int64_t leaf, int x, int y, in
   const int64_t nb = leaf + dx * l12 + dy * int64_t(d.L[2]) + dz;
   return (nb << 9) | ((x & 7) << 6) | ((y & 7) << 3) | (z & 7);
 }
-// Halo pairs of the single-reduction pass. The 864 halo cells of a leaf
-// (face layers at distance 1 and 2, edge rows at distance (1, 1)) are loaded
-// as 464 pairs of z-neighbours (two values per lane, one vector access):
-//   [0, 128)   x faces, x = -2, -1, 8, 9;
-//   [128, 256) y faces, y = -2, -1, 8, 9;
-//   [256, 384) z faces, z = (-2, -1) and (8, 9);
-//   [384, 400) xy edge rows (x, y = -1 or 8), z pairs;
-//   [400, 432) xz edge rows (x = -1 or 8, z = -1 or 8), loaded with the cell
-//              beyond the edge in z (the pair (-2, -1) or (8, 9));
-//   [432, 464) yz edge rows, likewise.
-// Pair slot 0 of thread t is pair t; slot 1 is pair 256 + wave * 52 + lane
-// on lanes 0-51 (every wave carries the same work); idle lanes load pair 0
-// and discard it.
+// Halo pairs of the single-reduction pass. With the edge form of u.Au (each
+// face of the fluid rows counted at its lower cell), a leaf needs u_{k+1} on
+// itself and on its three upper face layers (x, y or z = 8; 192 cells), so
+// A p_k there, so p_k on the leaf, its six face layers at distance 1, the
+// upper face layers at distance 2 and nine edge rows: 648 halo cells. They
+// are loaded as 380 pairs of z-neighbours (two values per lane, one vector
+// access):
+//   [0, 64)    x = -1 and x = 8 faces;     [64, 128)  y = -1 and y = 8 faces;
+//   [128, 256) z faces, pairs z = (-2, -1) and (8, 9);
+//   [256, 288) x = 9 face;                 [288, 320) y = 9 face;
+//   [320, 332) xy edge rows (8, -1), (8, 8), (-1, 8), z pairs;
+//   [332, 356) xz edge rows (x, z) = (8, -1), (8, 8), (-1, 8), loaded with
+//              the cell beyond the edge in z (pairs (-2, -1) or (8, 9));
+//   [356, 380) yz edge rows (y, z), likewise.
+// Pair slot 0 of thread t is pair t; slot 1 is pair 256 + wave * 31 + lane
+// on lanes 0-30; idle lanes load pair 0 and discard it.
 struct Run {
   int x, y, z;  // first cell of the pair, relative to the leaf origin
 };
 __device__ inline Run halo_pair(int u) {
+  if (u < 128) {
+    const int i = u & 31, c = (u >> 5) & 1 ? 8 : -1;
+    return u < 64 ? Run{c, i >> 2, (i & 3) * 2} : Run{i >> 2, c, (i & 3) * 2};
+  }
   if (u < 256) {
-    const int side = (u >> 6) & 1, i = u & 63, layer = i >> 5;
-    const int c = side ? 8 + layer : layer - 2;
-    if (u < 128) return Run{c, (i >> 2) & 7, (i & 3) * 2};
-    return Run{i >> 3, (i >> 2) & 1 ? (side ? 9 : -1) : (side ? 8 : -2), (i & 3) * 2};
+    const int v = u - 128, i = v & 63;
+    return Run{i >> 3, i & 7, v >> 6 ? 8 : -2};
   }
-  if (u < 384) {
-    const int side = (u >> 6) & 1, i = u & 63;
-    return Run{i >> 3, i & 7, side ? 8 : -2};
+  if (u < 320) {
+    const int v = u - 256, i = v & 31;
+    return v < 32 ? Run{9, i >> 2, (i & 3) * 2} : Run{i >> 2, 9, (i & 3) * 2};
   }
-  if (u < 400) {
-    const int e = (u - 384) >> 2;
-    return Run{(e & 2) ? 8 : -1, (e & 1) ? 8 : -1, (u & 3) * 2};
+  // The three edge rows of a pair of axes: (8, -1), (8, 8), (-1, 8).
+  if (u < 332) {
+    const int v = u - 320, e = v >> 2;
+    return Run{e < 2 ? 8 : -1, e == 0 ? -1 : 8, (v & 3) * 2};
   }
-  const int e = ((u - 400) >> 3) & 3, f = u & 7, a = (e & 2) ? 8 : -1, b = (e & 1) ? 8 : -2;
-  return u < 432 ? Run{a, f, b} : Run{f, a, b};
+  const int v = u < 356 ? u - 332 : u - 356, e = v >> 3, f = v & 7;
+  const int a = e < 2 ? 8 : -1, b = e == 0 ? -2 : 8;
+  return u < 356 ? Run{a, f, b} : Run{f, a, b};
 }
 __device__ inline int pair_of(int t, int s) {
   const int w = t >> 6, l = t & 63;
-  return s == 0 ? t : (l < 52 ? 256 + w * 52 + l : -1);
+  return s == 0 ? t : (l < 31 ? 256 + w * 31 + l : -1);
 }
-// Distance-1 cells of a pair: bit e of the mask is set when cell e of the
-// pair is distance-1 cell base + e (index f * 64 + a * 8 + b of fhalo_cell).
+// Upper distance-1 cell m in [0, 192): face m / 64 (x = 8, y = 8, z = 8).
+__device__ inline void r1_cell(int m, int &x, int &y, int &z) {
+  const int f = m >> 6, a = (m >> 3) & 7, b = m & 7;
+  if (f == 0) { x = 8; y = a; z = b; }
+  else if (f == 1) { x = a; y = 8; z = b; }
+  else { x = a; y = b; z = 8; }
+}
+// Upper distance-1 cells of a pair: bit e of the mask is set when cell e of
+// the pair is cell base + e of r1_cell.
 __device__ inline void pair_r1(int u, const Run &r, int &mask, int &base) {
   mask = 0;
   base = 0;
-  if (u < 256) {
-    const bool xf = u < 128;
-    const int c = xf ? r.x : r.y;
-    if (c == -1 || c == 8) {
-      mask = 3;
-      base = (xf ? (c < 0 ? 0 : 64) : (c < 0 ? 128 : 192)) + (xf ? r.y : r.x) * 8 + r.z;
-    }
-  } else if (u < 384) {
-    const bool lo = r.z < 0;
-    mask = lo ? 2 : 1;
-    base = (lo ? 256 : 320) + r.x * 8 + r.y - (lo ? 1 : 0);
-  }
+  if (u >= 32 && u < 64) { mask = 3; base = r.y * 8 + r.z; }
+  else if (u >= 96 && u < 128) { mask = 3; base = 64 + r.x * 8 + r.z; }
+  else if (u >= 192 && u < 256) { mask = 1; base = 128 + r.x * 8 + r.y; }
 }
 
 // N consecutive elements from element i (i aligned to N), one vector access.
@@ -836,11 +843,11 @@ __device__ inline void store_pair(T *p, Off i, T a, T b) {
 // are 32-bit and each access is a scalar base plus a vector offset.
 //
 // Per leaf group (256 threads): thread t owns cells 2t and 2t+1 (pairs along
-// z, loaded and stored as vectors) and up to two halo runs. Phase 1 forms
-// p_k on the leaf and its halo in the P tile and hands the distance-1 cells'
-// r and labels to a balanced mapping (thread t: distance-1 cells t and, on
-// lanes 0-31, 256 + wave * 32 + lane) through LDS; phase 2 applies A to p_k
-// there and on the own cells; phase 3 applies A to u_{k+1} on the own cells.
+// z, loaded and stored as vectors) and up to two halo pairs. Phase 1 forms
+// p_k on the leaf and its halo in the P tile and hands the r and labels of
+// the upper distance-1 cells to a balanced mapping (48 per wave) through
+// LDS; phase 2 applies A to p_k there and on the own cells and forms
+// u_{k+1}; phase 3 accumulates gamma, delta (edge form) and r.r.
 template <typename V, typename A, bool ORD, int GR, bool OFF32>
 __global__ __launch_bounds__(kG * GR) void k_cg(Dom d, const int32_t *flist, int64_t nf,
                                                 const uint8_t *lab, const V *rin, V *rout,
@@ -851,8 +858,8 @@ __global__ __launch_bounds__(kG * GR) void k_cg(Dom d, const int32_t *flist, int
   using Off = typename std::conditional<OFF32, uint32_t, int64_t>::type;
   __shared__ V s_p[GR][kPSize];
   __shared__ V s_u[GR][kUSize];
-  __shared__ V s_r1[GR][384];        // r of the distance-1 cells
-  __shared__ uint8_t s_l1[GR][384];  // their labels
+  __shared__ V s_r1[GR][192];        // r of the upper distance-1 cells
+  __shared__ uint8_t s_l1[GR][192];  // their labels
   // Leaf schedule: entry j is the list entry of round j of this group
   // (the first list entry for rounds past the end, which are discarded).
   constexpr int kSched = kG;
@@ -885,15 +892,15 @@ __global__ __launch_bounds__(kG * GR) void k_cg(Dom d, const int32_t *flist, int
     hp0[s] = t12(r.x, r.y, r.z);
     pair_r1(hon[s] ? u : 512, r, rmask[s], rbase[s]);
   }
-  const bool m1ok = l < 32;
-  const int m1[2] = {t, 256 + w * 32 + (l & 31)};
-  int r1p[2], r1u[2];
-#pragma unroll
-  for (int s = 0; s < 2; ++s) {
+  // Upper distance-1 cell of phase 2: 48 per wave.
+  const bool m1ok = l < 48;
+  const int m1 = w * 48 + (m1ok ? l : 0);
+  int r1p, r1u;
+  {
     int hx, hy, hz;
-    fhalo_cell(m1[s], hx, hy, hz);
-    r1p[s] = t12(hx, hy, hz);
-    r1u[s] = u10(hx, hy, hz);
+    r1_cell(m1, hx, hy, hz);
+    r1p = t12(hx, hy, hz);
+    r1u = u10(hx, hy, hz);
   }
   const int op12 = t12_of_local(2 * t), ou10 = u10_of_local(2 * t);  // cell 2t+1 is +1
 
@@ -1012,7 +1019,7 @@ __global__ __launch_bounds__(kG * GR) void k_cg(Dom d, const int32_t *flist, int
   V *tr1 = s_r1[g];
   uint8_t *tl1 = s_l1[g];
   while (q.more()) {
-    // Phase 1: p_k on the 1376 cells into the P tile; distance-1 cells' r
+    // Phase 1: p_k on the 1160 cells into the P tile; upper distance-1 cells' r
     // and labels to LDS.
     const bool valid = cur.valid;
     const Off base = cur.base;
@@ -1056,7 +1063,7 @@ __global__ __launch_bounds__(kG * GR) void k_cg(Dom d, const int32_t *flist, int
       load(q.valid(), s_leaf[g][j % kSched], cur);
     }
     __syncthreads();
-    // Phase 2: A p_k on the own pair and the distance-1 cells; r_{k+1} and
+    // Phase 2: A p_k on the own pair and the upper distance-1 cells; r_{k+1} and
     // u_{k+1} there; x, r and p of the own pair are written.
     V rn[2], un[2];
     if (valid) {
@@ -1105,14 +1112,12 @@ __global__ __launch_bounds__(kG * GR) void k_cg(Dom d, const int32_t *flist, int
       }
       store_pair(rout, v, rn[0], rn[1]);
       store_pair(pout, v, wp[0], wp[1]);
-#pragma unroll
-      for (int s = 0; s < 2; ++s) {
-        if (s == 1 && !m1ok) continue;
-        const uint8_t lb = tl1[m1[s]];
+      if (m1ok) {
+        const uint8_t lb = tl1[m1];
         const V di = s_dinv[lb & 31];
         // A non-row cell has dinv 0: its u is 0 whatever its stencil.
-        const V ap = stencil_s<V, ORD, kPX, kPY>(tp, r1p[s], lb >> 2);
-        const V rh0 = tr1[m1[s]];
+        const V ap = stencil_s<V, ORD, kPX, kPY>(tp, r1p, lb >> 2);
+        const V rh0 = tr1[m1];
         V rh;
         if (ORD) {
           const V m2 = (-alpha) * ap;
@@ -1120,20 +1125,25 @@ __global__ __launch_bounds__(kG * GR) void k_cg(Dom d, const int32_t *flist, int
         } else {
           rh = fmav(-alpha, ap, rh0);
         }
-        tu[r1u[s]] = di == V(0) ? V(0) : rh * di;
+        tu[r1u] = di == V(0) ? V(0) : rh * di;
       }
     }
     __syncthreads();
-    // Phase 3: A u_{k+1} on the own pair; gamma, delta and r.r of k+1.
+    // Phase 3: gamma, delta and r.r of iteration k+1 on the own pair, with
+    // u.Au in edge form: diag u_c^2 - 2 u_c (u_{c+x} + u_{c+y} + u_{c+z})
+    // counts every face between two rows once, at its lower cell (A is
+    // symmetric; u is 0 off the rows). FP64 accumulation, one rounding per
+    // term.
     if (valid) {
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
         if (!is_fluid(wl[i])) continue;
-        const V au = stencil_s<V, ORD, kUX, kUY>(tu, ou10 + i, wl[i] >> 2);
-        // FP64 accumulation with one rounding per term (a float product is
-        // exact in double, so the fma only rounds the sum).
-        acc[0] = fmav(A(rn[i]), A(un[i]), acc[0]);
-        acc[1] = fmav(A(un[i]), A(au), acc[1]);
+        const int c = ou10 + i;
+        const A uc = A(un[i]);
+        const A sn = (A(tu[c + kUX]) + A(tu[c + kUY])) + A(tu[c + 1]);
+        const A tt = fmav(A(wl[i] >> 2), uc, A(-2) * sn);
+        acc[0] = fmav(A(rn[i]), uc, acc[0]);
+        acc[1] = fmav(uc, tt, acc[1]);
         acc[2] = fmav(A(rn[i]), A(rn[i]), acc[2]);
       }
     }
@@ -1142,10 +1152,10 @@ __global__ __launch_bounds__(kG * GR) void k_cg(Dom d, const int32_t *flist, int
 }
 
 // MIXED residual refresh (single-reduction path): r = b - A x64 in FP64 on
-// the leaf and its distance-1 cells from x64 staged on the two-cell halo
-// (the layout of k_cg: own z-pairs, 464 halo pairs, distance-1 cells through
-// LDS), stored as float on the leaf; then u = M^-1 r, A u on the leaf, and
-// the partials (gamma, delta, r.r) of the refreshed residual. x has been
+// the leaf and its upper distance-1 cells from x64 staged on the halo of
+// k_cg (own z-pairs, 380 halo pairs, upper distance-1 cells through LDS),
+// stored as float on the leaf; then u = M^-1 r and the partials (gamma,
+// delta in edge form, r.r) of the refreshed residual. x has been
 // folded into x64 before (by the k_cg launch before the refresh, or by
 // k_refresh_x).
 template <bool OFF32>
@@ -1158,8 +1168,8 @@ __global__ __launch_bounds__(kG) void k_refresh_cg(Dom d, const int32_t *flist, 
   using Off = typename std::conditional<OFF32, uint32_t, int64_t>::type;
   __shared__ double s_p[kPSize];
   __shared__ float s_u[kUSize];
-  __shared__ float s_b1[384];
-  __shared__ uint8_t s_l1[384];
+  __shared__ float s_b1[192];
+  __shared__ uint8_t s_l1[192];
   __shared__ float s_dinv[32];
   if ((!force && st->done) || st->status) return;
   const int t = threadIdx.x;
@@ -1181,15 +1191,15 @@ __global__ __launch_bounds__(kG) void k_refresh_cg(Dom d, const int32_t *flist, 
     hp0[s] = t12(r.x, r.y, r.z);
     pair_r1(hon[s] ? u : 512, r, rmask[s], rbase[s]);
   }
-  const bool m1ok = l < 32;
-  const int m1[2] = {t, 256 + w * 32 + (l & 31)};
-  int r1p[2], r1u[2];
-#pragma unroll
-  for (int s = 0; s < 2; ++s) {
+  // Upper distance-1 cell of phase 2: 48 per wave.
+  const bool m1ok = l < 48;
+  const int m1 = w * 48 + (m1ok ? l : 0);
+  int r1p, r1u;
+  {
     int hx, hy, hz;
-    fhalo_cell(m1[s], hx, hy, hz);
-    r1p[s] = t12(hx, hy, hz);
-    r1u[s] = u10(hx, hy, hz);
+    r1_cell(m1, hx, hy, hz);
+    r1p = t12(hx, hy, hz);
+    r1u = u10(hx, hy, hz);
   }
   const int op12 = t12_of_local(2 * t), ou10 = u10_of_local(2 * t);
   double acc[3] = {0.0, 0.0, 0.0};
@@ -1212,7 +1222,7 @@ __global__ __launch_bounds__(kG) void k_refresh_cg(Dom d, const int32_t *flist, 
       load_run<2>(x64, hv, hx[s]);
       load_run<2>(div, hv, hb[s]);
     }
-    // Phase 1: x64 on the 1376 cells (0 off the rows) into the tile.
+    // Phase 1: x64 on the 1160 cells (0 off the rows) into the tile.
     uint8_t wl[2];
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
@@ -1233,7 +1243,7 @@ __global__ __launch_bounds__(kG) void k_refresh_cg(Dom d, const int32_t *flist, 
       }
     }
     __syncthreads();
-    // Phase 2: r = b - A x64 (FP64) on the own pair and the distance-1 cells.
+    // Phase 2: r = b - A x64 (FP64) on the own pair and the upper distance-1 cells.
     float rn[2], un[2];
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
@@ -1244,24 +1254,25 @@ __global__ __launch_bounds__(kG) void k_refresh_cg(Dom d, const int32_t *flist, 
       s_u[ou10 + i] = un[i];
     }
     store_pair(rout, v, rn[0], rn[1]);
-#pragma unroll
-    for (int s = 0; s < 2; ++s) {
-      if (s == 1 && !m1ok) continue;
-      const uint8_t lb = s_l1[m1[s]];
+    if (m1ok) {
+      const uint8_t lb = s_l1[m1];
       const float di = s_dinv[lb & 31];
       float uh = 0.0f;
       if (di != 0.0f)
-        uh = float(double(s_b1[m1[s]]) - stencil_s<double, false, kPX, kPY>(s_p, r1p[s], lb >> 2)) * di;
-      s_u[r1u[s]] = uh;
+        uh = float(double(s_b1[m1]) - stencil_s<double, false, kPX, kPY>(s_p, r1p, lb >> 2)) * di;
+      s_u[r1u] = uh;
     }
     __syncthreads();
-    // Phase 3: A u on the own pair; gamma, delta, r.r in FP64.
+    // Phase 3: gamma, delta (edge form, as in k_cg), r.r in FP64.
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
       if (!is_fluid(wl[i])) continue;
-      const float au = stencil_s<float, false, kUX, kUY>(s_u, ou10 + i, wl[i] >> 2);
-      acc[0] = fmav(double(rn[i]), double(un[i]), acc[0]);
-      acc[1] = fmav(double(un[i]), double(au), acc[1]);
+      const int c = ou10 + i;
+      const double uc = double(un[i]);
+      const double sn = (double(s_u[c + kUX]) + double(s_u[c + kUY])) + double(s_u[c + 1]);
+      const double tt = fmav(double(wl[i] >> 2), uc, -2.0 * sn);
+      acc[0] = fmav(double(rn[i]), uc, acc[0]);
+      acc[1] = fmav(uc, tt, acc[1]);
       acc[2] = fmav(double(rn[i]), double(rn[i]), acc[2]);
     }
     __syncthreads();
