@@ -1755,15 +1755,19 @@ static int pcg_run_sr(fsl_ctx *ctx, const fsl_solve_params *prm, fsl_solve_stats
     }
     if (h->status) break;
     if (!h->done) {
-      if (adaptive && h->it > it_prev && h->rr > 0.0 && rr_prev > 0.0) {
-        const double rate = std::log(h->rr / rr_prev) / double(h->it - it_prev);
+      // r.r of CG is not monotone: the decay rate is taken from the running
+      // minimum, and a batch covers at most 60% of the predicted remainder
+      // (launches past convergence cost about 2 us each).
+      const double rr_min = std::min(h->rr, rr_prev);
+      if (adaptive && h->it > it_prev && rr_min > 0.0 && rr_prev > 0.0) {
+        const double rate = std::log(rr_min / rr_prev) / double(h->it - it_prev);
         const double target = h->stop_mode == FSL_STOP_REL ? h->thr * h->thr : h->thr;
         if (rate < 0.0 && target > 0.0) {
-          const double need = std::log(target / h->rr) / rate;
-          nb = int(std::min(double(nb_cap), std::max(16.0, 0.8 * need)));
+          const double need = std::log(target / rr_min) / rate;
+          nb = int(std::min(double(std::min(nb_cap, 256)), std::max(16.0, 0.6 * need)));
         }
       }
-      rr_prev = h->rr;
+      rr_prev = rr_min;
       it_prev = h->it;
       continue;
     }
