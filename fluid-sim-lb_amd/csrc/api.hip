@@ -167,7 +167,7 @@ int fsl_create(fsl_ctx **out, const int dims[3], const int tiles[3], int device,
   if (!rc) rc = dev_alloc(ctx, &ctx->scan_tmp, size_t(ctx->scan_tmp_len));
   if (!rc) rc = dev_alloc(ctx, &ctx->flags, 1);
   if (!rc && hipHostMalloc(reinterpret_cast<void **>(&ctx->flags_host), sizeof(fsl::DevFlags)) != hipSuccess) rc = FSL_ERR_HIP;
-  if (!rc && hipHostMalloc(reinterpret_cast<void **>(&ctx->pcg_host), sizeof(fsl::PcgState)) != hipSuccess) rc = FSL_ERR_HIP;
+  if (!rc && hipHostMalloc(reinterpret_cast<void **>(&ctx->pcg_host), 3 * sizeof(fsl::PcgState)) != hipSuccess) rc = FSL_ERR_HIP;
   if (!rc && hipHostMalloc(reinterpret_cast<void **>(&ctx->host_u32), 64 * sizeof(uint32_t)) != hipSuccess) rc = FSL_ERR_HIP;
   if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = FSL_ERR_HIP;
   if (rc) return fail(rc);

@@ -107,7 +107,7 @@ struct fsl_ctx {
   double *x64 = nullptr;
   void *partials = nullptr;
   fsl::PcgState *pcg = nullptr;
-  fsl::PcgState *pcg_host = nullptr;  // pinned mirror
+  fsl::PcgState *pcg_host = nullptr;  // pinned mirror; entries 1 and 2: state after pipelined batches
   int pcg_precision = FSL_PCG_FP32, pcg_stop = FSL_STOP_ABS_RR, pcg_refresh = 32;
   int pcg_pbuf = 0;            // which p buffer holds the current direction
   bool pcg_ready = false;

@@ -1698,6 +1698,64 @@ int pcg_iterate_once(fsl_ctx *ctx) {
 // The single-reduction solve: one k_cg launch per iteration. Batches of
 // iterations run between host checks of the device state; launches past
 // convergence exit at once (the done flag).
+// Batches of iterations with one batch always queued behind the one whose
+// state the host reads: after enqueueing batch c the host waits for the
+// state after batch c - 1 (copied to a pinned slot), so the GPU never idles
+// on the host's decision. Launches queued past convergence exit on the done
+// flag (about 2 us each); adaptive batches cover at most 60% of the
+// iterations the residual decay still predicts beyond those in flight.
+// issue(k) enqueues iteration k; k ends past the last enqueued iteration.
+// Returns once the state reports done or an error, with the final state in
+// ctx->pcg_host.
+template <typename Issue>
+static int run_pipelined(fsl_ctx *ctx, int &k, int batch, bool adaptive, Issue &&issue) {
+  struct Events {
+    hipEvent_t e[2] = {nullptr, nullptr};
+    ~Events() {
+      for (hipEvent_t x : e)
+        if (x) (void)hipEventDestroy(x);
+    }
+  } ev;
+  for (hipEvent_t &e : ev.e) FSL_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  PcgState *slot = ctx->pcg_host + 1;
+  auto post = [&](int c, int n) -> int {
+    for (int b = 0; b < n; ++b, ++k) issue(k);
+    FSL_HIP(hipGetLastError());
+    FSL_HIP(hipMemcpyAsync(&slot[c & 1], ctx->pcg, sizeof(PcgState), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    FSL_HIP(hipEventRecord(ev.e[c & 1], ctx->stream));
+    return FSL_OK;
+  };
+  double rr_prev = ctx->pcg_host->rr;
+  int it_prev = ctx->pcg_host->it;
+  int nb = batch;
+  int rc = post(0, nb);
+  if (rc) return rc;
+  for (int c = 1;; ++c) {
+    rc = post(c, nb);
+    if (rc) return rc;
+    FSL_HIP(hipEventSynchronize(ev.e[(c - 1) & 1]));
+    const PcgState &h = slot[(c - 1) & 1];
+    if (h.done || h.status) break;
+    if (!adaptive) continue;
+    // r.r of CG is not monotone: the decay rate comes from the running
+    // minimum.
+    const double rr_min = std::min(h.rr, rr_prev);
+    if (h.it > it_prev && rr_min > 0.0 && rr_prev > 0.0) {
+      const double rate = std::log(rr_min / rr_prev) / double(h.it - it_prev);
+      const double target = h.stop_mode == FSL_STOP_REL ? h.thr * h.thr : h.thr;
+      if (rate < 0.0 && target > 0.0) {
+        const double need = std::log(target / rr_min) / rate - double(k - 1 - h.it);
+        nb = int(std::min(256.0, std::max(8.0, 0.6 * need)));
+      }
+    }
+    rr_prev = rr_min;
+    it_prev = h.it;
+  }
+  // The queued batch exits on the done flag; the mirror gets the final state.
+  return sync_state(ctx);
+}
+
 static int pcg_run_sr(fsl_ctx *ctx, const fsl_solve_params *prm, fsl_solve_stats *st) {
   int rc = pcg_setup(ctx, prm);
   if (rc) return rc;
@@ -1727,23 +1785,45 @@ static int pcg_run_sr(fsl_ctx *ctx, const fsl_solve_params *prm, fsl_solve_stats
   // alpha_{k-1} p_{k-1} + alpha_k p_k; a kernel that stops at an odd k applies
   // the pending term. After a forced refresh x is complete (resume_k).
   int resume_k = -1;
-  while (!ctx->pcg_host->done) {
+  auto mode_of = [&](int kk) {
+    return ((kk & 1) ? (kk == resume_k ? 1 : 3) : 0) |
+           (mixed && (kk + 1) % ctx->pcg_refresh == 0 ? 4 : 0);
+  };
+  while (!ctx->timing && !ctx->pcg_host->done) {
+    rc = run_pipelined(ctx, k, nb, adaptive, [&](int kk) {
+      dispatch(ctx->pcg_precision, [&](auto K) {
+        decltype(K)::sr_step(ctx, kk, mode_of(kk), nullptr, nullptr);
+      });
+      if (mixed && (kk + 1) % ctx->pcg_refresh == 0) launch_refresh_sr(ctx, kk + 1, 0);
+    });
+    if (rc) return rc;
+    const PcgState *h = ctx->pcg_host;
+    if (h->status || !(mixed && h->it > 0 && h->it < h->max_iter)) break;
+    // Converged on the recursive residual: confirm on the true residual of
+    // x_it, and continue from iteration it if it does not hold.
+    const int last = h->it;
+    launch_refresh_sr(ctx, last, 1);
+    FSL_HIP(hipGetLastError());
+    rc = sync_state(ctx);
+    if (rc) return rc;
+    if (ctx->pcg_host->done) break;
+    k = last;
+    resume_k = last;
+  }
+  // With the timing hook, batches run one at a time: each launch records
+  // its own events, which are read back after the batch.
+  while (ctx->timing && !ctx->pcg_host->done) {
     const int k0 = k;
     for (int b = 0; b < nb; ++b, ++k) {
       hipEvent_t *e = pev + 2 * b;
-      const bool t = ctx->timing;
-      const int mode = ((k & 1) ? (k == resume_k ? 1 : 3) : 0) |
-                       (mixed && (k + 1) % ctx->pcg_refresh == 0 ? 4 : 0);
-      dispatch(ctx->pcg_precision, [&](auto K) {
-        decltype(K)::sr_step(ctx, k, mode, t ? e[0] : nullptr, t ? e[1] : nullptr);
-      });
+      dispatch(ctx->pcg_precision, [&](auto K) { decltype(K)::sr_step(ctx, k, mode_of(k), e[0], e[1]); });
       if (mixed && (k + 1) % ctx->pcg_refresh == 0) launch_refresh_sr(ctx, k + 1, 0);
     }
     FSL_HIP(hipGetLastError());
     rc = sync_state(ctx);
     if (rc) return rc;
     const PcgState *h = ctx->pcg_host;
-    if (ctx->timing) {
+    {
       // Launches k0..k-1 ran; those before the stopping one did the work.
       const int worked = std::min(h->done ? h->it - k0 : nb, nb);
       for (int b = 0; b < worked; ++b) {
@@ -1819,17 +1899,37 @@ int pcg_run(fsl_ctx *ctx, const fsl_solve_params *prm, fsl_solve_stats *st) {
   int nb = std::min(batch, nb_cap);
   hipEvent_t *pev = ctx->tev + 32;
   int k = 0;  // next iteration to launch
-  while (!ctx->pcg_host->done) {
+  // Without the timing hook: pipelined batches of A(k), B(k) pairs. A(k)
+  // holds the stopping test of iteration k, and every pass after a stopping
+  // A(k) exits on the done flag.
+  while (!ctx->timing && !ctx->pcg_host->done) {
+    rc = run_pipelined(ctx, k, nb, adaptive, [&](int kk) {
+      dispatch(ctx->pcg_precision, [&](auto K) { decltype(K)::pass_a(ctx, kk); });
+      dispatch(ctx->pcg_precision, [&](auto K) { decltype(K)::pass_b(ctx, kk); });
+      if (mixed && (kk + 1) % ctx->pcg_refresh == 0) launch_refresh(ctx, kk, 0);
+    });
+    if (rc) return rc;
+    const PcgState *h = ctx->pcg_host;
+    if (h->status || !(mixed && h->it > 0 && h->it < h->max_iter)) break;
+    // Converged on the recursive residual: confirm on the true residual
+    // (refresh after B(it-1)), and continue from A(it) if it does not hold.
+    const int last = h->it;
+    launch_refresh(ctx, last - 1, 1);
+    FSL_HIP(hipGetLastError());
+    rc = sync_state(ctx);
+    if (rc) return rc;
+    if (ctx->pcg_host->done) break;
+    ctx->pcg_host->it = last;
+    k = last;
+  }
+  // With the timing hook, batches run one at a time, each pass recording its
+  // own events; a batch ends with the stopping test A(k).
+  while (ctx->timing && !ctx->pcg_host->done) {
     const int k0 = k;
     for (int b = 0; b < nb; ++b, ++k) {
       hipEvent_t *e = pev + 4 * b;
-      const bool t = ctx->timing;
-      dispatch(ctx->pcg_precision, [&](auto K) {
-        decltype(K)::pass_a(ctx, k, t ? e[0] : nullptr, t ? e[1] : nullptr);
-      });
-      dispatch(ctx->pcg_precision, [&](auto K) {
-        decltype(K)::pass_b(ctx, k, t ? e[2] : nullptr, t ? e[3] : nullptr);
-      });
+      dispatch(ctx->pcg_precision, [&](auto K) { decltype(K)::pass_a(ctx, k, e[0], e[1]); });
+      dispatch(ctx->pcg_precision, [&](auto K) { decltype(K)::pass_b(ctx, k, e[2], e[3]); });
       if (mixed && (k + 1) % ctx->pcg_refresh == 0) launch_refresh(ctx, k, 0);
     }
     // The stopping test of iteration k runs at the start of A(k).
@@ -1838,7 +1938,7 @@ int pcg_run(fsl_ctx *ctx, const fsl_solve_params *prm, fsl_solve_stats *st) {
     rc = sync_state(ctx);
     if (rc) return rc;
     const PcgState *h = ctx->pcg_host;
-    if (ctx->timing) {
+    {
       // A(j) and B(j) did work for j < it (done) or j < k (running).
       const int worked = std::min((h->done ? h->it : k) - k0, nb);
       for (int b = 0; b < worked; ++b) {
