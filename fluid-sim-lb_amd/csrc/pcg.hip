@@ -1453,7 +1453,7 @@ inline int grid_for(int64_t n, int cap = kBlocksPerXcd) {
 // runtime's occupancy figure), all resident at once.
 constexpr int kCgGroups = 1;
 template <typename K>
-int resident_per_xcd(K kernel, int threads) {
+int resident_per_xcd(K kernel, int threads, int max_per_cu) {
   static int cached = 0;  // one per kernel instantiation
   if (!cached) {
     int per_cu = 0, dev = 0, ncu = 0;
@@ -1462,13 +1462,18 @@ int resident_per_xcd(K kernel, int threads) {
         hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess ||
         per_cu <= 0)
       return 32;
-    cached = std::max(1, per_cu * std::max(1, ncu / kXcd));
+    cached = std::max(1, std::min(per_cu, max_per_cu) * std::max(1, ncu / kXcd));
   }
   return cached;
 }
+// Workgroups of k_cg per CU. Each launch begins with every workgroup reading
+// all partials of the previous launch and ends with a workgroup reduction
+// and a store, so these per-launch costs grow with the workgroup count;
+// four per CU still keep the leaf rounds fed.
+constexpr int kCgMaxPerCu = 4;
 template <typename K>
 inline int grid_cg(int64_t n, K kernel) {
-  const int64_t cap = std::min(resident_per_xcd(kernel, kG * kCgGroups), 256);
+  const int64_t cap = std::min(resident_per_xcd(kernel, kG * kCgGroups, kCgMaxPerCu), 256);
   const int64_t chunk = (n + kXcd - 1) / kXcd;
   return int(kXcd * std::max<int64_t>(1, std::min<int64_t>(chunk, cap)));
 }
