@@ -160,6 +160,12 @@ int fsl_create(fsl_ctx **out, const int dims[3], const int tiles[3], int device,
   if (!rc) rc = dev_alloc(ctx, &ctx->key, size_t(ctx->cap));
   if (!rc) rc = dev_alloc(ctx, &ctx->perm, size_t(ctx->cap));
   if (!rc) rc = dev_alloc(ctx, &ctx->skey, size_t(ctx->cap));
+  if (!rc) rc = dev_alloc(ctx, &ctx->p2g_w, 3 * nv);
+  if (!rc) {
+    uint8_t *ring = nullptr;
+    rc = dev_alloc(ctx, &ring, fsl::p2g_ring_bytes(ctx->nleaves));
+    ctx->p2g_ring = ring;
+  }
   if (!rc) rc = dev_alloc(ctx, &ctx->vcount, nv);
   if (!rc) rc = dev_alloc(ctx, &ctx->vstart, nv + 1);
   const int64_t ncell = int64_t(dims[0]) * dims[1] * dims[2];
@@ -185,7 +191,7 @@ int fsl_destroy(fsl_ctx *ctx) {
                   ctx->marker, ctx->solid, ctx->lab, ctx->div, ctx->pres, ctx->x, ctx->r, ctx->z,
                   ctx->p[0], ctx->p[1], ctx->Ap, ctx->x64, ctx->partials, ctx->pcg, ctx->lhas,
                   ctx->lact, ctx->lfluid, ctx->lscan, ctx->act_list, ctx->fluid_list, ctx->morton,
-                  ctx->lflag_m, ctx->key, ctx->skey,
+                  ctx->lflag_m, ctx->key, ctx->skey, ctx->p2g_w, ctx->p2g_ring,
                   ctx->perm, ctx->vcount, ctx->vstart, ctx->scan_tmp, ctx->flags};
   for (void *b : bufs) if (b) (void)hipFree(b);
   for (int a = 0; a < 9; ++a) {

@@ -126,6 +126,10 @@ struct fsl_ctx {
   fsl::Particles P{}, Q{};
   uint32_t *key = nullptr, *perm = nullptr;
   uint32_t *skey = nullptr;  // cell of each sorted slot
+  // P2G scratch: weight sums of the own cells (3 * nvox) and the ring blocks
+  // that carry contributions to neighbouring leaves.
+  float *p2g_w = nullptr;
+  void *p2g_ring = nullptr;
   uint32_t *vcount = nullptr, *vstart = nullptr;  // per voxel; vstart has nvox + 1
   uint32_t *scan_tmp = nullptr;
   int64_t scan_tmp_len = 0;
@@ -155,6 +159,7 @@ hipError_t launch_sort_tail(fsl_ctx *c, uint32_t n_alive);
 hipError_t launch_mask_count(fsl_ctx *c, const uint8_t *mask_dev, int64_t n, uint32_t *cnt);
 hipError_t launch_leaf_flags(fsl_ctx *c);
 hipError_t launch_compact(fsl_ctx *c, const uint32_t *flag, int32_t *list);
+size_t p2g_ring_bytes(int64_t nleaves);
 hipError_t launch_p2g(fsl_ctx *c, float *const wsum[3] = nullptr);
 hipError_t launch_g2p_transpose(fsl_ctx *c, float *const pval[3], float *const out[3]);
 hipError_t launch_p2g_transpose(fsl_ctx *c, float *const g[3], float *const wsum[3], float keep,

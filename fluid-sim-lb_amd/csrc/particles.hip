@@ -209,14 +209,11 @@ __global__ void k_compact(int64_t n, const int32_t *order, const uint32_t *fm,
 }
 
 // ------------------------------------------------------------------ P2G
-// TransferParticlesToGrid (flip_simulation.cc:808-923) restated as a gather:
-// one workgroup per active leaf owns the leaf's 512 face samples of each
-// component. It visits the 10^3 cells of the leaf and its one-cell ring (the
-// only cells whose particles reach the leaf's samples, flip_simulation.cc:
-// 936-1009); each thread sums its cell's particles in registers over the 18
-// samples a cell can reach per component, then the 18 offsets are committed
-// to LDS one phase at a time. Within a phase every cell targets a different
-// sample, so no atomics are needed and the order is fixed.
+// TransferParticlesToGrid (flip_simulation.cc:808-923): a particle in cell c
+// reaches, per component, the 18 samples c + (sd, sa - 1, sb - 1) along (D,
+// a, b) with sd in {0, 1} and sa, sb in {0, 1, 2} (flip_simulation.cc:
+// 936-1009). Each thread sums its cell's particles over these 18 slots in
+// registers before anything is written.
 template <int D>
 struct Axes;  // the two axes other than D
 template <> struct Axes<0> { static constexpr int a = 1, b = 2; };
@@ -285,49 +282,67 @@ __host__ __device__ constexpr uint32_t slot_classes(int sa, int sb) {
   return m;
 }
 
+// Ring of a leaf's 10^3 region (region coordinate r <-> local voxel r - 1):
+// the 488 cells with a coordinate 0 or 9, i.e. the samples of neighbouring
+// leaves that the leaf's own cells reach. Ring entry i: [0, 200) x = 0 or 9
+// (y, z in 0..9); [200, 360) y = 0 or 9 (x in 1..8, z in 0..9); [360, 488)
+// z = 0 or 9 (x, y in 1..8).
+constexpr int kRing = 488;
+__device__ inline void ring_cell(int i, int &x, int &y, int &z) {
+  if (i < 200) {
+    x = i < 100 ? 0 : 9;
+    const int j = i % 100;
+    y = j / 10; z = j % 10;
+  } else if (i < 360) {
+    const int j = i - 200, m = j % 80;
+    y = j < 80 ? 0 : 9;
+    x = 1 + m / 10; z = m % 10;
+  } else {
+    const int j = i - 360, m = j % 64;
+    z = j < 64 ? 0 : 9;
+    x = 1 + m / 8; y = 1 + m % 8;
+  }
+}
+__device__ inline int ring_index(int x, int y, int z) {
+  if (x == 0 || x == 9) return (x ? 100 : 0) + y * 10 + z;
+  if (y == 0 || y == 9) return 200 + (y ? 80 : 0) + (x - 1) * 10 + z;
+  return 360 + (z ? 64 : 0) + (x - 1) * 8 + (y - 1);
+}
+
+// One component of a leaf's own cells: thread t sums its cell's particles
+// into the 18 slots in registers, then the slots are committed to the
+// region's LDS samples one phase at a time (within a phase every cell
+// targets a different sample, so the order is fixed and no atomics are
+// needed). r is the cell's region index (x * 100 + y * 10 + z).
 template <int D>
-__device__ inline void p2g_component(const Dom &d, const Particles &P, const int rc[2][3],
-                                     const uint32_t beg[2], const uint32_t end[2], bool has[2],
-                                     float *s_vw, float *s_w, uint8_t *s_touch) {
-  float avw[2][18], aw[2][18];
-  uint32_t cls[2] = {0u, 0u};
+__device__ inline void p2g_own(const Particles &P, const int c[3], int r, uint32_t beg,
+                               uint32_t end, float *s_vw, float *s_w, uint8_t *s_touch) {
+  float avw[18], aw[18];
+  uint32_t cls = 0u;
 #pragma unroll
-  for (int s = 0; s < 2; ++s)
-#pragma unroll
-    for (int q = 0; q < 18; ++q) { avw[s][q] = 0.0f; aw[s][q] = 0.0f; }
+  for (int q = 0; q < 18; ++q) { avw[q] = 0.0f; aw[q] = 0.0f; }
   // Four particles per trip, their loads issued together (indices clamped
-  // to the cell's last particle; the clamped copies are not accumulated), so
-  // a thread has four particles in flight; the sums run in particle order.
+  // to the cell's last particle; the clamped copies are not accumulated);
+  // the sums run in particle order.
+  for (uint32_t q = beg; q < end; q += 4) {
+    float px[4], py[4], pz[4], pv[4];
 #pragma unroll
-  for (int s = 0; s < 2; ++s) {
-    if (!has[s]) continue;
-    const int c[3] = {rc[s][0], rc[s][1], rc[s][2]};
-    for (uint32_t q = beg[s]; q < end[s]; q += 4) {
-      float px[4], py[4], pz[4], pv[4];
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t qq = q + u < end ? q + u : end - 1;
+      px[u] = P.a[0][qq];
+      py[u] = P.a[1][qq];
+      pz[u] = P.a[2][qq];
+      pv[u] = P.a[3 + D][qq];
+    }
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const uint32_t qq = q + u < end[s] ? q + u : end[s] - 1;
-        px[u] = P.a[0][qq];
-        py[u] = P.a[1][qq];
-        pz[u] = P.a[2][qq];
-        pv[u] = P.a[3 + D][qq];
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        if (q + u >= end[s]) break;
-        const float pos[3] = {px[u], py[u], pz[u]};
-        p2g_particle<D>(pos, c, pv[u], avw[s], aw[s], cls[s]);
-      }
+    for (int u = 0; u < 4; ++u) {
+      if (q + u >= end) break;
+      const float pos[3] = {px[u], py[u], pz[u]};
+      p2g_particle<D>(pos, c, pv[u], avw, aw, cls);
     }
   }
-  // rc holds region coordinates in the ring-extended leaf: own samples are
-  // region coordinates 1..8; region coordinate r <-> local voxel r - 1.
-  int rr[2][3];
-#pragma unroll
-  for (int s = 0; s < 2; ++s) {
-    const int idx = int(threadIdx.x) + s * 512;
-    rr[s][0] = idx / 100; rr[s][1] = (idx / 10) % 10; rr[s][2] = idx % 10;
-  }
+  constexpr int A = Axes<D>::a, B = Axes<D>::b;
+  constexpr int st[3] = {100, 10, 1};
 #pragma unroll
   for (int sd = 0; sd < 2; ++sd)
 #pragma unroll
@@ -335,85 +350,144 @@ __device__ inline void p2g_component(const Dom &d, const Particles &P, const int
 #pragma unroll
       for (int sb = 0; sb < 3; ++sb) {
         const int slot = sd * 9 + sa * 3 + sb;
-        constexpr int A = Axes<D>::a, B = Axes<D>::b;
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-          if (!has[s]) continue;
-          int t[3] = {rr[s][0], rr[s][1], rr[s][2]};
-          t[D] += sd; t[A] += sa - 1; t[B] += sb - 1;
-          if (t[0] < 1 || t[1] < 1 || t[2] < 1 || t[0] > 8 || t[1] > 8 || t[2] > 8) continue;
-          const int own = ((t[0] - 1) << 6) | ((t[1] - 1) << 3) | (t[2] - 1);
-          s_vw[own] += avw[s][slot];
-          s_w[own] += aw[s][slot];
-          if (cls[s] & slot_classes(sa, sb)) s_touch[own] = 1;
+        if (end > beg) {
+          const int g = r + sd * st[D] + (sa - 1) * st[A] + (sb - 1) * st[B];
+          s_vw[g] += avw[slot];
+          s_w[g] += aw[slot];
+          if (cls & slot_classes(sa, sb)) s_touch[g] = 1;
         }
         __syncthreads();
       }
 }
 
-// ADJ = false: TransferParticlesToGrid; the weight sums are also stored in
-// wsum when it is given (the adjoint's P2G transpose divides by them).
+// TransferParticlesToGrid (flip_simulation.cc:808-923) restated as a gather
+// in two kernels. k_p2g_own: one workgroup per leaf with particles sums its
+// own 512 cells' particles into the samples of its 10^3 region (each
+// particle is visited once per component); the own samples' partial sums go
+// to vx/vy/vz, ws* and vflag, the ring samples (those of neighbouring
+// leaves) to the leaf's ring block. k_p2g_combine then completes every
+// active leaf's samples from the ring blocks of its neighbours, in a fixed
+// order, and normalises.
 // ADJ = true: the transpose of G2P's interpolation, i.e. the un-normalised
 // sums sum_p w * value_p with the particle values taken from P.a[3..5]; no
-// labels or activity are written.
+// weights, labels or activity are written.
 template <bool ADJ>
-__global__ __launch_bounds__(512) void k_p2g(Dom d, const int32_t *list, int64_t nlist,
-                                             const uint32_t *vstart, Particles P, float *vx,
-                                             float *vy, float *vz, uint8_t *vflag,
-                                             uint8_t *marker, float *wsx, float *wsy,
-                                             float *wsz) {
-  __shared__ float s_vw[3][512];
-  __shared__ float s_w[3][512];
-  __shared__ uint8_t s_touch[512];
-  const int64_t w = xcd_work(nlist);
-  if (w >= nlist) return;
-  const int64_t leaf = list[w];
+__global__ __launch_bounds__(512) void k_p2g_own(Dom d, const int32_t *list, int64_t nlist,
+                                                 const uint32_t *lhas, const uint32_t *vstart,
+                                                 Particles P, float *vx, float *vy, float *vz,
+                                                 float *wx, float *wy, float *wz, uint8_t *vflag,
+                                                 float *ring, uint8_t *ring_t, int64_t rstride) {
+  __shared__ float s_vw[3][1000];
+  __shared__ float s_w[3][1000];
+  __shared__ uint8_t s_touch[1000];
+  const int64_t wi = xcd_work(nlist);
+  if (wi >= nlist) return;
+  const int64_t leaf = list[wi];
+  if (!lhas[leaf]) return;
   int o[3];
   leaf_origin(d, leaf, o);
   const int t = threadIdx.x;
-  for (int a = 0; a < 3; ++a) { s_vw[a][t] = 0.0f; s_w[a][t] = 0.0f; }
-  s_touch[t] = 0;
-  int rc[2][3];
-  uint32_t beg[2] = {0u, 0u}, end[2] = {0u, 0u};
-  bool has[2] = {false, false};
+  for (int i = t; i < 1000; i += 512) {
 #pragma unroll
-  for (int s = 0; s < 2; ++s) {
-    const int idx = t + s * 512;
-    if (idx >= 1000) continue;
-    rc[s][0] = o[0] - 1 + idx / 100;
-    rc[s][1] = o[1] - 1 + (idx / 10) % 10;
-    rc[s][2] = o[2] - 1 + idx % 10;
-    if (rc[s][0] < 0 || rc[s][1] < 0 || rc[s][2] < 0 || rc[s][0] > d.n[0] ||
-        rc[s][1] > d.n[1] || rc[s][2] > d.n[2])
-      continue;
-    const int64_t v = vox_of(d, rc[s][0], rc[s][1], rc[s][2]);
-    beg[s] = vstart[v];
-    end[s] = vstart[v + 1];
-    has[s] = end[s] > beg[s];
+    for (int a = 0; a < 3; ++a) { s_vw[a][i] = 0.0f; s_w[a][i] = 0.0f; }
+    s_touch[i] = 0;
   }
-  __syncthreads();
-  p2g_component<0>(d, P, rc, beg, end, has, s_vw[0], s_w[0], s_touch);
-  p2g_component<1>(d, P, rc, beg, end, has, s_vw[1], s_w[1], s_touch);
-  p2g_component<2>(d, P, rc, beg, end, has, s_vw[2], s_w[2], s_touch);
-  // Normalise (flip_simulation.cc:900-922) and mark FLUID
-  // (flip_simulation.cc:852-857).
-  const int i = o[0] + (t >> 6), j = o[1] + ((t >> 3) & 7), k = o[2] + (t & 7);
+  const int lx = t >> 6, ly = (t >> 3) & 7, lz = t & 7;
+  const int c[3] = {o[0] + lx, o[1] + ly, o[2] + lz};
+  const int r = (lx + 1) * 100 + (ly + 1) * 10 + (lz + 1);
   const int64_t v = (leaf << 9) | t;
-  if (ADJ) {
-    vx[v] = s_vw[0][t]; vy[v] = s_vw[1][t]; vz[v] = s_vw[2][t];
-    return;
+  const uint32_t beg = vstart[v], end = vstart[v + 1];
+  __syncthreads();
+  p2g_own<0>(P, c, r, beg, end, s_vw[0], s_w[0], s_touch);
+  p2g_own<1>(P, c, r, beg, end, s_vw[1], s_w[1], s_touch);
+  p2g_own<2>(P, c, r, beg, end, s_vw[2], s_w[2], s_touch);
+  vx[v] = s_vw[0][r]; vy[v] = s_vw[1][r]; vz[v] = s_vw[2][r];
+  if (!ADJ) {
+    wx[v] = s_w[0][r]; wy[v] = s_w[1][r]; wz[v] = s_w[2][r];
+    vflag[v] = s_touch[r];
   }
-  if (wsx) { wsx[v] = s_w[0][t]; wsy[v] = s_w[1][t]; wsz[v] = s_w[2][t]; }
-  float out[3];
+  for (int i = t; i < kRing; i += 512) {
+    int x, y, z;
+    ring_cell(i, x, y, z);
+    const int g = x * 100 + y * 10 + z;
+    const int64_t e = leaf * kRing + i;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) ring[a * rstride + e] = s_vw[a][g];
+    if (!ADJ) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) ring[(3 + a) * rstride + e] = s_w[a][g];
+      ring_t[e] = s_touch[g];
+    }
+  }
+}
+
+// Completes the samples of every active leaf: its own partial sums (zero
+// when it holds no particles) plus, for samples on its boundary, the ring
+// entries of the neighbouring leaves with particles, neighbours in (dx, dy,
+// dz) order; then normalises (flip_simulation.cc:900-922) and marks FLUID
+// (flip_simulation.cc:852-857).
+template <bool ADJ>
+__global__ __launch_bounds__(512) void k_p2g_combine(Dom d, const int32_t *list, int64_t nlist,
+                                                     const uint32_t *lhas,
+                                                     const uint32_t *vstart, float *vx,
+                                                     float *vy, float *vz, float *wx, float *wy,
+                                                     float *wz, uint8_t *vflag, uint8_t *marker,
+                                                     const float *ring, const uint8_t *ring_t,
+                                                     int64_t rstride, float *wsx, float *wsy,
+                                                     float *wsz) {
+  const int64_t wi = xcd_work(nlist);
+  if (wi >= nlist) return;
+  const int64_t leaf = list[wi];
+  const int t = threadIdx.x;
+  const int lx = t >> 6, ly = (t >> 3) & 7, lz = t & 7;
+  const int64_t v = (leaf << 9) | t;
+  float vw[3] = {0.0f, 0.0f, 0.0f}, w[3] = {0.0f, 0.0f, 0.0f};
+  uint8_t touch = 0;
+  if (lhas[leaf]) {
+    vw[0] = vx[v]; vw[1] = vy[v]; vw[2] = vz[v];
+    if (!ADJ) {
+      w[0] = wx[v]; w[1] = wy[v]; w[2] = wz[v];
+      touch = vflag[v];
+    }
+  }
+  const int64_t l2 = d.L[2], l12 = int64_t(d.L[1]) * d.L[2];
+  const int L[3] = {int(leaf / l12), int((leaf / l2) % d.L[1]), int(leaf % l2)};
+  const int lc[3] = {lx, ly, lz};
+  int lo[3], hi[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const float ws = s_w[a][t];
-    out[a] = double(ws) < 1e-5 ? 0.0f : s_vw[a][t] / ws;
+    lo[a] = lc[a] == 0 && L[a] > 0 ? -1 : 0;
+    hi[a] = lc[a] == 7 && L[a] + 1 < d.L[a] ? 1 : 0;
   }
+  for (int dx = lo[0]; dx <= hi[0]; ++dx)
+    for (int dy = lo[1]; dy <= hi[1]; ++dy)
+      for (int dz = lo[2]; dz <= hi[2]; ++dz) {
+        if (dx == 0 && dy == 0 && dz == 0) continue;
+        const int64_t nb = leaf + dx * l12 + dy * l2 + dz;
+        if (!lhas[nb]) continue;
+        const int64_t e = nb * kRing + ring_index(lx + 1 - 8 * dx, ly + 1 - 8 * dy, lz + 1 - 8 * dz);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) vw[a] += ring[a * rstride + e];
+        if (!ADJ) {
+#pragma unroll
+          for (int a = 0; a < 3; ++a) w[a] += ring[(3 + a) * rstride + e];
+          touch |= ring_t[e];
+        }
+      }
+  if (ADJ) {
+    vx[v] = vw[0]; vy[v] = vw[1]; vz[v] = vw[2];
+    return;
+  }
+  if (wsx) { wsx[v] = w[0]; wsy[v] = w[1]; wsz[v] = w[2]; }
+  float out[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) out[a] = double(w[a]) < 1e-5 ? 0.0f : vw[a] / w[a];
   vx[v] = out[0]; vy[v] = out[1]; vz[v] = out[2];
-  vflag[v] = s_touch[t] ? 1 : 0;
+  vflag[v] = touch ? 1 : 0;
+  int o[3];
+  leaf_origin(d, leaf, o);
   uint8_t m = kAir;
-  if (in_bbox(d, i, j, k) && vstart[v + 1] > vstart[v]) m = kFluid;
+  if (in_bbox(d, o[0] + lx, o[1] + ly, o[2] + lz) && vstart[v + 1] > vstart[v]) m = kFluid;
   marker[v] = m;
 }
 
@@ -615,11 +689,25 @@ hipError_t launch_compact(fsl_ctx *c, const uint32_t *flag, int32_t *list) {
   return hipGetLastError();
 }
 
+// Ring blocks: six float planes (vw of x, y, z, then w of x, y, z) of
+// nleaves * kRing entries, then the touch bytes.
+size_t p2g_ring_bytes(int64_t nleaves) {
+  return size_t(nleaves) * kRing * (6 * sizeof(float) + 1);
+}
+
 hipError_t launch_p2g(fsl_ctx *c, float *const wsum[3]) {
   if (c->n_act == 0) return hipSuccess;
-  k_p2g<false><<<xcd_grid(c->n_act), 512, 0, c->stream>>>(
-      c->dom, c->act_list, c->n_act, c->vstart, c->P, c->v[0], c->v[1], c->v[2], c->vflag,
-      c->marker, wsum ? wsum[0] : nullptr, wsum ? wsum[1] : nullptr, wsum ? wsum[2] : nullptr);
+  const int64_t rs = c->nleaves * kRing;
+  float *ring = static_cast<float *>(c->p2g_ring);
+  uint8_t *ring_t = reinterpret_cast<uint8_t *>(ring + 6 * rs);
+  float *w[3] = {c->p2g_w, c->p2g_w + c->nvox, c->p2g_w + 2 * c->nvox};
+  k_p2g_own<false><<<xcd_grid(c->n_act), 512, 0, c->stream>>>(
+      c->dom, c->act_list, c->n_act, c->lhas, c->vstart, c->P, c->v[0], c->v[1], c->v[2], w[0],
+      w[1], w[2], c->vflag, ring, ring_t, rs);
+  k_p2g_combine<false><<<xcd_grid(c->n_act), 512, 0, c->stream>>>(
+      c->dom, c->act_list, c->n_act, c->lhas, c->vstart, c->v[0], c->v[1], c->v[2], w[0], w[1],
+      w[2], c->vflag, c->marker, ring, ring_t, rs, wsum ? wsum[0] : nullptr,
+      wsum ? wsum[1] : nullptr, wsum ? wsum[2] : nullptr);
   return hipGetLastError();
 }
 
@@ -627,9 +715,14 @@ hipError_t launch_g2p_transpose(fsl_ctx *c, float *const pval[3], float *const o
   if (c->n_act == 0) return hipSuccess;
   Particles A = c->P;
   for (int a = 0; a < 3; ++a) A.a[3 + a] = pval[a];
-  k_p2g<true><<<xcd_grid(c->n_act), 512, 0, c->stream>>>(
-      c->dom, c->act_list, c->n_act, c->vstart, A, out[0], out[1], out[2], nullptr, nullptr,
-      nullptr, nullptr, nullptr);
+  const int64_t rs = c->nleaves * kRing;
+  float *ring = static_cast<float *>(c->p2g_ring);
+  k_p2g_own<true><<<xcd_grid(c->n_act), 512, 0, c->stream>>>(
+      c->dom, c->act_list, c->n_act, c->lhas, c->vstart, A, out[0], out[1], out[2], nullptr,
+      nullptr, nullptr, nullptr, ring, nullptr, rs);
+  k_p2g_combine<true><<<xcd_grid(c->n_act), 512, 0, c->stream>>>(
+      c->dom, c->act_list, c->n_act, c->lhas, c->vstart, out[0], out[1], out[2], nullptr,
+      nullptr, nullptr, nullptr, nullptr, ring, nullptr, rs, nullptr, nullptr, nullptr);
   return hipGetLastError();
 }
 
