@@ -47,13 +47,13 @@ int sync_flags(fsl_ctx *ctx) {
 
 // Advect by dt (0 for a plain re-sort), drop deleted particles, and counting
 // sort by cell.
-int sort_particles(fsl_ctx *ctx, float dt, bool drop_outside) {
+int sort_particles(fsl_ctx *ctx, float dt, bool drop_outside, bool keep_gv) {
   FSL_HIP(fsl::launch_advect(ctx, dt, drop_outside));
   FSL_HIP(fsl::exclusive_scan(ctx->vcount, ctx->vstart, ctx->nvox, ctx->scan_tmp, ctx->stream));
   uint32_t alive = 0;
   int rc = read_u32(ctx, ctx->vstart + ctx->nvox, &alive);
   if (rc) return rc;
-  FSL_HIP(fsl::launch_sort_tail(ctx, alive));
+  FSL_HIP(fsl::launch_sort_tail(ctx, alive, keep_gv));
   ctx->outside_deleted = false;
   return FSL_OK;
 }
@@ -268,7 +268,7 @@ int fsl_set_particles(fsl_ctx *ctx, int64_t n, const float *pos, const float *ve
     if (n) FSL_HIP(hipMemcpy(ctx->P.a[a], col.data(), size_t(n) * sizeof(float), hipMemcpyHostToDevice));
   }
   ctx->np = n;
-  return sort_particles(ctx, 0.0f, false);
+  return sort_particles(ctx, 0.0f, false, true);
 }
 
 int fsl_seed_particles(fsl_ctx *ctx, const uint8_t *mask, int ppc, uint32_t seed,
@@ -296,7 +296,7 @@ int fsl_seed_particles(fsl_ctx *ctx, const uint8_t *mask, int ppc, uint32_t seed
   if (n > ctx->cap) { ctx->err = "particle capacity exceeded"; return done(FSL_ERR_CAPACITY); }
   if (fsl::launch_seed(ctx, mdev, offs, ppc, seed, velocity) != hipSuccess) return done(FSL_ERR_HIP);
   ctx->np = n;
-  rc = sort_particles(ctx, 0.0f, false);
+  rc = sort_particles(ctx, 0.0f, false, true);
   return done(rc);
 }
 
@@ -372,7 +372,7 @@ int fsl_cfl(fsl_ctx *ctx, float *inv_speed) {
 
 int fsl_move_particles(fsl_ctx *ctx, float dt) {
   CHECK_CTX();
-  return sort_particles(ctx, dt, ctx->outside_deleted);
+  return sort_particles(ctx, dt, ctx->outside_deleted, true);
 }
 
 int fsl_clear(fsl_ctx *ctx) {
@@ -522,7 +522,9 @@ int fsl_step(fsl_ctx *ctx, float dt, const fsl_solve_params *params, fsl_solve_s
   if ((rc = mark(2 * (s)))) return rc;     \
   body;                                    \
   if ((rc = mark(2 * (s) + 1))) return rc;
-  STAGE(FSL_T_MOVE, if ((rc = fsl_move_particles(ctx, dt))) return rc)
+  // Within a step nothing reads the particles' grid velocities between the
+  // move and G2P, which rewrites them: the sort leaves them behind.
+  STAGE(FSL_T_MOVE, if ((rc = sort_particles(ctx, dt, ctx->outside_deleted, false))) return rc)
   STAGE(FSL_T_CLEAR, if ((rc = fsl_clear(ctx))) return rc)
   STAGE(FSL_T_P2G, if ((rc = fsl_p2g(ctx))) return rc)
   // ApplyBoundaryConditions + SaveGridVelocities in one pass.

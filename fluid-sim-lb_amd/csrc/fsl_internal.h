@@ -155,7 +155,7 @@ hipError_t exclusive_scan(const uint32_t *in, uint32_t *out, int64_t n, uint32_t
 hipError_t launch_seed(fsl_ctx *c, const uint8_t *mask_dev, const uint32_t *offs, int ppc,
                        uint32_t seed, const float vel[3]);
 hipError_t launch_advect(fsl_ctx *c, float dt, bool drop_outside);
-hipError_t launch_sort_tail(fsl_ctx *c, uint32_t n_alive);
+hipError_t launch_sort_tail(fsl_ctx *c, uint32_t n_alive, bool keep_gv = true);
 hipError_t launch_mask_count(fsl_ctx *c, const uint8_t *mask_dev, int64_t n, uint32_t *cnt);
 hipError_t launch_leaf_flags(fsl_ctx *c);
 hipError_t launch_compact(fsl_ctx *c, const uint32_t *flag, int32_t *list);

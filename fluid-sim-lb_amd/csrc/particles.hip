@@ -141,8 +141,13 @@ __global__ void k_scatter(int64_t np, const uint32_t *key, const uint32_t *vstar
 }
 
 // Restore the previous order inside each cell (stable sort). The thread at
-// a cell's first slot checks the cell's old indices and insertion-sorts them
-// when runs arrived out of order (a cell filled by several runs).
+// a cell's first slot checks the cell's old indices and sorts them when runs
+// arrived out of order (a cell filled by several runs). A cell of at most
+// kCellRegs particles is loaded into registers with all loads in flight at
+// once, checked and sorted there (odd-even transposition, padding sorts
+// last) and written back only when it was out of order; larger cells take
+// the insertion sort in memory.
+constexpr uint32_t kCellRegs = 16;
 __global__ void k_cell_sort(uint32_t n, const uint32_t *skey, const uint32_t *vstart,
                             uint32_t *perm) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -151,6 +156,33 @@ __global__ void k_cell_sort(uint32_t n, const uint32_t *skey, const uint32_t *vs
   if (s > 0 && skey[s - 1] == k) return;
   const uint32_t b = s, e = vstart[k + 1];
   if (e - b < 2) return;
+  const uint32_t m = e - b;
+  if (m <= kCellRegs) {
+    uint32_t v[kCellRegs];
+#pragma unroll
+    for (uint32_t i = 0; i < kCellRegs; ++i) v[i] = perm[b + (i < m ? i : m - 1)];
+    bool in_order = true;
+#pragma unroll
+    for (uint32_t i = 1; i < kCellRegs; ++i)
+      if (i < m) in_order = in_order && v[i - 1] < v[i];
+    if (in_order) return;
+#pragma unroll
+    for (uint32_t i = 0; i < kCellRegs; ++i)
+      if (i >= m) v[i] = 0xFFFFFFFFu;
+#pragma unroll
+    for (uint32_t round = 0; round < kCellRegs; ++round)
+#pragma unroll
+      for (uint32_t i = round & 1; i + 1 < kCellRegs; i += 2) {
+        const uint32_t lo = v[i] < v[i + 1] ? v[i] : v[i + 1];
+        const uint32_t hi = v[i] < v[i + 1] ? v[i + 1] : v[i];
+        v[i] = lo;
+        v[i + 1] = hi;
+      }
+#pragma unroll
+    for (uint32_t i = 0; i < kCellRegs; ++i)
+      if (i < m) perm[b + i] = v[i];
+    return;
+  }
   bool sorted = true;
   for (uint32_t x = b + 1; x < e && sorted; ++x) sorted = perm[x - 1] < perm[x];
   if (sorted) return;
@@ -162,12 +194,15 @@ __global__ void k_cell_sort(uint32_t n, const uint32_t *skey, const uint32_t *vs
   }
 }
 
+// Moves arrays 0..NA-1 into sorted order (NA = 6 leaves out the grid
+// velocities gx, gy, gz, which only advection reads and G2P rewrites).
+template <int NA>
 __global__ void k_gather(uint32_t n, const uint32_t *perm, Particles in, Particles out) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const uint32_t q = perm[s];
 #pragma unroll
-  for (int a = 0; a < 9; ++a) out.a[a][s] = in.a[a][q];
+  for (int a = 0; a < NA; ++a) out.a[a][s] = in.a[a][q];
 }
 
 // ------------------------------------------------------------- leaf lists
@@ -649,8 +684,9 @@ hipError_t launch_advect(fsl_ctx *c, float dt, bool drop_outside) {
 }
 
 // After the advection kernel and the scan of vcount into vstart: scatter,
-// restore in-cell order, gather into the other buffer and swap.
-hipError_t launch_sort_tail(fsl_ctx *c, uint32_t n_alive) {
+// restore in-cell order, gather into the other buffer and swap. Without
+// keep_gv the grid velocities are left behind (stale until G2P writes them).
+hipError_t launch_sort_tail(fsl_ctx *c, uint32_t n_alive, bool keep_gv) {
   if (c->np > 0) {
     k_scatter<<<blocks_for(c->np, kThreads), kThreads, 0, c->stream>>>(
         c->np, c->key, c->vstart, c->vcount, c->perm, c->skey);
@@ -658,8 +694,12 @@ hipError_t launch_sort_tail(fsl_ctx *c, uint32_t n_alive) {
   if (n_alive > 0) {
     k_cell_sort<<<blocks_for(n_alive, kThreads), kThreads, 0, c->stream>>>(n_alive, c->skey,
                                                                             c->vstart, c->perm);
-    k_gather<<<blocks_for(n_alive, kThreads), kThreads, 0, c->stream>>>(n_alive, c->perm, c->P,
-                                                                         c->Q);
+    if (keep_gv)
+      k_gather<9><<<blocks_for(n_alive, kThreads), kThreads, 0, c->stream>>>(n_alive, c->perm,
+                                                                              c->P, c->Q);
+    else
+      k_gather<6><<<blocks_for(n_alive, kThreads), kThreads, 0, c->stream>>>(n_alive, c->perm,
+                                                                              c->P, c->Q);
   }
   Particles t = c->P;
   c->P = c->Q;
