@@ -44,14 +44,30 @@ __device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t *lds, uint32_t *
   return res;
 }
 
+// The kScanItems values of a thread: two 16-byte loads when all of them are
+// in range and the array is 16-byte aligned, else one load each (0 past n).
+__device__ inline void load_items(const uint32_t *in, int64_t base, int64_t n,
+                                  uint32_t v[kScanItems]) {
+  if (base + kScanItems <= n && (reinterpret_cast<uintptr_t>(in) & 15) == 0) {
+    const uint4 a = *reinterpret_cast<const uint4 *>(in + base);
+    const uint4 b = *reinterpret_cast<const uint4 *>(in + base + 4);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+  } else {
+#pragma unroll
+    for (int q = 0; q < kScanItems; ++q) v[q] = base + q < n ? in[base + q] : 0u;
+  }
+}
+
 __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t *in, int64_t n,
                                                               uint32_t *tile_sum) {
   __shared__ uint32_t lds[64];
   const int64_t base = int64_t(blockIdx.x) * kScanTile + int64_t(threadIdx.x) * kScanItems;
+  uint32_t vals[kScanItems];
+  load_items(in, base, n, vals);
   uint32_t s = 0;
 #pragma unroll
-  for (int q = 0; q < kScanItems; ++q)
-    if (base + q < n) s += in[base + q];
+  for (int q = 0; q < kScanItems; ++q) s += vals[q];
   uint32_t tot;
   block_excl_scan(s, lds, &tot);
   if (threadIdx.x == 0) tile_sum[blockIdx.x] = tot;
@@ -81,20 +97,26 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(const uint32_t *in,
   __shared__ uint32_t lds[64];
   const int64_t base = int64_t(blockIdx.x) * kScanTile + int64_t(threadIdx.x) * kScanItems;
   uint32_t vals[kScanItems];
+  load_items(in, base, n, vals);
   uint32_t s = 0;
 #pragma unroll
-  for (int q = 0; q < kScanItems; ++q) {
-    vals[q] = base + q < n ? in[base + q] : 0u;
-    s += vals[q];
-  }
+  for (int q = 0; q < kScanItems; ++q) s += vals[q];
   uint32_t tot;
   uint32_t run = block_excl_scan(s, lds, &tot) + tile_off[blockIdx.x];
+  uint32_t res[kScanItems];
 #pragma unroll
-  for (int q = 0; q < kScanItems; ++q)
-    if (base + q < n) {
-      out[base + q] = run;
-      run += vals[q];
-    }
+  for (int q = 0; q < kScanItems; ++q) {
+    res[q] = run;
+    run += vals[q];
+  }
+  if (base + kScanItems <= n && (reinterpret_cast<uintptr_t>(out) & 15) == 0) {
+    *reinterpret_cast<uint4 *>(out + base) = make_uint4(res[0], res[1], res[2], res[3]);
+    *reinterpret_cast<uint4 *>(out + base + 4) = make_uint4(res[4], res[5], res[6], res[7]);
+  } else {
+#pragma unroll
+    for (int q = 0; q < kScanItems; ++q)
+      if (base + q < n) out[base + q] = res[q];
+  }
 }
 
 }  // namespace
