@@ -344,40 +344,47 @@ __device__ inline int ring_index(int x, int y, int z) {
   return 360 + (z ? 64 : 0) + (x - 1) * 8 + (y - 1);
 }
 
-// One component of a leaf's own cells: thread t sums its cell's particles
-// into the 18 slots in registers, then the slots are committed to the
-// region's LDS samples one phase at a time (within a phase every cell
-// targets a different sample, so the order is fixed and no atomics are
-// needed). r is the cell's region index (x * 100 + y * 10 + z).
-template <int D>
+// A leaf's own cells, all three components in one pass over the particles
+// (each particle's position and velocity are loaded once): thread t sums
+// its cell's particles into the 18 slots of each component in registers,
+// then the slots are committed to the region's LDS samples one phase at a
+// time, the three components in the same phase (within a phase every cell
+// targets a different sample of each component, so the order is fixed and
+// no atomics are needed; the touch flags are only ever set to 1). r is the
+// cell's region index (x * 100 + y * 10 + z).
 __device__ inline void p2g_own(const Particles &P, const int c[3], int r, uint32_t beg,
-                               uint32_t end, float *s_vw, float *s_w, uint8_t *s_touch) {
-  float avw[18], aw[18];
-  uint32_t cls = 0u;
+                               uint32_t end, float (*s_vw)[1000], float (*s_w)[1000],
+                               uint8_t *s_touch) {
+  float avw[3][18], aw[3][18];
+  uint32_t cls[3] = {0u, 0u, 0u};
 #pragma unroll
-  for (int q = 0; q < 18; ++q) { avw[q] = 0.0f; aw[q] = 0.0f; }
-  // Four particles per trip, their loads issued together (indices clamped
-  // to the cell's last particle; the clamped copies are not accumulated);
-  // the sums run in particle order.
-  for (uint32_t q = beg; q < end; q += 4) {
-    float px[4], py[4], pz[4], pv[4];
+  for (int a = 0; a < 3; ++a)
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int q = 0; q < 18; ++q) { avw[a][q] = 0.0f; aw[a][q] = 0.0f; }
+  // Two particles per trip, their loads issued together (indices clamped to
+  // the cell's last particle; the clamped copy is not accumulated); the sums
+  // run in particle order.
+  for (uint32_t q = beg; q < end; q += 2) {
+    float pp[2][3], pv[2][3];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
       const uint32_t qq = q + u < end ? q + u : end - 1;
-      px[u] = P.a[0][qq];
-      py[u] = P.a[1][qq];
-      pz[u] = P.a[2][qq];
-      pv[u] = P.a[3 + D][qq];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        pp[u][a] = P.a[a][qq];
+        pv[u][a] = P.a[3 + a][qq];
+      }
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < 2; ++u) {
       if (q + u >= end) break;
-      const float pos[3] = {px[u], py[u], pz[u]};
-      p2g_particle<D>(pos, c, pv[u], avw, aw, cls);
+      p2g_particle<0>(pp[u], c, pv[u][0], avw[0], aw[0], cls[0]);
+      p2g_particle<1>(pp[u], c, pv[u][1], avw[1], aw[1], cls[1]);
+      p2g_particle<2>(pp[u], c, pv[u][2], avw[2], aw[2], cls[2]);
     }
   }
-  constexpr int A = Axes<D>::a, B = Axes<D>::b;
   constexpr int st[3] = {100, 10, 1};
+  const bool any = end > beg;
 #pragma unroll
   for (int sd = 0; sd < 2; ++sd)
 #pragma unroll
@@ -385,11 +392,15 @@ __device__ inline void p2g_own(const Particles &P, const int c[3], int r, uint32
 #pragma unroll
       for (int sb = 0; sb < 3; ++sb) {
         const int slot = sd * 9 + sa * 3 + sb;
-        if (end > beg) {
-          const int g = r + sd * st[D] + (sa - 1) * st[A] + (sb - 1) * st[B];
-          s_vw[g] += avw[slot];
-          s_w[g] += aw[slot];
-          if (cls & slot_classes(sa, sb)) s_touch[g] = 1;
+        if (any) {
+#pragma unroll
+          for (int D = 0; D < 3; ++D) {
+            const int A = D == 0 ? 1 : 0, B = D == 2 ? 1 : 2;
+            const int g = r + sd * st[D] + (sa - 1) * st[A] + (sb - 1) * st[B];
+            s_vw[D][g] += avw[D][slot];
+            s_w[D][g] += aw[D][slot];
+            if (cls[D] & slot_classes(sa, sb)) s_touch[g] = 1;
+          }
         }
         __syncthreads();
       }
@@ -398,7 +409,7 @@ __device__ inline void p2g_own(const Particles &P, const int c[3], int r, uint32
 // TransferParticlesToGrid (flip_simulation.cc:808-923) restated as a gather
 // in two kernels. k_p2g_own: one workgroup per leaf with particles sums its
 // own 512 cells' particles into the samples of its 10^3 region (each
-// particle is visited once per component); the own samples' partial sums go
+// particle is visited once); the own samples' partial sums go
 // to vx/vy/vz, ws* and vflag, the ring samples (those of neighbouring
 // leaves) to the leaf's ring block. k_p2g_combine then completes every
 // active leaf's samples from the ring blocks of its neighbours, in a fixed
@@ -433,9 +444,7 @@ __global__ __launch_bounds__(512) void k_p2g_own(Dom d, const int32_t *list, int
   const int64_t v = (leaf << 9) | t;
   const uint32_t beg = vstart[v], end = vstart[v + 1];
   __syncthreads();
-  p2g_own<0>(P, c, r, beg, end, s_vw[0], s_w[0], s_touch);
-  p2g_own<1>(P, c, r, beg, end, s_vw[1], s_w[1], s_touch);
-  p2g_own<2>(P, c, r, beg, end, s_vw[2], s_w[2], s_touch);
+  p2g_own(P, c, r, beg, end, s_vw, s_w, s_touch);
   vx[v] = s_vw[0][r]; vy[v] = s_vw[1][r]; vz[v] = s_vw[2][r];
   if (!ADJ) {
     wx[v] = s_w[0][r]; wy[v] = s_w[1][r]; wz[v] = s_w[2][r];
