@@ -594,22 +594,41 @@ __global__ __launch_bounds__(256) void k_g2p(Dom d, const int32_t *list, int64_t
   }
   __syncthreads();
   const uint32_t b = vstart[leaf << 9], e = vstart[(leaf + 1) << 9];
-  for (uint32_t q = b + threadIdx.x; q < e; q += blockDim.x) {
-    const float pos[3] = {P.a[0][q], P.a[1][q], P.a[2][q]};
-    const int c[3] = {int(floorf(pos[0])), int(floorf(pos[1])), int(floorf(pos[2]))};
-    if (!in_bbox(d, c[0], c[1], c[2])) continue;  // removed by DeleteOutsideParticles
-    const int lc[3] = {c[0] - o[0] + 1, c[1] - o[1] + 1, c[2] - o[2] + 1};
-    const float gv[3] = {interp<0>(pos, c, lc, s_v[0]), interp<1>(pos, c, lc, s_v[1]),
-                         interp<2>(pos, c, lc, s_v[2])};
-    const float dv[3] = {interp<0>(pos, c, lc, s_u[0]), interp<1>(pos, c, lc, s_u[1]),
-                         interp<2>(pos, c, lc, s_u[2])};
+  // Four particles per trip (q, q + 256, ...), their loads issued together
+  // (indices clamped to the leaf's last particle; clamped copies are not
+  // processed).
+  constexpr uint32_t kTrip = 4;
+  for (uint32_t q0 = b + threadIdx.x; q0 < e; q0 += kTrip * blockDim.x) {
+    float pp[kTrip][3], pv[kTrip][3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      const float s1 = dv[a] + P.a[3 + a][q];
-      const float s2 = s1 - gv[a];
-      const float s3 = flip * s2;
-      P.a[3 + a][q] = gv[a] + s3;
-      P.a[6 + a][q] = gv[a];
+    for (uint32_t u = 0; u < kTrip; ++u) {
+      const uint32_t q = q0 + u * blockDim.x, qq = q < e ? q : e - 1;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        pp[u][a] = P.a[a][qq];
+        pv[u][a] = P.a[3 + a][qq];
+      }
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kTrip; ++u) {
+      const uint32_t q = q0 + u * blockDim.x;
+      if (q >= e) break;
+      const float *pos = pp[u];
+      const int c[3] = {int(floorf(pos[0])), int(floorf(pos[1])), int(floorf(pos[2]))};
+      if (!in_bbox(d, c[0], c[1], c[2])) continue;  // removed by DeleteOutsideParticles
+      const int lc[3] = {c[0] - o[0] + 1, c[1] - o[1] + 1, c[2] - o[2] + 1};
+      const float gv[3] = {interp<0>(pos, c, lc, s_v[0]), interp<1>(pos, c, lc, s_v[1]),
+                           interp<2>(pos, c, lc, s_v[2])};
+      const float dv[3] = {interp<0>(pos, c, lc, s_u[0]), interp<1>(pos, c, lc, s_u[1]),
+                           interp<2>(pos, c, lc, s_u[2])};
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const float s1 = dv[a] + pv[u][a];
+        const float s2 = s1 - gv[a];
+        const float s3 = flip * s2;
+        P.a[3 + a][q] = gv[a] + s3;
+        P.a[6 + a][q] = gv[a];
+      }
     }
   }
 }
