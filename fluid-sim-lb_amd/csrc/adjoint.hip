@@ -691,6 +691,7 @@ int fsl_rollout_checkpoint(fsl_ctx *ctx, int k, float *pos, float *vel, uint32_t
   if (n) *n = c.n;
   if (n_prev) *n_prev = c.n_prev;
   if (dt) *dt = c.dt;
+  FSL_HIP(hipStreamSynchronize(ctx->stream));  // hipMemcpy does not wait for it
   std::vector<float> tmp(size_t(c.n));
   for (int a = 0; a < 6; ++a) {
     float *dst = a < 3 ? pos : vel;

@@ -47,8 +47,9 @@ int sync_flags(fsl_ctx *ctx) {
 
 // Advect by dt (0 for a plain re-sort), drop deleted particles, and counting
 // sort by cell.
-int sort_particles(fsl_ctx *ctx, float dt, bool drop_outside, bool keep_gv) {
-  FSL_HIP(fsl::launch_advect(ctx, dt, drop_outside));
+int sort_particles(fsl_ctx *ctx, float dt, bool drop_outside, bool keep_gv,
+                   const fsl::KillBoxes &kill = fsl::KillBoxes()) {
+  FSL_HIP(fsl::launch_advect(ctx, dt, drop_outside, kill));
   FSL_HIP(fsl::exclusive_scan(ctx->vcount, ctx->vstart, ctx->nvox, ctx->scan_tmp, ctx->stream));
   uint32_t alive = 0;
   int rc = read_u32(ctx, ctx->vstart + ctx->nvox, &alive);
@@ -198,6 +199,7 @@ int fsl_destroy(fsl_ctx *ctx) {
     if (ctx->P.a[a]) (void)hipFree(ctx->P.a[a]);
     if (ctx->Q.a[a]) (void)hipFree(ctx->Q.a[a]);
   }
+  if (ctx->src_buf) (void)hipFree(ctx->src_buf);
   for (hipEvent_t e : ctx->tev) if (e) (void)hipEventDestroy(e);
   if (ctx->flags_host) (void)hipHostFree(ctx->flags_host);
   if (ctx->pcg_host) (void)hipHostFree(ctx->pcg_host);
@@ -226,6 +228,9 @@ int fsl_set_params(fsl_ctx *ctx, float gravity_scaled, const float force[3], flo
 
 int fsl_set_box_boundary(fsl_ctx *ctx, int tank) {
   CHECK_CTX();
+  // hipMemcpy runs on the null stream, which does not wait for the
+  // context's (non-blocking) stream: drain it first.
+  FSL_HIP(hipStreamSynchronize(ctx->stream));
   const Dom &d = ctx->dom;
   std::vector<uint8_t> s(size_t(ctx->nvox), 0);
   FSL_HIP(hipMemcpy(s.data(), ctx->solid, s.size(), hipMemcpyDeviceToHost));
@@ -243,6 +248,7 @@ int fsl_set_box_boundary(fsl_ctx *ctx, int tank) {
 int fsl_add_solid_box(fsl_ctx *ctx, const int lo[3], const int hi[3]) {
   CHECK_CTX();
   if (!lo || !hi) return FSL_ERR_ARG;
+  FSL_HIP(hipStreamSynchronize(ctx->stream));
   const Dom &d = ctx->dom;
   std::vector<uint8_t> s(size_t(ctx->nvox), 0);
   FSL_HIP(hipMemcpy(s.data(), ctx->solid, s.size(), hipMemcpyDeviceToHost));
@@ -260,6 +266,7 @@ int fsl_set_particles(fsl_ctx *ctx, int64_t n, const float *pos, const float *ve
   CHECK_CTX();
   if (n < 0 || (n > 0 && (!pos || !vel))) return FSL_ERR_ARG;
   if (n > ctx->cap) { ctx->err = "particle capacity exceeded"; return FSL_ERR_CAPACITY; }
+  FSL_HIP(hipStreamSynchronize(ctx->stream));
   std::vector<float> col(static_cast<size_t>(n));
   for (int a = 0; a < 9; ++a) {
     const float *src = a < 3 ? pos : a < 6 ? vel : (gv ? gv : vel);
@@ -300,10 +307,71 @@ int fsl_seed_particles(fsl_ctx *ctx, const uint8_t *mask, int ppc, uint32_t seed
   return done(rc);
 }
 
+int fsl_add_source(fsl_ctx *ctx, const float lo[3], const float hi[3], const float velocity[3],
+                   int ppc, uint32_t seed, uint32_t stream) {
+  CHECK_CTX();
+  if (!lo || !hi || !velocity || ppc <= 0) return FSL_ERR_ARG;
+  fsl::SourceBox s{};
+  int64_t ncell = 1;
+  for (int e = 0; e < 3; ++e) {
+    // Coord(lo), Coord(hi): conversion to int truncates; then the
+    // intersection with the domain box.
+    s.c0[e] = std::max(int(lo[e]), 0);
+    s.c1[e] = std::min(int(hi[e]), ctx->dom.n[e] - 1);
+    if (s.c1[e] < s.c0[e]) return FSL_OK;
+    ncell *= s.c1[e] - s.c0[e] + 1;
+    s.lo[e] = lo[e];
+    s.hi[e] = hi[e];
+    s.vel[e] = velocity[e];
+  }
+  s.ppc = ppc;
+  s.k0 = seed;
+  s.k1 = stream;
+  if (ctx->src_cap < ncell + 1) {
+    if (ctx->src_buf) FSL_HIP(hipFree(ctx->src_buf));
+    ctx->src_buf = nullptr;
+    ctx->src_cap = 0;
+    FSL_HIP(hipMalloc(reinterpret_cast<void **>(&ctx->src_buf), size_t(ncell + 1) * sizeof(uint32_t)));
+    ctx->src_cap = ncell + 1;
+  }
+  FSL_HIP(fsl::launch_source_count(ctx, s, ncell, ctx->src_buf));
+  FSL_HIP(fsl::exclusive_scan(ctx->src_buf, ctx->src_buf, ncell, ctx->scan_tmp, ctx->stream));
+  uint32_t total = 0;
+  int rc = read_u32(ctx, ctx->src_buf + ncell, &total);
+  if (rc) return rc;
+  if (ctx->np + int64_t(total) > ctx->cap) {
+    ctx->err = "particle capacity exceeded";
+    return FSL_ERR_CAPACITY;
+  }
+  if (total) FSL_HIP(fsl::launch_source_emit(ctx, s, ncell, ctx->src_buf, ctx->np));
+  ctx->np += total;
+  return FSL_OK;
+}
+
+int fsl_delete_particles_in_boxes(fsl_ctx *ctx, int n, const float *boxes) {
+  CHECK_CTX();
+  if (n < 0 || (n > 0 && !boxes)) return FSL_ERR_ARG;
+  // A re-sort without advection (dt = 0) whose advection pass also drops
+  // the particles in the boxes; up to kMaxKillBoxes boxes per pass.
+  for (int b0 = 0; b0 < n; b0 += fsl::kMaxKillBoxes) {
+    fsl::KillBoxes kill;
+    kill.n = std::min(fsl::kMaxKillBoxes, n - b0);
+    for (int b = 0; b < kill.n; ++b)
+      for (int e = 0; e < 3; ++e) {
+        kill.lo[b][e] = boxes[6 * (b0 + b) + e];
+        kill.hi[b][e] = boxes[6 * (b0 + b) + 3 + e];
+      }
+    int rc = sort_particles(ctx, 0.0f, ctx->outside_deleted, true, kill);
+    if (rc) return rc;
+  }
+  return FSL_OK;
+}
+
 int fsl_num_particles(fsl_ctx *ctx, int64_t *n) {
   CHECK_CTX();
   if (!n) return FSL_ERR_ARG;
   if (!ctx->outside_deleted) { *n = ctx->np; return FSL_OK; }
+  FSL_HIP(hipStreamSynchronize(ctx->stream));
   // Particles outside the partition box are deleted lazily (dropped by the
   // next sort); count the ones that remain.
   int64_t m = 0;
@@ -325,6 +393,7 @@ int fsl_num_particles(fsl_ctx *ctx, int64_t *n) {
 
 int fsl_get_particles(fsl_ctx *ctx, float *pos, float *vel, float *gv) {
   CHECK_CTX();
+  FSL_HIP(hipStreamSynchronize(ctx->stream));
   const int64_t n = ctx->np;
   std::vector<float> col[9];
   for (int a = 0; a < 9; ++a) {
@@ -702,6 +771,7 @@ int fsl_get_active_leaves(fsl_ctx *ctx, int which, int32_t *origins, int64_t cap
   const int64_t n = which == 0 ? ctx->n_act : ctx->n_fluid;
   *count = n;
   if (!origins) return FSL_OK;
+  FSL_HIP(hipStreamSynchronize(ctx->stream));
   std::vector<int32_t> list(static_cast<size_t>(n));
   if (n) FSL_HIP(hipMemcpy(list.data(), which == 0 ? ctx->act_list : ctx->fluid_list,
                            size_t(n) * sizeof(int32_t), hipMemcpyDeviceToHost));

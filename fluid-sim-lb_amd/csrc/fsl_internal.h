@@ -76,6 +76,24 @@ struct PcgState {
 };
 
 // Device-side error flags of the grid stages.
+// Particle deletion boxes applied by the advection kernel (positions after
+// the move; bounds inclusive, as Cube::IsInside, primitives.cc:57-64).
+constexpr int kMaxKillBoxes = 16;
+struct KillBoxes {
+  int n = 0;
+  float lo[kMaxKillBoxes][3], hi[kMaxKillBoxes][3];
+};
+// One evaluated water source (AddContributionFromSources,
+// flip_simulation.cc:1206-1266): the cells c0..c1 (inclusive, inside the
+// domain), the box used for the inside test, the velocity, the particles per
+// cell and the Philox key.
+struct SourceBox {
+  int c0[3], c1[3];
+  float lo[3], hi[3], vel[3];
+  int ppc;
+  uint32_t k0, k1;
+};
+
 struct DevFlags {
   unsigned zero_diag;
   unsigned vel_update;
@@ -133,6 +151,9 @@ struct fsl_ctx {
   uint32_t *vcount = nullptr, *vstart = nullptr;  // per voxel; vstart has nvox + 1
   uint32_t *scan_tmp = nullptr;
   int64_t scan_tmp_len = 0;
+  // Per-cell counts and offsets of a water source (grown on demand).
+  uint32_t *src_buf = nullptr;
+  int64_t src_cap = 0;
   // Flags and staging.
   fsl::DevFlags *flags = nullptr, *flags_host = nullptr;
   uint32_t *host_u32 = nullptr;  // pinned scratch for small readbacks
@@ -154,7 +175,10 @@ hipError_t exclusive_scan(const uint32_t *in, uint32_t *out, int64_t n, uint32_t
 // particles.hip
 hipError_t launch_seed(fsl_ctx *c, const uint8_t *mask_dev, const uint32_t *offs, int ppc,
                        uint32_t seed, const float vel[3]);
-hipError_t launch_advect(fsl_ctx *c, float dt, bool drop_outside);
+hipError_t launch_advect(fsl_ctx *c, float dt, bool drop_outside, const KillBoxes &kill);
+hipError_t launch_source_count(fsl_ctx *c, const SourceBox &s, int64_t ncell, uint32_t *cnt);
+hipError_t launch_source_emit(fsl_ctx *c, const SourceBox &s, int64_t ncell, const uint32_t *off,
+                              int64_t base);
 hipError_t launch_sort_tail(fsl_ctx *c, uint32_t n_alive, bool keep_gv = true);
 hipError_t launch_mask_count(fsl_ctx *c, const uint8_t *mask_dev, int64_t n, uint32_t *cnt);
 hipError_t launch_leaf_flags(fsl_ctx *c);

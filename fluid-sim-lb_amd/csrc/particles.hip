@@ -57,6 +57,64 @@ __global__ void k_seed(Dom d, const uint8_t *mask, const uint32_t *offs, int ppc
   }
 }
 
+// AddContributionFromSources (flip_simulation.cc:1206-1266), particles only
+// (the marker and face velocities the reference also sets there are cleared
+// by ClearGridData before anything reads them). Cell m of the source's cell
+// box (x-major) makes up to 32 attempts, each a point cell + (u0, u1, u2)
+// with u uniform in [0, 1) from Philox4x32-10 (counter (cell, attempt, 2, 0),
+// key (k0, k1)) in place of glibc random(); points inside the box (bounds
+// inclusive) are kept until ppc of them are.
+__device__ inline void source_cell(const SourceBox &s, int64_t m, int c[3]) {
+  const int64_t e1 = s.c1[1] - s.c0[1] + 1, e2 = s.c1[2] - s.c0[2] + 1;
+  c[0] = s.c0[0] + int(m / (e1 * e2));
+  c[1] = s.c0[1] + int((m / e2) % e1);
+  c[2] = s.c0[2] + int(m % e2);
+}
+template <typename F>
+__device__ inline void source_points(const Dom &d, const SourceBox &s, const int c[3], F keep) {
+  const uint32_t lin = uint32_t((int64_t(c[0]) * d.n[1] + c[1]) * d.n[2] + c[2]);
+  int kept = 0;
+  for (int a = 0; a < 32 && kept < s.ppc; ++a) {
+    uint32_t x[4] = {lin, uint32_t(a), 2u, 0u};
+    philox10(x, s.k0, s.k1);
+    float p[3];
+    bool in = true;
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+      p[e] = __fadd_rn(float(c[e]), float(x[e] >> 8) * (1.0f / 16777216.0f));
+      in = in && p[e] >= s.lo[e] && p[e] <= s.hi[e];
+    }
+    if (!in) continue;
+    keep(kept, p);
+    ++kept;
+  }
+}
+__global__ void k_source_count(Dom d, SourceBox s, int64_t ncell, uint32_t *cnt) {
+  const int64_t m = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (m >= ncell) return;
+  int c[3];
+  source_cell(s, m, c);
+  uint32_t n = 0;
+  source_points(d, s, c, [&](int, const float *) { ++n; });
+  cnt[m] = n;
+}
+__global__ void k_source_emit(Dom d, SourceBox s, int64_t ncell, const uint32_t *off,
+                              Particles P, int64_t base) {
+  const int64_t m = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (m >= ncell) return;
+  int c[3];
+  source_cell(s, m, c);
+  const int64_t o = base + off[m];
+  source_points(d, s, c, [&](int j, const float *p) {
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+      P.a[e][o + j] = p[e];
+      P.a[3 + e][o + j] = s.vel[e];
+      P.a[6 + e][o + j] = s.vel[e];
+    }
+  });
+}
+
 __global__ void k_mask_to_count(const uint8_t *mask, int64_t n, uint32_t *cnt) {
   const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (q < n) cnt[q] = mask[q] ? 1u : 0u;
@@ -87,7 +145,7 @@ __device__ inline KeyRun key_run(uint32_t key) {
 // (particles.cc:202-219). Particles outside the partition box were deleted by
 // DeleteOutsideParticles after the previous P2G; they are dropped here.
 __global__ void k_advect(Dom d, int64_t np, Particles P, float dt, const uint8_t *solid,
-                         uint32_t *key, uint32_t *vcount, int drop_outside) {
+                         uint32_t *key, uint32_t *vcount, int drop_outside, KillBoxes kill) {
 #pragma clang fp contract(off)
   const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   uint32_t k = kDead;
@@ -112,7 +170,12 @@ __global__ void k_advect(Dom d, int64_t np, Particles P, float dt, const uint8_t
         if (id[a] >= d.n[a]) { id[a] = d.n[a]; p[a] = float(d.n[a]); }
       P.a[0][q] = p[0]; P.a[1][q] = p[1]; P.a[2][q] = p[2];
       const int64_t v = vox_of(d, id[0], id[1], id[2]);
-      if (!solid[v]) k = uint32_t(v);
+      bool killed = false;
+      for (int b = 0; b < kill.n; ++b)
+        killed = killed || (p[0] >= kill.lo[b][0] && p[0] <= kill.hi[b][0] &&
+                            p[1] >= kill.lo[b][1] && p[1] <= kill.hi[b][1] &&
+                            p[2] >= kill.lo[b][2] && p[2] <= kill.hi[b][2]);
+      if (!solid[v] && !killed) k = uint32_t(v);
     }
     key[q] = k;
   }
@@ -704,10 +767,22 @@ hipError_t launch_mask_count(fsl_ctx *c, const uint8_t *mask_dev, int64_t n, uin
   return hipGetLastError();
 }
 
-hipError_t launch_advect(fsl_ctx *c, float dt, bool drop_outside) {
+hipError_t launch_advect(fsl_ctx *c, float dt, bool drop_outside, const KillBoxes &kill) {
   if (c->np == 0) return hipSuccess;
   k_advect<<<blocks_for(c->np, kThreads), kThreads, 0, c->stream>>>(
-      c->dom, c->np, c->P, dt, c->solid, c->key, c->vcount, drop_outside ? 1 : 0);
+      c->dom, c->np, c->P, dt, c->solid, c->key, c->vcount, drop_outside ? 1 : 0, kill);
+  return hipGetLastError();
+}
+
+hipError_t launch_source_count(fsl_ctx *c, const SourceBox &s, int64_t ncell, uint32_t *cnt) {
+  k_source_count<<<blocks_for(ncell, kThreads), kThreads, 0, c->stream>>>(c->dom, s, ncell, cnt);
+  return hipGetLastError();
+}
+
+hipError_t launch_source_emit(fsl_ctx *c, const SourceBox &s, int64_t ncell, const uint32_t *off,
+                              int64_t base) {
+  k_source_emit<<<blocks_for(ncell, kThreads), kThreads, 0, c->stream>>>(c->dom, s, ncell, off,
+                                                                          c->P, base);
   return hipGetLastError();
 }
 

@@ -67,6 +67,8 @@ SIGNATURES = {
     "fsl_add_solid_box": [_P, _IP, _IP],
     "fsl_set_particles": [_P, _I64, _P, _P, _P],
     "fsl_seed_particles": [_P, _P, _I, C.c_uint32, _FP],
+    "fsl_add_source": [_P, _FP, _FP, _FP, _I, C.c_uint32, C.c_uint32],
+    "fsl_delete_particles_in_boxes": [_P, _I, _FP],
     "fsl_num_particles": [_P, C.POINTER(_I64)],
     "fsl_get_particles": [_P, _P, _P, _P],
     "fsl_cfl": [_P, _FP],
@@ -211,6 +213,19 @@ class Context:
         m = np.ascontiguousarray(mask, dtype=np.uint8)
         assert m.shape == self.dims
         self.call("fsl_seed_particles", _ptr(m), int(ppc), int(seed), (C.c_float * 3)(*velocity))
+
+    def add_source(self, lo, hi, velocity, ppc, seed, stream=0):
+        """One evaluated source box (end already reduced by 1, as
+        Source::get_box returns it); appends its particles."""
+        f3 = C.c_float * 3
+        self.call("fsl_add_source", f3(*lo), f3(*hi), f3(*velocity), int(ppc), int(seed),
+                  int(stream))
+
+    def delete_particles_in_boxes(self, boxes):
+        """boxes: (n, 6) array of lo x, y, z, hi x, y, z (bounds inclusive)."""
+        b = np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 6)
+        self.call("fsl_delete_particles_in_boxes", len(b),
+                  b.ctypes.data_as(C.POINTER(C.c_float)))
 
     def num_particles(self) -> int:
         n = C.c_int64()

@@ -122,6 +122,20 @@ int fsl_set_particles(fsl_ctx *ctx, int64_t n, const float *pos, const float *ve
  * uint8 over [0, n) x-major; velocity and grid velocity are `velocity`. */
 int fsl_seed_particles(fsl_ctx *ctx, const uint8_t *mask, int ppc, uint32_t seed,
                        const float velocity[3]);
+/* AddContributionFromSources (flip_simulation.cc:1206-1266) for one source
+ * box already evaluated at the step's time (Source::get_box, sources.cc:64-82,
+ * i.e. with its end reduced by 1): cells from (int)lo to (int)hi, clipped to
+ * the domain, each get up to ppc particles at Philox points (key seed, stream)
+ * inside the box (bounds inclusive), out of at most 32 attempts, with
+ * velocity and grid velocity `velocity`. Particles are appended and join the
+ * cell order at the next move. FSL_ERR_CAPACITY if they do not fit. */
+int fsl_add_source(fsl_ctx *ctx, const float lo[3], const float hi[3], const float velocity[3],
+                   int ppc, uint32_t seed, uint32_t stream);
+/* DeleteFluidInSources / Particles::DeleteParticlesFromRegions
+ * (flip_simulation.cc:1268-1277, particles.cc:222-271): deletes the particles
+ * whose position lies in any of the n boxes (boxes[6 b .. 6 b + 5] = lo x, y,
+ * z, hi x, y, z; bounds inclusive), keeping the others' order. */
+int fsl_delete_particles_in_boxes(fsl_ctx *ctx, int n, const float *boxes);
 /* Particles::size (particles.h). */
 int fsl_num_particles(fsl_ctx *ctx, int64_t *n);
 /* Particle data in device order (sorted by cell). Any pointer may be NULL. */

@@ -71,6 +71,82 @@ struct Cube {
 
 enum PreconditionerType { NO_PRECONDITIONER = 0, JACOBI = 1, INCOMPLETE_CHOLESKY = 2 };
 
+/* application::Source (common/sources.h:11-76, sources.cc): a water source,
+ * static, timed or oscillating. Behaviour kept as the reference has it:
+ *  - get_box returns box1 with its end reduced by 1 whether or not the box
+ *    oscillates (the oscillating branch of sources.cc:64-82 fills a local
+ *    that shadows the returned box);
+ *  - an oscillating velocity uses the box period (sources.cc:88).
+ * Boxes are in normalised coordinates until ScaleBox. */
+class Source {
+ public:
+  Source() = default;
+  Source(const Cube &box, const Vec3 &velocity) : box1_(box), velocity1_(velocity) {}
+
+  void set_box(const Cube &box) {
+    box1_ = box;
+    box_oscillating_ = false;
+  }
+  void set_velocity(const Vec3 &velocity) {
+    velocity1_ = velocity;
+    velocity_oscillating_ = false;
+  }
+  void set_box(const Cube &box1, const Cube &box2, float period) {
+    box1_ = box1;
+    box2_ = box2;
+    box_period_ = period;
+    box_oscillating_ = true;
+  }
+  void set_velocity(const Vec3 &velocity1, const Vec3 &velocity2, float period) {
+    velocity1_ = velocity1;
+    velocity2_ = velocity2;
+    velocity_period_ = period;
+    velocity_oscillating_ = true;
+  }
+  void set_timed(float start, float end) {
+    start_time_ = start;
+    end_time_ = end;
+    timed_ = true;
+  }
+
+  Cube get_box(float t = 0) const {
+    (void)t;
+    Cube box = box1_;
+    box.end = Vec3(box.end.x - 1, box.end.y - 1, box.end.z - 1);
+    return box;
+  }
+  Vec3 get_velocity(float t = 0) const {
+    if (!velocity_oscillating_) return velocity1_;
+    const float w = float(M_PI) * t / box_period_;
+    const float c = std::cos(w) + 1;
+    return Vec3(velocity1_.x + 0.5f * (velocity2_.x - velocity1_.x) * c,
+                velocity1_.y + 0.5f * (velocity2_.y - velocity1_.y) * c,
+                velocity1_.z + 0.5f * (velocity2_.z - velocity1_.z) * c);
+  }
+  bool is_active(float t = 0) const {
+    return !timed_ || (t >= start_time_ && t <= end_time_);
+  }
+
+  /* Cube::Scale (primitives.cc:76-83) of both boxes. */
+  void ScaleBox(const Coord scale) {
+    for (Cube *b : {&box1_, &box2_}) {
+      b->start = Vec3(b->start.x * scale.x, b->start.y * scale.y, b->start.z * scale.z);
+      b->end = Vec3(b->end.x * scale.x, b->end.y * scale.y, b->end.z * scale.z);
+    }
+  }
+  void ScaleVelocity(float scale) {
+    velocity1_ = Vec3(velocity1_.x * scale, velocity1_.y * scale, velocity1_.z * scale);
+    velocity2_ = Vec3(velocity2_.x * scale, velocity2_.y * scale, velocity2_.z * scale);
+  }
+
+ private:
+  Cube box1_, box2_;
+  Vec3 velocity1_, velocity2_;
+  float box_period_ = 0, velocity_period_ = 0;
+  float start_time_ = 0, end_time_ = 0;
+  bool box_oscillating_ = false, velocity_oscillating_ = false, timed_ = false;
+};
+
 class FlipSimulation {
  public:
   FlipSimulation() = default;
@@ -154,6 +230,35 @@ class FlipSimulation {
     const int lo[3] = {box.lo.x, box.lo.y, box.lo.z}, hi[3] = {box.hi.x, box.hi.y, box.hi.z};
     check(fsl_add_solid_box(ctx_, lo, hi));
   }
+
+  /* AddWaterSource (flip_simulation.h): a source, already scaled
+   * (flip_app_sim.cc:98-104). */
+  void AddWaterSource(const Source &source) { sources_.push_back(source); }
+  /* AddContributionFromSources (flip_simulation.cc:1206-1266): particles of
+   * every source active at time t. Each call draws a new Philox stream. */
+  void AddContributionFromSources(float t) {
+    for (const Source &src : sources_) {
+      if (!src.is_active(t)) continue;
+      const Cube box = src.get_box(t);
+      const Vec3 v = src.get_velocity(t);
+      const float lo[3] = {box.start.x, box.start.y, box.start.z};
+      const float hi[3] = {box.end.x, box.end.y, box.end.z};
+      const float vel[3] = {v.x, v.y, v.z};
+      check(fsl_add_source(ctx_, lo, hi, vel, ppc_, seed_, source_stream_++));
+    }
+  }
+  /* DeleteFluidInSources (flip_simulation.cc:1268-1277): the particles in
+   * every source's box (active or not). */
+  void DeleteFluidInSources(float t) {
+    std::vector<float> boxes;
+    for (const Source &src : sources_) {
+      const Cube b = src.get_box(t);
+      boxes.insert(boxes.end(), {b.start.x, b.start.y, b.start.z, b.end.x, b.end.y, b.end.z});
+    }
+    if (!boxes.empty())
+      check(fsl_delete_particles_in_boxes(ctx_, int(boxes.size() / 6), boxes.data()));
+  }
+  bool HasSources() const { return !sources_.empty(); }
 
   void ClearGridData() { check(fsl_clear(ctx_)); }
   size_t FluidCellCount() {
@@ -353,6 +458,8 @@ class FlipSimulation {
   float gravity_ = 9.8f, flip_ = 1.0f;
   Vec3 force_{0, 1, 0};
   uint32_t seed_ = 1;
+  uint32_t source_stream_ = 0;  // Philox key word of the next source call
+  std::vector<Source> sources_;
   std::vector<uint8_t> mask_;
 };
 
@@ -425,11 +532,14 @@ class FlipDriver {
     return dt;
   }
 
-  /* RunOneStep (flip_app_sim.cc:211-254), sources and level-set stages
-   * excluded (not part of this core). */
+  /* RunOneStep (flip_app_sim.cc:211-254); the level-set stages (phi) are
+   * not part of this core. Sources use the time at the start of the step
+   * (advanced time, flip_app_sim.cc:438-462). */
   void RunOneStep(float dt) {
+    if (sim_->HasSources()) sim_->AddContributionFromSources(advanced_time_);
     sim_->MoveParticles(dt);
     sim_->DeleteParticlesInSolids();
+    if (sim_->HasSources()) sim_->DeleteFluidInSources(advanced_time_);
     sim_->ClearGridData();
     sim_->TransferParticlesToGrid();
     sim_->ApplyBoundaryConditions();
@@ -442,6 +552,7 @@ class FlipDriver {
     sim_->UpdateVelocityFromPressure();
     sim_->GetVelocityUpdate();
     sim_->UpdateParticlesFromGrid();
+    advanced_time_ += dt;
     ++steps_;
   }
 
@@ -468,6 +579,7 @@ class FlipDriver {
   float frame_step_ = 1.0f / 30;
   float step_ = 0;
   float time_ = 0;
+  float advanced_time_ = 0;  // sum of the steps' dt (AdvanceTime)
   int steps_ = 0, frame_ = 0;
 };
 

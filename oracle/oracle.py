@@ -217,3 +217,73 @@ class Oracle:
         out = np.empty(self.box, np.int32)
         self.lib.or_cell_counts(self.h, _ptr(out))
         return out
+
+
+# ---------------------------------------------------------------- sources
+# Numpy restatements (no C part): the water sources of the reference with
+# Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11; 10 rounds, Weyl key
+# schedule) in place of glibc random().
+
+_PH_M0, _PH_M1 = 0xD2511F53, 0xCD9E8D57
+_PH_W0, _PH_W1 = 0x9E3779B9, 0xBB67AE85
+_U32 = 0xFFFFFFFF
+
+
+def philox(c0, c1, c2, c3, k0, k1):
+    """Philox4x32-10 of uint32 counter arrays under the key (k0, k1)."""
+    c = [np.asarray(x, np.uint64) & _U32 for x in (c0, c1, c2, c3)]
+    c = np.broadcast_arrays(*c)
+    c = [x.copy() for x in c]
+    k0, k1 = int(k0) & _U32, int(k1) & _U32
+    for r in range(10):
+        if r:
+            k0 = (k0 + _PH_W0) & _U32
+            k1 = (k1 + _PH_W1) & _U32
+        p0 = np.uint64(_PH_M0) * c[0]
+        p1 = np.uint64(_PH_M1) * c[2]
+        hi0, lo0 = p0 >> np.uint64(32), p0 & np.uint64(_U32)
+        hi1, lo1 = p1 >> np.uint64(32), p1 & np.uint64(_U32)
+        c = [hi1 ^ c[1] ^ np.uint64(k0), lo1, hi0 ^ c[3] ^ np.uint64(k1), lo0]
+    return [x.astype(np.uint32) for x in c]
+
+
+def source_particles(dims, lo, hi, velocity, ppc, seed, stream):
+    """AddContributionFromSources (flip_simulation.cc:1206-1266), particles
+    only, for one box as Source::get_box returns it (end already reduced by
+    1). Cells from int(lo) to int(hi) (truncation, as the Coord conversion),
+    clipped to the domain, in x-major order; each makes up to 32 attempts,
+    attempt a giving the point cell + u with u = (x >> 8) * 2^-24 from
+    philox((lin, a, 2, 0), (seed, stream)), lin the cell's x-major index;
+    points inside the box (bounds inclusive, Cube::IsInside,
+    primitives.cc:57-64) are kept, the first ppc of them. Returns positions
+    (n, 3) float32 in emission order and velocities (n, 3)."""
+    lo = np.asarray(lo, np.float32)
+    hi = np.asarray(hi, np.float32)
+    c0 = [max(int(lo[e]), 0) for e in range(3)]
+    c1 = [min(int(hi[e]), dims[e] - 1) for e in range(3)]
+    if any(c1[e] < c0[e] for e in range(3)):
+        return np.zeros((0, 3), np.float32), np.zeros((0, 3), np.float32)
+    ax = [np.arange(c0[e], c1[e] + 1) for e in range(3)]
+    cells = np.stack(np.meshgrid(*ax, indexing="ij"), -1).reshape(-1, 3)
+    lin = (cells[:, 0].astype(np.int64) * dims[1] + cells[:, 1]) * dims[2] + cells[:, 2]
+    att = np.arange(32, dtype=np.int64)
+    x = philox(lin[:, None], att[None, :], 2, 0, seed, stream)
+    pos = np.empty(cells.shape[:1] + (32, 3), np.float32)
+    for e in range(3):
+        u = (x[e] >> np.uint32(8)).astype(np.float32) * np.float32(1.0 / 16777216.0)
+        pos[..., e] = cells[:, e, None].astype(np.float32) + u
+    inside = np.all((pos >= lo) & (pos <= hi), axis=-1)
+    keep = inside & (np.cumsum(inside, axis=1) <= ppc)
+    out = pos[keep]
+    vel = np.tile(np.asarray(velocity, np.float32), (out.shape[0], 1))
+    return out, vel
+
+
+def inside_boxes(pos, boxes):
+    """Mask of the particles that DeleteParticlesFromRegions (particles.cc:
+    222-271) removes: position inside any box (bounds inclusive)."""
+    pos = np.asarray(pos, np.float32)
+    m = np.zeros(pos.shape[0], bool)
+    for b in np.asarray(boxes, np.float32).reshape(-1, 6):
+        m |= np.all((pos >= b[:3]) & (pos <= b[3:]), axis=1)
+    return m

@@ -380,3 +380,76 @@ def test_sealed_pocket_and_enclosed_cell_errors():
     o2.divergence()
     rc, _, _ = o2.solve()
     assert rc == -2
+
+
+# ---- water sources (oracle/oracle.py's numpy restatement)
+from oracle import oracle as _orc  # noqa: E402
+
+
+def test_oracle_philox_known_answers():
+    f = _orc.philox
+    assert [int(x[0]) for x in f([0], [0], [0], [0], 0, 0)] == [
+        0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]
+    m = 0xFFFFFFFF
+    assert [int(x[0]) for x in f([m], [m], [m], [m], m, m)] == [
+        0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]
+    assert [int(x[0]) for x in f([0x243F6A88], [0x85A308D3], [0x13198A2E], [0x03707344],
+                                 0xA4093822, 0x299F31D0)] == [
+        0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]
+
+
+def _source_scalar(dims, lo, hi, ppc, seed, stream):
+    """AddContributionFromSources' loops (flip_simulation.cc:1222-1262) one
+    cell and one attempt at a time."""
+    lo = [np.float32(v) for v in lo]
+    hi = [np.float32(v) for v in hi]
+    out = []
+    for i in range(max(int(lo[0]), 0), min(int(hi[0]), dims[0] - 1) + 1):
+        for j in range(max(int(lo[1]), 0), min(int(hi[1]), dims[1] - 1) + 1):
+            for k in range(max(int(lo[2]), 0), min(int(hi[2]), dims[2] - 1) + 1):
+                lin = (i * dims[1] + j) * dims[2] + k
+                kept = 0
+                for a in range(32):
+                    if kept >= ppc:
+                        break
+                    x = [int(v[0]) for v in _orc.philox([lin], [a], [2], [0], seed, stream)]
+                    p = [np.float32(c) + np.float32((x[e] >> 8) / 16777216.0)
+                         for e, c in enumerate((i, j, k))]
+                    if all(lo[e] <= p[e] <= hi[e] for e in range(3)):
+                        out.append(p)
+                        kept += 1
+    return np.array(out, np.float32).reshape(-1, 3)
+
+
+def test_source_particles_match_scalar_loops():
+    dims = (12, 10, 9)
+    lo, hi = (2.3, 0.0, 6.5), (5.7, 2.25, 9.8)  # clipped in z, partial cells
+    pos, vel = _orc.source_particles(dims, lo, hi, (0.5, -1.0, 0.0), 6, 11, 3)
+    ref = _source_scalar(dims, lo, hi, 6, 11, 3)
+    assert pos.shape[0] > 0
+    assert np.array_equal(pos, ref)
+    assert np.array_equal(vel, np.tile(np.float32([0.5, -1.0, 0.0]), (pos.shape[0], 1)))
+
+
+def test_source_particles_closed_form_counts():
+    # The box as Source::get_box returns it for a scaled cube [0, 5]^3: end
+    # reduced to 4. Cells 0..3 lie inside (every attempt kept: 8 each); a
+    # point of a cell at index 4 is inside only at u = 0 on each axis.
+    dims = (16, 16, 16)
+    pos, _ = _orc.source_particles(dims, (0, 0, 0), (4, 4, 4), (0, 0, 0), 8, 7, 0)
+    assert pos.shape[0] == 4 ** 3 * 8
+    cells = np.floor(pos).astype(int)
+    assert cells.min() == 0 and cells.max() == 3
+    _, counts = np.unique(cells, axis=0, return_counts=True)
+    assert (counts == 8).all() and counts.size == 64
+    # Another stream gives other points in the same cells.
+    pos2, _ = _orc.source_particles(dims, (0, 0, 0), (4, 4, 4), (0, 0, 0), 8, 7, 1)
+    assert pos2.shape == pos.shape and not np.array_equal(pos, pos2)
+
+
+def test_inside_boxes_bounds_inclusive():
+    pos = np.float32([[1, 1, 1], [2, 2, 2], [2.0001, 1, 1], [0.5, 0.5, 0.5]])
+    m = _orc.inside_boxes(pos, [[1, 1, 1, 2, 2, 2]])
+    assert m.tolist() == [True, True, False, False]
+    m2 = _orc.inside_boxes(pos, [[1, 1, 1, 2, 2, 2], [0, 0, 0, 0.5, 0.5, 0.5]])
+    assert m2.tolist() == [True, True, False, True]
