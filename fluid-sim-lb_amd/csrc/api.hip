@@ -507,12 +507,13 @@ int fsl_divergence(fsl_ctx *ctx) {
   FSL_HIP(hipMemsetAsync(ctx->flags, 0, sizeof(fsl::DevFlags), ctx->stream));
   FSL_HIP(fsl::launch_divergence(ctx));
   FSL_HIP(fsl::launch_compact(ctx, ctx->lfluid, ctx->fluid_list));
-  uint32_t nf = 0;
-  rc = read_u32(ctx, ctx->lscan + ctx->nleaves, &nf);
-  if (rc) return rc;
-  ctx->n_fluid = nf;
-  rc = sync_flags(ctx);
-  if (rc) return rc;
+  // The fluid-leaf count and the flags in one round trip.
+  FSL_HIP(hipMemcpyAsync(ctx->host_u32, ctx->lscan + ctx->nleaves, sizeof(uint32_t),
+                         hipMemcpyDeviceToHost, ctx->stream));
+  FSL_HIP(hipMemcpyAsync(ctx->flags_host, ctx->flags, sizeof(fsl::DevFlags),
+                         hipMemcpyDeviceToHost, ctx->stream));
+  FSL_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->n_fluid = ctx->host_u32[0];
   ctx->pcg_ready = false;
   return FSL_OK;
 }
@@ -603,10 +604,19 @@ int fsl_step(fsl_ctx *ctx, float dt, const fsl_solve_params *params, fsl_solve_s
         if ((rc = fsl_apply_bc(ctx))) return rc)
   STAGE(FSL_T_DIVERGENCE, if ((rc = fsl_divergence(ctx))) return rc)
   STAGE(FSL_T_SOLVE, if ((rc = fsl_pcg_solve(ctx, params, stats))) return rc)
+  // The velocity update's consistency flag is read once, after G2P, so the
+  // update, the extrapolation and G2P are enqueued without a host round trip.
   STAGE(FSL_T_UPDATE, if ((rc = fsl_pressure_update(ctx))) return rc;
-        if ((rc = fsl_velocity_update(ctx))) return rc)
-  STAGE(FSL_T_G2P, if ((rc = fsl_g2p(ctx))) return rc)
+        FSL_HIP(hipMemsetAsync(&ctx->flags->vel_update, 0, sizeof(unsigned), ctx->stream));
+        FSL_HIP(fsl::launch_velocity_update(ctx)))
+  STAGE(FSL_T_G2P, FSL_HIP(fsl::launch_extrapolate(ctx)); FSL_HIP(fsl::launch_g2p(ctx)))
 #undef STAGE
+  uint32_t bad = 0;
+  if ((rc = read_u32(ctx, &ctx->flags->vel_update, &bad))) return rc;
+  if (bad) {
+    ctx->err = "face active in grid_velocity_ but not in grid_velocity_update_";
+    return FSL_ERR_VELOCITY_UPDATE;
+  }
   if (ctx->timing) {
     FSL_HIP(hipStreamSynchronize(ctx->stream));
     for (int s = FSL_T_MOVE; s <= FSL_T_G2P; ++s) {

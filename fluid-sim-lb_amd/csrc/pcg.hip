@@ -1744,8 +1744,8 @@ static int run_pipelined(fsl_ctx *ctx, int &k, int batch, bool adaptive, Issue &
     if (h.done || h.status) break;
     if (!adaptive) continue;
     // r.r of CG is not monotone: the decay rate comes from the running
-    // minimum.
-    const double rr_min = std::min(h.rr, rr_prev);
+    // minimum (rr_prev = 0: no earlier value read yet).
+    const double rr_min = rr_prev > 0.0 ? std::min(h.rr, rr_prev) : h.rr;
     if (h.it > it_prev && rr_min > 0.0 && rr_prev > 0.0) {
       const double rate = std::log(rr_min / rr_prev) / double(h.it - it_prev);
       const double target = h.stop_mode == FSL_STOP_REL ? h.thr * h.thr : h.thr;
@@ -1769,10 +1769,10 @@ static int pcg_run_sr(fsl_ctx *ctx, const fsl_solve_params *prm, fsl_solve_stats
              [&](auto K) { decltype(K)::sr_prepare(ctx, prm->warm_start ? 1 : 0); });
     FSL_HIP(hipGetLastError());
   }
-  rc = sync_state(ctx);
-  if (rc) return rc;
+  // The batches start without reading the prepared state back: a prepare
+  // that stops (converged at x0, or an error status) sets the done flag, on
+  // which every launch exits, and the state is read after the first batch.
   ctx->pcg_ready = true;
-  if (ctx->pcg_host->status) return ctx->pcg_host->status;
   const bool adaptive = prm->batch <= 0;
   const int batch = prm->batch > 0 ? prm->batch : 16;
   const bool mixed = ctx->pcg_precision == FSL_PCG_MIXED;
@@ -1843,7 +1843,7 @@ static int pcg_run_sr(fsl_ctx *ctx, const fsl_solve_params *prm, fsl_solve_stats
       // r.r of CG is not monotone: the decay rate is taken from the running
       // minimum, and a batch covers at most 60% of the predicted remainder
       // (launches past convergence cost about 2 us each).
-      const double rr_min = std::min(h->rr, rr_prev);
+      const double rr_min = rr_prev > 0.0 ? std::min(h->rr, rr_prev) : h->rr;
       if (adaptive && h->it > it_prev && rr_min > 0.0 && rr_prev > 0.0) {
         const double rate = std::log(rr_min / rr_prev) / double(h->it - it_prev);
         const double target = h->stop_mode == FSL_STOP_REL ? h->thr * h->thr : h->thr;
