@@ -107,23 +107,31 @@ __global__ __launch_bounds__(512) void k_divergence(Dom d, const int32_t *list, 
   int64_t v;
   int c[3];
   if (!leaf_voxel(d, list, nlist, v, c)) return;
+  // Every load is issued whatever the marker turns out to be, so none waits
+  // for it: cells outside the box read their own voxel in place of the
+  // neighbours (in the box, the neighbours lie in [-1, n], inside storage).
+  const bool inb = in_bbox(d, c[0], c[1], c[2]);
+  const int off[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+  int64_t nv[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q)
+    nv[q] = inb ? vox_of(d, c[0] + off[q][0], c[1] + off[q][1], c[2] + off[q][2]) : v;
   const uint8_t m = marker[v];
-  const bool fluid = m == kFluid && in_bbox(d, c[0], c[1], c[2]);
+  uint8_t nm[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) nm[q] = marker[nv[q]];
+  const float ux = vx[v], uxp = vx[nv[1]], uy = vy[v], uyp = vy[nv[3]], uz = vz[v],
+              uzp = vz[nv[5]];
+  const bool fluid = m == kFluid && inb;
   float dv = 0.0f;
   uint8_t l = m;
   if (fluid) {
-    const int64_t xp = vox_of(d, c[0] + 1, c[1], c[2]), yp = vox_of(d, c[0], c[1] + 1, c[2]),
-                  zp = vox_of(d, c[0], c[1], c[2] + 1);
-    dv = vx[xp] - vx[v];
-    dv = dv + (vy[yp] - vy[v]);
-    dv = dv + (vz[zp] - vz[v]);
+    dv = uxp - ux;
+    dv = dv + (uyp - uy);
+    dv = dv + (uzp - uz);
     int diag = 0;
-    const int off[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
 #pragma unroll
-    for (int q = 0; q < 6; ++q) {
-      const uint8_t nm = marker[vox_of(d, c[0] + off[q][0], c[1] + off[q][1], c[2] + off[q][2])];
-      diag += (nm == kFluid || nm == kAir) ? 1 : 0;
-    }
+    for (int q = 0; q < 6; ++q) diag += (nm[q] == kFluid || nm[q] == kAir) ? 1 : 0;
     if (diag == 0) atomicOr(&flags->zero_diag, 1u);
     l = uint8_t(kFluid | (diag << 2));
   }
@@ -234,27 +242,38 @@ __global__ void k_clear_shell(Dom d, uint8_t *vflag) {
 
 // CFL (flip_simulation.cc:459-488): max |v| over active faces inside the
 // partition box, skipping samples past the last face of each component.
+// v_inf of CFL (flip_simulation.cc:459-488): a grid-stride walk over the
+// active leaves (at most kCflBlocks workgroups, so one atomic each); the
+// activity byte and the three components are loaded together.
+constexpr int kCflBlocks = 1024;
 __global__ __launch_bounds__(512) void k_cfl(Dom d, const int32_t *list, int64_t nlist,
                                              const uint8_t *vflag, const float *vx,
                                              const float *vy, const float *vz, unsigned *bits) {
   __shared__ float s_m[8];
-  int64_t v;
-  int c[3];
-  if (!leaf_voxel(d, list, nlist, v, c)) return;
+  const int t = threadIdx.x;
+  const int e0 = d.n[0] - 1, e1 = d.n[1] - 1, e2 = d.n[2] - 1;
   float m = 0.0f;
-  if (in_bbox(d, c[0], c[1], c[2]) && (vflag[v] & 1)) {
-    const int e0 = d.n[0] - 1, e1 = d.n[1] - 1, e2 = d.n[2] - 1;
-    if (c[1] != e1 && c[2] != e2) m = fmaxf(m, fabsf(vx[v]));
-    if (c[0] != e0 && c[2] != e2) m = fmaxf(m, fabsf(vy[v]));
-    if (c[0] != e0 && c[1] != e1) m = fmaxf(m, fabsf(vz[v]));
+  for (int64_t w = blockIdx.x; w < nlist; w += gridDim.x) {
+    const int64_t leaf = list[w];
+    int o[3];
+    leaf_origin(d, leaf, o);
+    const int c0 = o[0] + (t >> 6), c1 = o[1] + ((t >> 3) & 7), c2 = o[2] + (t & 7);
+    const int64_t v = (leaf << 9) | t;
+    const uint8_t f = vflag[v];
+    const float ax = fabsf(vx[v]), ay = fabsf(vy[v]), az = fabsf(vz[v]);
+    if (in_bbox(d, c0, c1, c2) && (f & 1)) {
+      if (c1 != e1 && c2 != e2) m = fmaxf(m, ax);
+      if (c0 != e0 && c2 != e2) m = fmaxf(m, ay);
+      if (c0 != e0 && c1 != e1) m = fmaxf(m, az);
+    }
   }
   m = wave_max(m);
-  if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = m;
+  if ((t & 63) == 0) s_m[t >> 6] = m;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    float t = 0.0f;
-    for (int q = 0; q < 8; ++q) t = fmaxf(t, s_m[q]);
-    atomicMax(bits, __float_as_uint(t));  // non-negative floats order as uints
+  if (t == 0) {
+    float x = 0.0f;
+    for (int q = 0; q < 8; ++q) x = fmaxf(x, s_m[q]);
+    atomicMax(bits, __float_as_uint(x));  // non-negative floats order as uints
   }
 }
 
@@ -379,8 +398,9 @@ hipError_t launch_clear_shell(fsl_ctx *c) {
 }
 hipError_t launch_cfl(fsl_ctx *c) {
   if (c->n_act == 0) return hipSuccess;
-  k_cfl<<<xcd_grid(c->n_act), 512, 0, c->stream>>>(c->dom, c->act_list, c->n_act, c->vflag,
-                                                    c->v[0], c->v[1], c->v[2], &c->flags->cfl_bits);
+  const int g = int(std::min<int64_t>(c->n_act, kCflBlocks));
+  k_cfl<<<g, 512, 0, c->stream>>>(c->dom, c->act_list, c->n_act, c->vflag, c->v[0], c->v[1],
+                                  c->v[2], &c->flags->cfl_bits);
   return hipGetLastError();
 }
 hipError_t launch_box_vec(fsl_ctx *c, int which, float *out3, uint8_t *act, bool to_box) {
