@@ -7,6 +7,7 @@
 //   partition_x=1 partition_y=1 partition_z=1   seed=1   ppc=8
 //   gravity=9.8 flip=0.95 frame_rate=30 cfl=1
 //   solver_max_iterations=1500 solver_threshold=5e-9 (times n^3)
+//   preconditioner=jacobi|ic   (ic: block IC(0) per partition)
 //   output=<prefix>   writes <prefix>NNNNNNNN binary particle frames
 //   grid_output=<prefix>  also writes text grid dumps <prefix>{velocity,
 //                     pressure,marker}_NNNNNNNN each frame
@@ -247,7 +248,10 @@ int main(int argc, char **argv) {
     const float threshold =
         float(double(n) * n * n) * std::strtof(get(kv, "solver_threshold", "5e-9").c_str(), nullptr);
     fsl::PCGDriver solver("pressure", std::atoi(get(kv, "solver_max_iterations", "1500").c_str()),
-                          threshold, false, fsl::JACOBI, parts, dims);
+                          threshold, false, fsl::IC, parts, dims);
+    // The reference passes IC and runs Jacobi (flip_app.cc:329-332);
+    // preconditioner=ic runs block IC(0).
+    if (get(kv, "preconditioner", "jacobi") == "ic") solver.set_preconditioner(fsl::IC);
     fsl::FlipDriver::Params prm;
     prm.frames = std::atoi(get(kv, "frames", "1").c_str());
     prm.frame_rate = std::atoi(get(kv, "frame_rate", "30").c_str());

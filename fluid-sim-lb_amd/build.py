@@ -27,7 +27,7 @@ FLAGS = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-result",
 ]
 
-SOURCES = ["scan.hip", "particles.hip", "grid.hip", "pcg.hip", "adjoint.hip", "api.hip"]
+SOURCES = ["scan.hip", "particles.hip", "grid.hip", "pcg.hip", "ic.hip", "adjoint.hip", "api.hip"]
 # The water application: fsl.hpp's host classes driving libfsl.so.
 APP_SRC = os.path.join(HERE, "app", "water_app.cpp")
 APP = os.path.join(HERE, "fsl_water_app")

@@ -188,6 +188,7 @@ int fsl_destroy(fsl_ctx *ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   fsl::adjoint_free(ctx);
+  fsl::ic_free(ctx);
   void *bufs[] = {ctx->v[0], ctx->v[1], ctx->v[2], ctx->u[0], ctx->u[1], ctx->u[2], ctx->vflag,
                   ctx->marker, ctx->solid, ctx->lab, ctx->div, ctx->pres, ctx->x, ctx->r, ctx->z,
                   ctx->p[0], ctx->p[1], ctx->Ap, ctx->x64, ctx->partials, ctx->pcg, ctx->lhas,

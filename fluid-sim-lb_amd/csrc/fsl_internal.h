@@ -16,6 +16,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 #include "fsl.h"
 
@@ -71,7 +72,8 @@ struct PcgState {
   double rzh[2];  // rz of iteration k in slot k & 1
   double alph[2]; // alpha of iteration k in slot k & 1 (single-reduction CG)
   int it, done, status, max_iter;
-  int stop_mode, precision, refresh_due, pad;
+  int stop_mode, precision, refresh_due;
+  int pc_fail;    // the IC(0) build met a pivot below 1e-5
   unsigned cnt_a, cnt_b, cnt_r, cnt_x;
 };
 
@@ -129,6 +131,13 @@ struct fsl_ctx {
   int pcg_precision = FSL_PCG_FP32, pcg_stop = FSL_STOP_ABS_RR, pcg_refresh = 32;
   int pcg_pbuf = 0;            // which p buffer holds the current direction
   bool pcg_ready = false;
+  // Block IC(0) preconditioner (allocated by the first IC solve): L's
+  // diagonal d and -1/d per voxel (sized for doubles), the FLUID leaves
+  // sorted by tile-local level, and the level offsets into that list.
+  bool pcg_ic = false;
+  void *ic_d = nullptr, *ic_o = nullptr;
+  int32_t *ic_leaves = nullptr;
+  std::vector<int> ic_off;
   // Leaves.
   uint32_t *lhas = nullptr;   // leaf holds particles
   uint32_t *lact = nullptr;   // leaf active (26-dilation of lhas)
@@ -215,6 +224,13 @@ int pcg_run(fsl_ctx *c, const fsl_solve_params *prm, fsl_solve_stats *st);
 int pcg_iterate_once(fsl_ctx *c);
 int pcg_finish(fsl_ctx *c, fsl_solve_stats *st);
 int pcg_true_residual(fsl_ctx *c, double *rr_true);
+
+// ic.hip: block IC(0). ic_factor sorts the FLUID leaves by level and factors
+// (once per solve, after the labels); ic_sweeps enqueues z = M^-1 r (y in Ap);
+// force: run even when the solve is done (a forced refresh).
+int ic_factor(fsl_ctx *ctx);
+void ic_sweeps(fsl_ctx *c, int force);
+void ic_free(fsl_ctx *c);
 
 // adjoint.hip
 int project_backward(fsl_ctx *c, const fsl_solve_params *prm, const float *grad_out,

@@ -377,7 +377,7 @@ __global__ __launch_bounds__(1024) void k_prepare_scalars(const A *part_b, const
   st->bnorm2 = bb;
   if (st->stop_mode == FSL_STOP_REL) st->thr = st->thr * sqrt(bb);
   st->it = 0;
-  st->status = isnan(rr) ? FSL_ERR_NAN : 0;
+  st->status = st->pc_fail ? FSL_ERR_PRECONDITIONER : (isnan(rr) ? FSL_ERR_NAN : 0);
   st->done = (st->status != 0 || converged(st, rr) || st->max_iter <= 0) ? 1 : 0;
 }
 
@@ -578,6 +578,30 @@ __global__ __launch_bounds__(kT) void k_pass_b(const int32_t *flist, int64_t nf,
       r[v] = rn;
       const V z = rn * dinv<V>(c.l[s] >> 2);
       zv[v] = z;
+      acc[0] += A(rn) * A(rn);
+      acc[1] += A(rn) * A(z);
+    }
+  }
+  publish<A, 2>(acc, part_b);
+}
+
+// Block IC(0) (ic.hip): after the sweeps wrote z = M^-1 r, (r.r, r.z) are
+// published as B(k)'s partials, in B's grid and summation form.
+template <typename V, typename A>
+__global__ __launch_bounds__(kT) void k_ic_dots(const int32_t *flist, int64_t nf,
+                                                const uint8_t *lab, const V *r, const V *zv,
+                                                A *part_b, const PcgState *st, int force) {
+#pragma clang fp contract(off)
+  if ((!force && st->done) || st->status) return;
+  A acc[2] = {A(0), A(0)};
+  for (Walk q = walk(nf); q.more(); q.next()) {
+    if (!q.valid()) continue;
+    const int64_t base = int64_t(flist[q.mine()]) << 9;
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int64_t v = base | (gtid() + s * kG);
+      if (!is_fluid(lab[v])) continue;
+      const V rn = r[v], z = zv[v];
       acc[0] += A(rn) * A(rn);
       acc[1] += A(rn) * A(z);
     }
@@ -1505,7 +1529,21 @@ struct Kernels {
     k_prepare<V, A, ORD, MIXED><<<g, kT, 0, c->stream>>>(
         c->dom, c->fluid_list, c->n_fluid, c->lab, c->div, c->pres, warm, (V *)c->x, (V *)c->r,
         (V *)c->z, c->x64, (A *)P.b[1], (A *)P.bb);
+    if (c->pcg_ic) ic_apply(c, P.b[1], 0);
     k_prepare_scalars<A><<<1, 1024, 0, c->stream>>>((const A *)P.b[1], (const A *)P.bb, g, c->pcg);
+  }
+  // Block IC(0) in place of B's Jacobi z: z = M^-1 r, then (r.r, r.z) into
+  // `part` (B's partials of the iteration).
+  static void ic_apply(fsl_ctx *c, void *part, int force) {
+    ic_sweeps(c, force);
+    k_ic_dots<V, A><<<grid_for(c->n_fluid), kT, 0, c->stream>>>(
+        c->fluid_list, c->n_fluid, c->lab, (const V *)c->r, (const V *)c->z, (A *)part, c->pcg,
+        force);
+  }
+  // B(k) and, with IC, the preconditioner apply that completes it.
+  static void pass_b_pc(fsl_ctx *c, int k, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+    pass_b(c, k, e0, e1);
+    if (c->pcg_ic) ic_apply(c, parts(c).b[k & 1], 0);
   }
   // With events, hipExtLaunchKernelGGL records the dispatch's own start and
   // end (the kernel duration, as a profiler reports it).
@@ -1591,6 +1629,7 @@ void launch_refresh(fsl_ctx *c, int k, int force) {
   k_refresh_r<<<g, kT, 0, c->stream>>>(c->dom, c->fluid_list, c->n_fluid, c->lab, c->div,
                                        c->x64, (float *)c->r, (float *)c->z, (double *)P.b[k & 1],
                                        c->pcg, force);
+  if (c->pcg_ic) Kernels<float, double, false, true>::ic_apply(c, P.b[k & 1], force);
   if (force) k_refresh_scalars<<<1, 1024, 0, c->stream>>>((const double *)P.b[k & 1], g, c->pcg);
 }
 
@@ -1637,6 +1676,9 @@ int clear_done(fsl_ctx *ctx) {
 size_t pcg_partials_bytes() { return kPartSlots * kPartSlot; }
 
 bool pcg_single_reduction(const fsl_solve_params *prm) {
+  // IC(0) needs z = M^-1 r between the residual update and r.z, which the
+  // single-reduction recurrence folds into one kernel: two passes only.
+  if (prm->preconditioner == FSL_PRECOND_IC) return false;
   if (prm->variant == FSL_PCG_SINGLE_REDUCTION) return true;
   if (prm->variant == FSL_PCG_TWO_PASS) return false;
   return prm->precision != FSL_PCG_FP32;
@@ -1645,10 +1687,16 @@ bool pcg_single_reduction(const fsl_solve_params *prm) {
 // Host-side solver state of a new solve; uploads it to the device.
 static int pcg_setup(fsl_ctx *ctx, const fsl_solve_params *prm) {
   if (prm->precision < 0 || prm->precision > 2 || prm->stop_mode < 0 || prm->stop_mode > 1 ||
-      prm->variant < 0 || prm->variant > 2) {
+      prm->variant < 0 || prm->variant > 2 || prm->preconditioner < 0 ||
+      prm->preconditioner > 1) {
     ctx->err = "invalid solver parameters";
     return FSL_ERR_ARG;
   }
+  if (prm->preconditioner == FSL_PRECOND_IC && prm->variant == FSL_PCG_SINGLE_REDUCTION) {
+    ctx->err = "the IC preconditioner runs with the two-pass variant only";
+    return FSL_ERR_ARG;
+  }
+  ctx->pcg_ic = prm->preconditioner == FSL_PRECOND_IC;
   ctx->pcg_precision = prm->precision;
   ctx->pcg_stop = prm->stop_mode;
   // An even period: the single-reduction path's x is complete after odd
@@ -1671,6 +1719,10 @@ static int pcg_setup(fsl_ctx *ctx, const fsl_solve_params *prm) {
 int pcg_prepare(fsl_ctx *ctx, const fsl_solve_params *prm) {
   int rc = pcg_setup(ctx, prm);
   if (rc) return rc;
+  if (ctx->pcg_ic) {
+    rc = ic_factor(ctx);
+    if (rc) return rc;
+  }
   if (ctx->n_fluid > 0) {
     dispatch(ctx->pcg_precision,
              [&](auto k) { decltype(k)::prepare(ctx, prm->warm_start ? 1 : 0); });
@@ -1689,7 +1741,7 @@ int pcg_iterate_once(fsl_ctx *ctx) {
   const int k = ctx->pcg_host->it;
   dispatch(ctx->pcg_precision, [&](auto K) {
     decltype(K)::pass_a(ctx, k);
-    decltype(K)::pass_b(ctx, k);
+    decltype(K)::pass_b_pc(ctx, k);
   });
   if (ctx->pcg_precision == FSL_PCG_MIXED && (k + 1) % ctx->pcg_refresh == 0)
     launch_refresh(ctx, k, 0);
@@ -1910,7 +1962,7 @@ int pcg_run(fsl_ctx *ctx, const fsl_solve_params *prm, fsl_solve_stats *st) {
   while (!ctx->timing && !ctx->pcg_host->done) {
     rc = run_pipelined(ctx, k, nb, adaptive, [&](int kk) {
       dispatch(ctx->pcg_precision, [&](auto K) { decltype(K)::pass_a(ctx, kk); });
-      dispatch(ctx->pcg_precision, [&](auto K) { decltype(K)::pass_b(ctx, kk); });
+      dispatch(ctx->pcg_precision, [&](auto K) { decltype(K)::pass_b_pc(ctx, kk); });
       if (mixed && (kk + 1) % ctx->pcg_refresh == 0) launch_refresh(ctx, kk, 0);
     });
     if (rc) return rc;
@@ -1934,7 +1986,7 @@ int pcg_run(fsl_ctx *ctx, const fsl_solve_params *prm, fsl_solve_stats *st) {
     for (int b = 0; b < nb; ++b, ++k) {
       hipEvent_t *e = pev + 4 * b;
       dispatch(ctx->pcg_precision, [&](auto K) { decltype(K)::pass_a(ctx, k, e[0], e[1]); });
-      dispatch(ctx->pcg_precision, [&](auto K) { decltype(K)::pass_b(ctx, k, e[2], e[3]); });
+      dispatch(ctx->pcg_precision, [&](auto K) { decltype(K)::pass_b_pc(ctx, k, e[2], e[3]); });
       if (mixed && (k + 1) % ctx->pcg_refresh == 0) launch_refresh(ctx, k, 0);
     }
     // The stopping test of iteration k runs at the start of A(k).
@@ -1970,7 +2022,7 @@ int pcg_run(fsl_ctx *ctx, const fsl_solve_params *prm, fsl_solve_stats *st) {
       rr_prev = h->rr;
       k_prev = k;
       // A(k) ran for real: the next batch starts at B(k).
-      dispatch(ctx->pcg_precision, [&](auto K) { decltype(K)::pass_b(ctx, k); });
+      dispatch(ctx->pcg_precision, [&](auto K) { decltype(K)::pass_b_pc(ctx, k); });
       if (mixed && (k + 1) % ctx->pcg_refresh == 0) launch_refresh(ctx, k, 0);
       ++k;
       continue;

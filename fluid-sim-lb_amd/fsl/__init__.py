@@ -20,11 +20,12 @@ OK = 0
 ERR = {
     -1: "FSL_ERR_ARG", -2: "FSL_ERR_HIP", -3: "FSL_ERR_STATE", -4: "FSL_ERR_CAPACITY",
     -10: "FSL_ERR_ZERO_DIAGONAL", -11: "FSL_ERR_NAN", -12: "FSL_ERR_ALPHA",
-    -13: "FSL_ERR_BETA", -14: "FSL_ERR_VELOCITY_UPDATE",
+    -13: "FSL_ERR_BETA", -14: "FSL_ERR_VELOCITY_UPDATE", -15: "FSL_ERR_PRECONDITIONER",
 }
 PCG_FP32, PCG_FP64, PCG_MIXED = 0, 1, 2
 PCG_AUTO, PCG_TWO_PASS, PCG_SINGLE_REDUCTION = 0, 1, 2
 STOP_ABS_RR, STOP_REL = 0, 1
+PRECOND_JACOBI, PRECOND_IC = 0, 1
 GRID_VELOCITY, GRID_VELOCITY_UPDATE = 0, 1
 GRID_DIVERGENCE, GRID_PRESSURE = 0, 1
 GRID_MARKER, GRID_SOLID, GRID_ROW_INDEX, GRID_CELL_COUNT = 0, 1, 2, 3
@@ -36,7 +37,7 @@ TIMING_NAMES = ("move", "clear", "p2g", "bc_save", "gravity_bc", "divergence", "
 class SolveParams(C.Structure):
     _fields_ = [("precision", C.c_int), ("stop_mode", C.c_int), ("threshold", C.c_double),
                 ("max_iterations", C.c_int), ("refresh_every", C.c_int), ("batch", C.c_int),
-                ("warm_start", C.c_int), ("variant", C.c_int)]
+                ("warm_start", C.c_int), ("variant", C.c_int), ("preconditioner", C.c_int)]
 
 
 class SolveStats(C.Structure):
@@ -142,16 +143,17 @@ def _ptr(a: np.ndarray | None):
 
 def solve_params(precision=PCG_FP32, stop_mode=STOP_ABS_RR, threshold=None, dims=None,
                  max_iterations=1500, refresh_every=32, batch=0,
-                 variant=PCG_AUTO) -> SolveParams:
+                 variant=PCG_AUTO, preconditioner=PRECOND_JACOBI) -> SolveParams:
     """Solver parameters; by default the reference's rule r.r < N^3 * 0.5e-8
     (flip_app.cc:326-328, flip_app.h:52-53). `variant` picks the iteration
-    structure: PCG_AUTO (two-pass for FP32, single-reduction otherwise),
-    PCG_TWO_PASS or PCG_SINGLE_REDUCTION."""
+    structure: PCG_AUTO (two-pass for FP32 or IC, single-reduction otherwise),
+    PCG_TWO_PASS or PCG_SINGLE_REDUCTION. `preconditioner`: PRECOND_JACOBI
+    (what the reference runs) or PRECOND_IC (block IC(0) per tile)."""
     if threshold is None:
         vol = int(np.prod(dims)) if dims is not None else 64 ** 3
         threshold = float(np.float32(np.float32(vol) * np.float32(0.5e-8)))
     return SolveParams(precision, stop_mode, threshold, max_iterations, refresh_every, batch, 0,
-                       variant)
+                       variant, preconditioner)
 
 
 class Context:

@@ -40,6 +40,8 @@ extern "C" {
 #define FSL_ERR_ALPHA (-12)         /* alpha > 1e5 (solver.cc:333) */
 #define FSL_ERR_BETA (-13)          /* beta > 1e5 (solver.cc:361) */
 #define FSL_ERR_VELOCITY_UPDATE (-14) /* face active in v, not saved (flip_simulation.cc:788-792) */
+#define FSL_ERR_PRECONDITIONER (-15) /* IC(0) pivot below 1e-5: the incomplete Cholesky
+                                        build failed (OpenVDB's compatibility test) */
 
 /* PCG arithmetic. */
 #define FSL_PCG_FP32 0  /* reference precision: FP32 vectors and dots, ordered stencil, no FMA */
@@ -47,12 +49,20 @@ extern "C" {
 #define FSL_PCG_MIXED 2 /* FP32 stencil/axpys, FP64 dots, FP64 true-residual refresh */
 
 /* PCG iteration structure. */
-#define FSL_PCG_AUTO 0     /* FP32: two-pass; FP64 and MIXED: single-reduction */
+#define FSL_PCG_AUTO 0     /* FP32 or IC: two-pass; FP64 and MIXED: single-reduction */
 #define FSL_PCG_TWO_PASS 1 /* the reference's stage order (solver_driver.cc:318-343): Ap and
                               p.Ap, then the axpys with r.r and r.z; two reductions */
 #define FSL_PCG_SINGLE_REDUCTION 2 /* one pass per iteration: p, Ap, x, r and A(M^-1 r) from
                               a two-cell halo, alpha from the Chronopoulos-Gear recurrence;
                               one reduction */
+
+/* Preconditioner (BuildPreconditioner, solver.cc:101-112). The reference's
+ * driver always ends up with Jacobi (solver_driver.cc:250 default-constructs
+ * the solver, so the type it was given is never used); IC selects the branch
+ * it would otherwise take: block IC(0) per tile (OpenVDB 4.0.1
+ * IncompleteCholeskyPreconditioner with openvdb.patch:209-224), two-pass only. */
+#define FSL_PRECOND_JACOBI 0
+#define FSL_PRECOND_IC 1
 
 /* PCG stopping rule. */
 #define FSL_STOP_ABS_RR 0 /* stop when r.r < threshold (solver_driver.cc:330) */
@@ -81,6 +91,7 @@ typedef struct fsl_solve_params {
   int warm_start;     /* 0: x0 = 0 (the cleared pressure grid, solver.cc:86-88);
                          1: x0 = the current pressure grid */
   int variant;        /* FSL_PCG_AUTO / TWO_PASS / SINGLE_REDUCTION */
+  int preconditioner; /* FSL_PRECOND_JACOBI (0, the reference's effective one) / IC */
 } fsl_solve_params;
 
 typedef struct fsl_solve_stats {

@@ -69,7 +69,8 @@ struct Cube {
   Vec3 start, end;
 };
 
-enum PreconditionerType { NO_PRECONDITIONER = 0, JACOBI = 1, INCOMPLETE_CHOLESKY = 2 };
+/* common::PreconditionerType (common/definitions.h:7). */
+enum PreconditionerType { IC, IC0, JACOBI, NUM_PRECONDITIONER_TYPES };
 
 /* application::Source (common/sources.h:11-76, sources.cc): a water source,
  * static, timed or oscillating. Behaviour kept as the reference has it:
@@ -464,14 +465,18 @@ class FlipSimulation {
 };
 
 /* PCGDriver (solver_driver.h:30-106): Jacobi PCG on the FLUID cells of the
- * simulation's marker grid, b = divergence grid, x = pressure grid. */
+ * simulation's marker grid, b = divergence grid, x = pressure grid. As in the
+ * reference, the constructor's preconditioner type is stored but the solve
+ * runs Jacobi (solver_driver.cc:250 default-constructs the solver);
+ * set_preconditioner(IC) selects the IC branch of BuildPreconditioner
+ * (solver.cc:103-111): block IC(0) per tile. */
 class PCGDriver {
  public:
   PCGDriver(std::string name, int max_iterations, float threshold, bool fixed_iterations,
             PreconditionerType preconditioner_type, Coord partitions, Coord global_dims)
       : name_(std::move(name)), max_iterations_(max_iterations), threshold_(threshold),
-        fixed_iterations_(fixed_iterations), partitions_(partitions), global_dims_(global_dims) {
-    (void)preconditioner_type;  // the reference runs Jacobi (solver.cc:103-106)
+        fixed_iterations_(fixed_iterations), preconditioner_type_(preconditioner_type),
+        partitions_(partitions), global_dims_(global_dims) {
     params_.precision = FSL_PCG_FP32;
     params_.stop_mode = FSL_STOP_ABS_RR;
   }
@@ -489,6 +494,13 @@ class PCGDriver {
     sim_->check(fsl_pcg_solve(sim_->context(), &params_, &stats_));
   }
 
+  /* The preconditioner the solve builds: IC gives block IC(0), every other
+   * type Jacobi (BuildPreconditioner's branches, solver.cc:103-111). */
+  void set_preconditioner(PreconditionerType t) {
+    params_.preconditioner = t == IC ? FSL_PRECOND_IC : FSL_PRECOND_JACOBI;
+  }
+  PreconditionerType preconditioner_type() const { return preconditioner_type_; }
+
   /* Precision / stopping rule beyond the reference's (FSL_PCG_*, FSL_STOP_*). */
   void set_solve_params(const fsl_solve_params &p) {
     params_ = p;
@@ -503,6 +515,7 @@ class PCGDriver {
   int max_iterations_;
   float threshold_;
   bool fixed_iterations_;
+  PreconditionerType preconditioner_type_;
   Coord partitions_, global_dims_;
   FlipSimulation *sim_ = nullptr;
   fsl_solve_params params_{};

@@ -541,14 +541,137 @@ struct Oracle {
     for (size_t q = 0; q < len; ++q) { T m = a[q] * b[q]; s = s + m; }
     return s;
   }
+  // ---- Block IC(0), the IC branch of BuildPreconditioner (solver.cc:103-106).
+  // Restates OpenVDB 4.0.1's published pcg::IncompleteCholeskyPreconditioner
+  // (math/ConjGradient.h; OpenVDB is not vendored) on the general sparse
+  // matrix, with openvdb.patch:209-224: a partition factors its own interior
+  // rows only (ghost columns are skipped). Here the partitions are the tiles
+  // of one global system, so an entry couples two rows only when both cells
+  // lie in the same tile. The patch leaves the innermost update loop
+  // unchecked for ghost columns (rows past the factor's end when a partition
+  // has ghosts); those entries are skipped here as well.
+  //   L = the lower triangle of A (block entries);
+  //   for k ascending: L_kk < 1e-5 fails the build; L_kk = sqrt(L_kk);
+  //     L_ik = A_ki / L_kk for i > k in row k of A;
+  //     for j > k in row k of A, for i >= j in row j of A: L_ij -= L_ik L_jk.
+  //   apply: L y = r by rows ascending, then L^T z = y by rows descending;
+  //     each row's dot runs over its stored columns in ascending order (the
+  //     diagonal contributes its product with the not yet written 0).
+  template <typename T>
+  struct Tri {
+    std::vector<int64_t> start;
+    std::vector<int32_t> col;  // ascending per row
+    std::vector<T> val;
+    int64_t find(int64_t r, int32_t c) const {
+      for (int64_t q = start[r]; q < start[r + 1]; ++q)
+        if (col[q] == c) return q;
+      return -1;
+    }
+    T get(int64_t r, int32_t c) const {
+      const int64_t q = find(r, c);
+      return q < 0 ? T(0) : val[q];
+    }
+  };
+  int tile_of_row(const std::vector<int32_t> &rows, int r, const int tiles[3]) const {
+    int t = 0;
+    for (int a = 2; a >= 0; --a) t = t * tiles[a] + rows[3 * r + a] / (n[a] / tiles[a]);
+    return t;
+  }
+  template <typename T>
+  bool ic_factor(const Csr &A, const std::vector<int32_t> &rows, const int tiles[3], Tri<T> &L,
+                 Tri<T> &U) const {
+    std::vector<int> tile(static_cast<size_t>(n_rows));
+    for (int r = 0; r < n_rows; ++r) tile[size_t(r)] = tile_of_row(rows, r, tiles);
+    auto coupled = [&](int a, int32_t b) { return b < n_rows && tile[size_t(a)] == tile[size_t(b)]; };
+    L.start.assign(size_t(n_rows) + 1, 0);
+    L.col.clear(); L.val.clear();
+    for (int r = 0; r < n_rows; ++r) {
+      for (int64_t e = A.start[r]; e < A.start[r + 1]; ++e)
+        if (A.col[e] <= r && coupled(r, A.col[e])) { L.col.push_back(A.col[e]); L.val.push_back(T(A.val[e])); }
+      L.start[r + 1] = int64_t(L.col.size());
+    }
+    bool ok = true;
+    for (int k = 0; k < n_rows && ok; ++k) {
+      T dk = L.get(k, k);
+      if (dk < T(1.e-5f)) { ok = false; break; }
+      dk = std::sqrt(dk);
+      L.val[size_t(L.find(k, k))] = dk;
+      for (int64_t e = A.start[k]; e < A.start[k + 1]; ++e) {
+        const int32_t ii = A.col[e];
+        if (ii < k + 1 || !coupled(k, ii)) continue;
+        L.val[size_t(L.find(ii, k))] = T(A.val[e]) / dk;
+      }
+      for (int64_t e = A.start[k]; e < A.start[k + 1]; ++e) {
+        const int32_t j = A.col[e];
+        if (j < k + 1 || !coupled(k, j)) continue;
+        const T a_jk = L.get(j, k);
+        for (int64_t f = A.start[j]; f < A.start[j + 1]; ++f) {
+          const int32_t i = A.col[f];
+          if (i < j || !coupled(j, i)) continue;
+          const int64_t q = L.find(i, j);
+          T a_ij = L.val[size_t(q)];
+          const T a_ik = L.get(i, k);
+          const T m = a_ik * a_jk;
+          a_ij = a_ij - m;
+          L.val[size_t(q)] = a_ij;
+        }
+      }
+    }
+    // U = L^T, rows ascending by column.
+    U.start.assign(size_t(n_rows) + 1, 0);
+    for (size_t q = 0; q < L.col.size(); ++q) ++U.start[size_t(L.col[q]) + 1];
+    for (int r = 0; r < n_rows; ++r) U.start[size_t(r) + 1] += U.start[size_t(r)];
+    U.col.assign(L.col.size(), 0);
+    U.val.assign(L.col.size(), T(0));
+    std::vector<int64_t> fill(U.start.begin(), U.start.end() - 1);
+    for (int r = 0; r < n_rows; ++r)
+      for (int64_t q = L.start[r]; q < L.start[r + 1]; ++q) {
+        const int64_t s = fill[size_t(L.col[q])]++;
+        U.col[size_t(s)] = r;
+        U.val[size_t(s)] = L.val[size_t(q)];
+      }
+    return ok;
+  }
+  template <typename T>
+  static void ic_apply(const Tri<T> &L, const Tri<T> &U, const std::vector<T> &r,
+                       std::vector<T> &z, std::vector<T> &tmp) {
+    const int64_t nr = int64_t(L.start.size()) - 1;
+    tmp.assign(size_t(nr), T(0));
+    z.assign(size_t(nr), T(0));
+    for (int64_t i = 0; i < nr; ++i) {
+      T dot = 0, diag = 0;
+      for (int64_t q = L.start[i]; q < L.start[i + 1]; ++q) {
+        if (L.col[q] == i) diag = L.val[q];
+        const T m = L.val[q] * tmp[size_t(L.col[q])];
+        dot = dot + m;
+      }
+      tmp[size_t(i)] = (r[size_t(i)] - dot) / diag;
+    }
+    for (int64_t i = nr - 1; i >= 0; --i) {
+      T dot = 0, diag = 0;
+      for (int64_t q = U.start[i]; q < U.start[i + 1]; ++q) {
+        if (U.col[q] == i) diag = U.val[q];
+        const T m = U.val[q] * z[size_t(U.col[q])];
+        dot = dot + m;
+      }
+      z[size_t(i)] = (tmp[size_t(i)] - dot) / diag;
+    }
+  }
+
   // Jacobi-PCG in the reference's stage structure (solver.cc:262-366,
   // solver_driver.cc:318-337). stop_mode 0: rr < threshold (absolute, the
   // reference rule); 1: sqrt(rr) < threshold * ||b|| (relative).
+  // precond 1: block IC(0) over `tiles` in place of Jacobi.
   template <typename T>
-  int solve(int stop_mode, double threshold, int max_iter) {
+  int solve(int stop_mode, double threshold, int max_iter, int precond = 0,
+            const int *tiles = nullptr) {
     Csr A;
     std::vector<int32_t> rows;
     assemble(A, rows);
+    const int one[3] = {1, 1, 1};
+    Tri<T> L, U;
+    std::vector<T> ic_tmp;
+    if (precond == 1 && !ic_factor<T>(A, rows, tiles ? tiles : one, L, U)) return -7;
     std::vector<T> b(n_rows), x(n_cols, T(0)), r(n_rows), z(n_rows), p(n_cols, T(0)),
         Ap(n_rows), dinv(n_rows);
     for (int q = 0; q < n_rows; ++q)
@@ -570,7 +693,11 @@ struct Oracle {
     auto done = [&](T rr) { return stop_mode == 0 ? double(rr) < thr : std::sqrt(double(rr)) <= thr; };
     // PreparePCGSolver.
     for (int q = 0; q < n_rows; ++q) { T m = row_dot(A, q, x); r[q] = b[q] - m; }
-    for (int q = 0; q < n_rows; ++q) z[q] = dinv[q] * r[q];
+    auto precondition = [&]() {
+      if (precond == 1) ic_apply<T>(L, U, r, z, ic_tmp);
+      else for (int q = 0; q < n_rows; ++q) z[q] = dinv[q] * r[q];
+    };
+    precondition();
     for (int q = 0; q < n_rows; ++q) p[q] = z[q];
     T rz = dot(r, z, n_rows), rr = dot(r, r, n_rows), rz_old = 0;
     rr_hist.clear();
@@ -585,7 +712,7 @@ struct Oracle {
       if (!(double(alpha) <= 1e5)) return -4;  // CHECK_LE(alpha, LARGE_FLOAT)
       for (int q = 0; q < n_cols; ++q) { T m = alpha * p[q]; x[q] = m + x[q]; }
       for (int q = 0; q < n_rows; ++q) { T m = -alpha * Ap[q]; r[q] = m + r[q]; }
-      for (int q = 0; q < n_rows; ++q) z[q] = dinv[q] * r[q];
+      precondition();
       rr = dot(r, r, n_rows);
       rz_old = rz;
       rz = dot(r, z, n_rows);
@@ -678,6 +805,36 @@ int or_solve(void *h, int precision, int stop_mode, double threshold, int max_it
   if (iters) *iters = o->iters;
   if (rr) *rr = o->rr_final;
   return rc;
+}
+// As or_solve; precond 0 Jacobi, 1 block IC(0) over tiles[3] (null: one tile).
+int or_solve_pc(void *h, int precision, int stop_mode, double threshold, int max_iter,
+                int precond, const int *tiles, int *iters, double *rr) {
+  Oracle *o = static_cast<Oracle *>(h);
+  int rc = precision == 0 ? o->solve<Real>(stop_mode, threshold, max_iter, precond, tiles)
+                          : o->solve<double>(stop_mode, threshold, max_iter, precond, tiles);
+  if (iters) *iters = o->iters;
+  if (rr) *rr = o->rr_final;
+  return rc;
+}
+// The block IC(0) factor of the current system in the solver's precision:
+// diag[row] = L_rr, and z = M^-1 r for r given in row order (z may be null).
+// Returns n_rows, or -7 when the build fails.
+int or_ic_factor(void *h, const int *tiles, double *diag, const double *r, double *z) {
+  Oracle *o = static_cast<Oracle *>(h);
+  Oracle::Csr A;
+  std::vector<int32_t> rows;
+  o->assemble(A, rows);
+  const int one[3] = {1, 1, 1};
+  Oracle::Tri<Real> L, U;
+  if (!o->ic_factor<Real>(A, rows, tiles ? tiles : one, L, U)) return -7;
+  for (int q = 0; q < o->n_rows; ++q) diag[q] = double(L.get(q, q));
+  if (r && z) {
+    std::vector<Real> rv(size_t(o->n_rows)), zv, tmp;
+    for (int q = 0; q < o->n_rows; ++q) rv[size_t(q)] = Real(r[q]);
+    Oracle::ic_apply<Real>(L, U, rv, zv, tmp);
+    for (int q = 0; q < o->n_rows; ++q) z[q] = double(zv[size_t(q)]);
+  }
+  return o->n_rows;
 }
 int or_solver_sizes(void *h, int *n_rows, int *n_cols) {
   Oracle *o = static_cast<Oracle *>(h);

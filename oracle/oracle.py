@@ -46,6 +46,8 @@ def load(fp64=False):
             "or_delete_outside": ([P], None), "or_add_gravity": ([P, F], None),
             "or_divergence": ([P], None),
             "or_solve": ([P, I, I, D, I, C.POINTER(I), C.POINTER(D)], I),
+            "or_solve_pc": ([P, I, I, D, I, I, P, C.POINTER(I), C.POINTER(D)], I),
+            "or_ic_factor": ([P, P, P, P, P], I),
             "or_solver_sizes": ([P, C.POINTER(I), C.POINTER(I)], I),
             "or_residual_history": ([P, P, I], I), "or_solution": ([P, P, I], I),
             "or_pressure_update": ([P], None), "or_velocity_update": ([P], I),
@@ -142,13 +144,30 @@ class Oracle:
     def divergence(self):
         self.lib.or_divergence(self.h)
 
-    def solve(self, precision=0, stop_mode=0, threshold=None, max_iter=1500):
+    def solve(self, precision=0, stop_mode=0, threshold=None, max_iter=1500, precond=0,
+              tiles=None):
+        """PCG on the current divergence; precond 0 Jacobi, 1 block IC(0) over
+        `tiles` (partitions per axis; None: one). Returns (rc, iterations, rr)."""
         if threshold is None:
             threshold = float(np.float32(np.float32(np.prod(self.dims)) * np.float32(0.5e-8)))
         it, rr = C.c_int(), C.c_double()
-        rc = self.lib.or_solve(self.h, precision, stop_mode, threshold, max_iter,
-                               C.byref(it), C.byref(rr))
+        t = None if tiles is None else np.ascontiguousarray(tiles, np.int32)
+        rc = self.lib.or_solve_pc(self.h, precision, stop_mode, threshold, max_iter, precond,
+                                  _ptr(t), C.byref(it), C.byref(rr))
         return rc, it.value, rr.value
+
+    def ic_factor(self, tiles=None, r=None):
+        """Block IC(0) of the current system: (L diagonal in row order, and
+        z = M^-1 r when r is given), or None when the build fails."""
+        cap = int(np.prod(self.dims))
+        diag = np.zeros(cap, np.float64)
+        t = None if tiles is None else np.ascontiguousarray(tiles, np.int32)
+        rv = None if r is None else np.ascontiguousarray(r, np.float64)
+        z = None if r is None else np.zeros(cap, np.float64)
+        m = self.lib.or_ic_factor(self.h, _ptr(t), _ptr(diag), _ptr(rv), _ptr(z))
+        if m < 0:
+            return None
+        return diag[:m].copy(), (None if z is None else z[:m].copy())
 
     def solver_sizes(self):
         a, b = C.c_int(), C.c_int()

@@ -453,3 +453,112 @@ def test_inside_boxes_bounds_inclusive():
     assert m.tolist() == [True, True, False, False]
     m2 = _orc.inside_boxes(pos, [[1, 1, 1, 2, 2, 2], [0, 0, 0, 0.5, 0.5, 0.5]])
     assert m2.tolist() == [True, True, False, True]
+
+
+# ---- block IC(0) (the IC branch of solver.cc:103-106, openvdb.patch:209-224) ----
+
+def _block_pattern(A, fl, n, tiles):
+    """A with the couplings between cells of different tiles dropped."""
+    w = np.array(n) // np.array(tiles)
+    t = [tuple((c - 1) // w) for c in fl]  # box index - 1 = cell index
+    Ab = A.copy()
+    for i, j in zip(*np.nonzero(A)):
+        if t[i] != t[j]:
+            Ab[i, j] = 0.0
+    return Ab
+
+
+def _l_from_diag(Ab, d):
+    """The IC(0) factor of a 7-point Laplacian from its diagonal: every lower
+    entry of column m is A_im / L_mm = -1 / L_mm."""
+    L = np.tril(Ab, -1) / d[None, :]
+    L[np.diag_indices_from(L)] = d
+    return L
+
+
+@pytest.mark.parametrize("tiles", [(1, 1, 1), (2, 2, 1), (1, 2, 2)])
+def test_ic0_factor_reproduces_a_on_its_pattern(tiles):
+    n = (8, 8, 8)
+    marker = drop_pocketless(random_labels(n, 21, p_fluid=0.75, p_solid=0.05))
+    A, fl = dense_laplacian(marker)
+    o = Oracle(n)
+    o.set_marker(marker)
+    d, _ = o.ic_factor(tiles)
+    assert d.shape == (len(fl),) and np.all(d > 0)
+    Ab = _block_pattern(A, fl, n, tiles)
+    L = _l_from_diag(Ab, d.astype(np.float64))
+    M = L @ L.T
+    pat = Ab != 0
+    # IC(0) defining property: L L^T equals A wherever A has an entry.
+    assert np.abs(M[pat] - Ab[pat]).max() < 1e-5
+    # and the factor differs from A only by fill outside the pattern.
+    assert np.abs((M - Ab)[pat]).max() < 1e-5 and np.abs(M - Ab).max() > 0
+
+
+def test_ic0_is_exact_cholesky_on_a_chain():
+    n = (4, 4, 16)
+    box = tuple(x + 2 for x in n)
+    marker = np.zeros(box, np.int32)
+    marker[0, :, :] = marker[-1, :, :] = marker[:, 0, :] = marker[:, -1, :] = 2
+    marker[:, :, 0] = marker[:, :, -1] = 2
+    marker[2, 2, 1:-1] = 1  # one FLUID column in AIR: a tridiagonal system
+    A, fl = dense_laplacian(marker)
+    o = Oracle(n)
+    o.set_marker(marker)
+    d, _ = o.ic_factor()
+    chol = np.linalg.cholesky(A)
+    assert np.allclose(d, np.diag(chol), rtol=2e-7, atol=0)
+
+
+@pytest.mark.parametrize("tiles", [(1, 1, 1), (2, 1, 2)])
+def test_ic0_apply_solves_the_factored_system(tiles):
+    n = (8, 8, 8)
+    marker = drop_pocketless(random_labels(n, 23, p_fluid=0.8, p_solid=0.05))
+    A, fl = dense_laplacian(marker)
+    rng = np.random.default_rng(24)
+    r = rng.standard_normal(len(fl)).astype(np.float32).astype(np.float64)
+    o = Oracle(n)
+    o.set_marker(marker)
+    d, z = o.ic_factor(tiles, r)
+    L = _l_from_diag(_block_pattern(A, fl, n, tiles), d)
+    y = scipy.linalg.solve_triangular(L, r, lower=True)
+    zd = scipy.linalg.solve_triangular(L.T, y, lower=False)
+    assert np.linalg.norm(z - zd) / np.linalg.norm(zd) < 1e-5
+
+
+def test_ic0_tile_blocks_are_independent():
+    n = (8, 8, 8)
+    marker = drop_pocketless(random_labels(n, 25, p_fluid=0.8, p_solid=0.05))
+    _, fl = dense_laplacian(marker)
+    o = Oracle(n)
+    o.set_marker(marker)
+    d1, _ = o.ic_factor((1, 1, 1))
+    d2, _ = o.ic_factor((2, 1, 1))
+    x = fl[:, 0] - 1
+    # Rows of the lower tile come first and see no later row: same bits.
+    assert np.array_equal(d1[x < 4], d2[x < 4])
+    # The first layer of the upper tile loses its x-1 coupling: its pivots
+    # grow back towards A_ii.
+    first = (x == 4)
+    assert np.all(d2[first] >= d1[first]) and np.any(d2[first] > d1[first])
+
+
+def test_ic_pcg_matches_dense_solve_in_fewer_iterations():
+    n = (8, 8, 8)
+    marker = drop_pocketless(random_labels(n, 27, p_fluid=0.8, p_solid=0.05))
+    A, fl = dense_laplacian(marker)
+    rng = np.random.default_rng(28)
+    box = marker.shape
+    v = rng.standard_normal(box + (3,)).astype(np.float32)
+    its = {}
+    for pc in (0, 1):
+        o = Oracle(n)
+        o.set_marker(marker)
+        o.set_vec(0, v, np.ones(box, np.uint8))
+        o.divergence()
+        b = o.get_scalar(0)[tuple(fl.T)].astype(np.float64)
+        rc, it, rr = o.solve(precision=1, stop_mode=1, threshold=1e-12, max_iter=5000, precond=pc)
+        assert rc == 0
+        its[pc] = it
+        assert np.allclose(o.solution(), scipy.linalg.solve(A, b), rtol=1e-9, atol=1e-9)
+    assert its[1] < its[0]
