@@ -38,6 +38,10 @@ HBM_PEAK_GBS = 8000.0
 # r, Ap, label and writes x, r, z.
 PASS_A_BYTES = {fsl.PCG_FP32: 17, fsl.PCG_MIXED: 17, fsl.PCG_FP64: 33}
 PASS_B_BYTES = {fsl.PCG_FP32: 29, fsl.PCG_MIXED: 29, fsl.PCG_FP64: 57}
+# Block IC(0) apply per fluid row: forward solve (read r, d, o, label; write
+# y), backward solve (read y, d, o, label; write z), and the r.r / r.z pass
+# (read r, z, label).
+IC_BYTES = {fsl.PCG_FP32: 43, fsl.PCG_MIXED: 43, fsl.PCG_FP64: 83}
 # Single-reduction pass (one kernel per iteration): reads r, p, label and
 # writes r, p every iteration; reads and writes x every other iteration (the
 # two-cell halo re-reads are not counted): 17 + 8/2 = 21 B per row with float
@@ -89,22 +93,27 @@ def pre_solve_stages(ctx, dt):
 def pcg_compare(ctx, dims):
     """Solve the same system with each precision and iteration structure
     (x0 = 0 every time). "fp32" is the plain solve in the reference's
-    precision and structure (FP32 vectors and dots, two passes per iteration)."""
+    precision and structure (FP32 vectors and dots, two passes per iteration).
+    The "ic_" solves use the block IC(0) preconditioner (the reference's
+    configured but inactive IC branch) in place of Jacobi."""
     out = {}
     pressures = {}
     SR, TP = fsl.PCG_SINGLE_REDUCTION, fsl.PCG_TWO_PASS
+    J, IC = fsl.PRECOND_JACOBI, fsl.PRECOND_IC
     runs = [
-        ("mixed", fsl.PCG_MIXED, SR, fsl.STOP_REL, 1e-6),
-        ("fp32", fsl.PCG_FP32, TP, fsl.STOP_REL, 1e-6),
-        ("fp64", fsl.PCG_FP64, SR, fsl.STOP_REL, 1e-6),
-        ("mixed_two_pass", fsl.PCG_MIXED, TP, fsl.STOP_REL, 1e-6),
-        ("fp64_two_pass", fsl.PCG_FP64, TP, fsl.STOP_REL, 1e-6),
-        ("fp32_reference_rule", fsl.PCG_FP32, TP, fsl.STOP_ABS_RR, None),
-        ("fp64_tight", fsl.PCG_FP64, SR, fsl.STOP_REL, 1e-12),
+        ("mixed", fsl.PCG_MIXED, SR, fsl.STOP_REL, 1e-6, J),
+        ("fp32", fsl.PCG_FP32, TP, fsl.STOP_REL, 1e-6, J),
+        ("fp64", fsl.PCG_FP64, SR, fsl.STOP_REL, 1e-6, J),
+        ("mixed_two_pass", fsl.PCG_MIXED, TP, fsl.STOP_REL, 1e-6, J),
+        ("fp64_two_pass", fsl.PCG_FP64, TP, fsl.STOP_REL, 1e-6, J),
+        ("fp32_reference_rule", fsl.PCG_FP32, TP, fsl.STOP_ABS_RR, None, J),
+        ("ic_fp32_reference_rule", fsl.PCG_FP32, TP, fsl.STOP_ABS_RR, None, IC),
+        ("ic_mixed", fsl.PCG_MIXED, TP, fsl.STOP_REL, 1e-6, IC),
+        ("fp64_tight", fsl.PCG_FP64, SR, fsl.STOP_REL, 1e-12, J),
     ]
-    for name, prec, var, stop, tol in runs:
+    for name, prec, var, stop, tol, pc in runs:
         prm = fsl.solve_params(precision=prec, stop_mode=stop, threshold=tol, dims=dims,
-                               max_iterations=20000, variant=var)
+                               max_iterations=20000, variant=var, preconditioner=pc)
         ctx.set_timing(True)
         st = ctx.solve(prm)
         t = ctx.timing()
@@ -113,6 +122,7 @@ def pcg_compare(ctx, dims):
         rows = st["n_rows"]
         rec = {
             "variant": "single_reduction" if var == SR else "two_pass",
+            "preconditioner": "ic0" if pc == IC else "jacobi",
             "iterations": st["iterations"],
             "solve_s": st["seconds"],
             "ms_per_iteration": 1e3 * st["seconds"] / it,
@@ -124,6 +134,8 @@ def pcg_compare(ctx, dims):
             rec["kernel_GBps"] = bytes_it / (rec["kernel_ms"] * 1e-3) / 1e9
         else:
             bytes_it = (PASS_A_BYTES[prec] + PASS_B_BYTES[prec]) * rows
+            if pc == IC:
+                bytes_it += IC_BYTES[prec] * rows
             rec["pass_a_ms"] = t["pass_a"] / max(t["pass_a_n"], 1)
             rec["pass_b_ms"] = t["pass_b"] / max(t["pass_b_n"], 1)
             rec["pass_a_GBps"] = PASS_A_BYTES[prec] * rows / (rec["pass_a_ms"] * 1e-3) / 1e9
@@ -135,9 +147,9 @@ def pcg_compare(ctx, dims):
         out[name] = rec
         pressures[name] = ctx.get_scalar(fsl.GRID_PRESSURE).astype(np.float64)
     # Un-hooked solve times (the timing hook's events slow the launches).
-    for name, prec, var, stop, tol in runs:
+    for name, prec, var, stop, tol, pc in runs:
         prm = fsl.solve_params(precision=prec, stop_mode=stop, threshold=tol, dims=dims,
-                               max_iterations=20000, variant=var)
+                               max_iterations=20000, variant=var, preconditioner=pc)
         out[name]["solve_s_unhooked"] = ctx.solve(prm)["seconds"]
     ref = pressures["fp64_tight"]
     den = np.linalg.norm(ref)
@@ -197,16 +209,18 @@ def cpu_baseline(iters_256, ppc=8):
     }
 
 
-def profile_solve(dims, mask, seed, tiles):
+def profile_solve(dims, mask, seed, tiles, ic=False):
     """One mixed-precision solve of the 256^3 dam-break system (after one step
     of particle stages), plus a plain device copy of 1 GiB (read 1 GiB, write
     1 GiB) that calibrates the HBM byte counters of a profiler run. Uses no
-    torch kernels (torch's kernels do not run under the profiler here)."""
+    torch kernels (torch's kernels do not run under the profiler here).
+    ic: the solve uses the block IC(0) preconditioner (two-pass)."""
     ctx = setup(dims, tiles, mask, seed)
     dt, _ = ctx.compute_dt(1, np.float32(1.0 / 30), 0.0)
     pre_solve_stages(ctx, dt)
     prm = fsl.solve_params(precision=fsl.PCG_MIXED, stop_mode=fsl.STOP_REL, threshold=1e-6,
-                           dims=dims, max_iterations=20000)
+                           dims=dims, max_iterations=20000,
+                           preconditioner=fsl.PRECOND_IC if ic else fsl.PRECOND_JACOBI)
     st = ctx.solve(prm)
     ctx.close()
     # Plain device-to-device copy through the HIP runtime (a blit kernel).
@@ -238,6 +252,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-solve", action="store_true",
                     help="one mixed solve + a 1 GiB copy, for profiler counter runs")
+    ap.add_argument("--ic", action="store_true",
+                    help="with --profile-solve: the solve uses the block IC(0) preconditioner")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only; the multi-GPU sharding of the "
@@ -248,7 +264,7 @@ def main():
     mask = mk(dims)
     tiles = (2, 2, 2) if dims[0] >= 256 else (1, 1, 1)
     if a.profile_solve:
-        profile_solve(dims, mask, seed, tiles)
+        profile_solve(dims, mask, seed, tiles, ic=a.ic)
         return
     t_setup = time.perf_counter()
     ctx = setup(dims, tiles, mask, seed)
