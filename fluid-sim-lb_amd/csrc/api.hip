@@ -152,7 +152,11 @@ int fsl_create(fsl_ctx **out, const int dims[3], const int tiles[3], int device,
     std::sort(keyed.begin(), keyed.end());
     std::vector<int32_t> order(nl);
     for (size_t q = 0; q < nl; ++q) order[q] = keyed[q].second;
-    if (hipMemcpy(ctx->morton, order.data(), nl * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess)
+    // On the context's stream, behind the zero fill dev_alloc queued there
+    // (a null-stream copy is not ordered with it and could be overwritten).
+    if (hipMemcpyAsync(ctx->morton, order.data(), nl * sizeof(int32_t), hipMemcpyHostToDevice,
+                       ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
       rc = FSL_ERR_HIP;
   }
   if (!rc) rc = dev_alloc(ctx, &ctx->fluid_list, nl);

@@ -27,11 +27,13 @@ def read_frame(path):
     return start, end, rec
 
 
-def test_water_app_matches_python_path(tmp_path):
+@pytest.mark.parametrize("pc", ["jacobi", "ic"])
+def test_water_app_matches_python_path(tmp_path, pc):
     n = 32
     out = str(tmp_path / "particles_")
     r = subprocess.run([APP, f"n={n}", "scene=sphere_drop", "frames=1", "seed=1",
-                        f"output={out}"], capture_output=True, text=True, timeout=120)
+                        f"output={out}", f"preconditioner={pc}"], capture_output=True, text=True,
+                       timeout=120)
     assert r.returncode == 0, r.stderr
     print(r.stdout.strip())
     start, end, rec = read_frame(out + "00000001")
@@ -44,7 +46,8 @@ def test_water_app_matches_python_path(tmp_path):
     ctx.set_params(9.8, (0.0, 9.8, 0.0), 0.95)
     ctx.set_box_boundary()
     ctx.seed_particles(mask, 8, 1)
-    prm = fsl.solve_params(dims=dims)
+    prm = fsl.solve_params(dims=dims, preconditioner=fsl.PRECOND_IC if pc == "ic"
+                           else fsl.PRECOND_JACOBI)
     frame = np.float32(1.0 / 30)
     step = np.float32(0.0)
     steps = 0
