@@ -18,7 +18,13 @@ computation or the published known answers of an algorithm it uses:
   advection (a4/5)  clamp rules and deletion in solids / outside the box
   extrapolation     two-hop corner value (order dependence) (a23)
   hydrostatics      a pool at rest stays at rest
+  block IC(0)       L L^T == A on A's block pattern; exact Cholesky on a chain;
+                    scipy triangular solves; independent tiles
+  fixtures          tests/golden/projection_systems.npz (make_golden.py): FP32
+                    divergence bit-exact, dense FP64 solutions, IC(0) diagonals
 """
+import os
+
 import numpy as np
 import pytest
 import scipy.linalg
@@ -562,3 +568,37 @@ def test_ic_pcg_matches_dense_solve_in_fewer_iterations():
         its[pc] = it
         assert np.allclose(o.solution(), scipy.linalg.solve(A, b), rtol=1e-9, atol=1e-9)
     assert its[1] < its[0]
+
+
+# ---- committed fixtures (tests/golden/make_golden.py) ----
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                      "projection_systems.npz")
+
+
+def golden_system(s):
+    g = np.load(GOLDEN)
+    return {k: g[f"{k}_{s}"] for k in ("dims", "tiles", "marker", "faces", "b", "x", "ic",
+                                        "ic_tiles")}
+
+
+@pytest.mark.parametrize("s", range(3))
+def test_oracle_matches_golden_systems(s):
+    g = golden_system(s)
+    n = tuple(int(d) for d in g["dims"])
+    rows = g["marker"] == 1
+    o = Oracle(n)
+    o.set_params(9.8, (0.0, 9.8, 0.0), 0.95)
+    o.set_box_boundary()
+    o.set_marker(g["marker"].astype(np.int32))
+    o.set_vec(0, g["faces"], np.ones(g["marker"].shape, np.uint8))
+    o.apply_bc()
+    o.divergence()
+    assert np.array_equal(o.get_scalar(0)[rows], g["b"])
+    for tiles, key in (((1, 1, 1), "ic"), (tuple(int(t) for t in g["tiles"]), "ic_tiles")):
+        d, _ = o.ic_factor(tiles)
+        assert np.max(np.abs(d - g[key]) / g[key]) < 1e-6
+    rc, it, _ = o.solve(precision=1, stop_mode=1, threshold=1e-13, max_iter=5000)
+    assert rc == 0
+    x = o.solution()
+    assert np.linalg.norm(x - g["x"]) / np.linalg.norm(g["x"]) < 1e-9
