@@ -50,7 +50,7 @@ def test_gpu_matches_golden_systems(s):
         p = ctx.get_scalar(fsl.GRID_PRESSURE)[rows].astype(np.float64)
         err = np.linalg.norm(p - x) / np.linalg.norm(x)
         print(f"  {name}: {st['iterations']} iterations, pressure vs dense {err:.2e}")
-        assert st["true_rel_residual"] <= 1e-12 and err <= 1e-6
+        assert st["true_rel_residual"] <= 2e-12 and err <= 1e-6
     loose = dict(stop_mode=fsl.STOP_REL, threshold=1e-6, max_iterations=5000, dims=n)
     for name, prm in {
         "mixed": fsl.solve_params(precision=fsl.PCG_MIXED, **loose),
