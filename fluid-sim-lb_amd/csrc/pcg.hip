@@ -1540,10 +1540,13 @@ struct Kernels {
         c->fluid_list, c->n_fluid, c->lab, (const V *)c->r, (const V *)c->z, (A *)part, c->pcg,
         force);
   }
-  // B(k) and, with IC, the preconditioner apply that completes it.
+  // B(k) and, with IC, the preconditioner apply that completes it. Every
+  // caller follows B(k) with the scheduled MIXED refresh when one is due,
+  // and the refresh applies IC to the refreshed residual itself.
   static void pass_b_pc(fsl_ctx *c, int k, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
     pass_b(c, k, e0, e1);
-    if (c->pcg_ic) ic_apply(c, parts(c).b[k & 1], 0);
+    const bool refresh_due = c->pcg_precision == FSL_PCG_MIXED && (k + 1) % c->pcg_refresh == 0;
+    if (c->pcg_ic && !refresh_due) ic_apply(c, parts(c).b[k & 1], 0);
   }
   // With events, hipExtLaunchKernelGGL records the dispatch's own start and
   // end (the kernel duration, as a profiler reports it).

@@ -151,6 +151,16 @@ def test_ic_solves_match_oracle(tiles):
           f"{st['true_rel_residual']:.2e}, pressure vs fp64 {rel(pm, p64):.2e}")
     assert st["true_rel_residual"] <= 1e-6 and rel(pm, p64) <= 1e-5
 
+    # MIXED with a refresh every 4 iterations: each refresh recomputes the
+    # residual in FP64 and applies IC to it in place of B's apply.
+    st = ctx.solve(ic_params(precision=fsl.PCG_MIXED, stop_mode=fsl.STOP_REL, threshold=1e-6,
+                             max_iterations=5000, refresh_every=4))
+    pm4 = ctx.get_scalar(fsl.GRID_PRESSURE)[rows]
+    print(f"  mixed, refresh every 4: {st['iterations']} iterations, true rel residual "
+          f"{st['true_rel_residual']:.2e}, pressure vs fp64 {rel(pm4, p64):.2e}")
+    assert st["iterations"] > 8
+    assert st["true_rel_residual"] <= 1e-6 and rel(pm4, p64) <= 1e-5
+
 
 def test_ic_stage_api_matches_solve():
     print("ic stage api", flush=True)
