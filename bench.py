@@ -276,8 +276,8 @@ def main():
                     help="with --profile-solve: the solve uses the block IC(0) preconditioner")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU only; the multi-GPU sharding of the "
-              "tiles is described in DESIGN.md and not run", file=sys.stderr)
+        print("bench.py: runs on one GPU only; sharding the tiles over several GPUs is "
+              "not implemented", file=sys.stderr)
         sys.exit(2)
 
     dims, mk, seed = scenes.WORKLOADS[a.workload]
