@@ -168,7 +168,8 @@ def pcg_compare(ctx, dims):
 
 def _oracle_tile_step(pos, vel, gv, dims):
     """One oracle step on a 128^3 dam-break tile: (non-PCG seconds, assembly
-    seconds, seconds per PCG iteration), with a bounded 40-iteration solve."""
+    seconds, seconds per PCG iteration, P2G seconds), with a bounded
+    40-iteration solve."""
     from oracle.oracle import Oracle
 
     o = Oracle(dims)
@@ -177,7 +178,11 @@ def _oracle_tile_step(pos, vel, gv, dims):
     o.set_particles(pos, vel, gv)
     dt, _ = o.compute_dt(1, 1.0 / 30, 0.0)
     t0 = time.perf_counter()
-    o.move_particles(dt); o.clear(); o.p2g(); o.apply_bc(); o.save_velocities()
+    o.move_particles(dt); o.clear()
+    tp = time.perf_counter()
+    o.p2g()
+    tp = time.perf_counter() - tp
+    o.apply_bc(); o.save_velocities()
     o.delete_outside(); o.add_gravity(dt); o.apply_bc(); o.divergence()
     t1 = time.perf_counter()
     o.solve(precision=0, stop_mode=1, threshold=1e-6, max_iter=0)
@@ -186,7 +191,7 @@ def _oracle_tile_step(pos, vel, gv, dims):
     t3 = time.perf_counter()
     o.pressure_update(); o.velocity_update(); o.g2p()
     t4 = time.perf_counter()
-    return (t1 - t0) + (t4 - t3), t2 - t1, ((t3 - t2) - (t2 - t1)) / max(it, 1)
+    return (t1 - t0) + (t4 - t3), t2 - t1, ((t3 - t2) - (t2 - t1)) / max(it, 1), tp
 
 
 def cpu_baseline(iters_256, ppc=8, threads=8):
@@ -207,11 +212,11 @@ def cpu_baseline(iters_256, ppc=8, threads=8):
     g = setup(dims, (1, 1, 1), mask, 2, ppc)
     pos, vel, gv = g.get_particles()
     g.close()
-    nonpcg1, asm1, it1 = _oracle_tile_step(pos, vel, gv, dims)
+    nonpcg1, asm1, it1, p2g1 = _oracle_tile_step(pos, vel, gv, dims)
     step_1 = 8 * (nonpcg1 + asm1) + 8 * it1 * iters_256
     with ThreadPoolExecutor(threads) as ex:
         res = list(ex.map(lambda _: _oracle_tile_step(pos, vel, gv, dims), range(threads)))
-    step_n = max(a + b + c * iters_256 for a, b, c in res)
+    step_n = max(a + b + c * iters_256 for a, b, c, _ in res)
     worst = max(res, key=lambda r: r[0] + r[1] + r[2] * iters_256)
     return {
         "value": 1.0 / step_n,
@@ -219,6 +224,8 @@ def cpu_baseline(iters_256, ppc=8, threads=8):
         "cores": threads,
         "kind": "port",
         "one_thread_value": 1.0 / step_1,
+        "one_thread_ms_per_pcg_iteration_256": 8e3 * it1,
+        "one_thread_p2g_particles_per_s": pos.shape[0] / p2g1,
         "sample": (f"oracle (CPU restatement of the reference step, FP32 Jacobi-PCG), the "
                    f"reference's execution model: {threads} tiles of 128^3 (the 256^3 dam "
                    f"break's partitions), one step each, concurrently on {threads} threads; "
