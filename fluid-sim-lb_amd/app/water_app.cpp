@@ -9,6 +9,10 @@
 //   solver_max_iterations=1500 solver_threshold=5e-9 (times n^3)
 //   preconditioner=jacobi|ic   (ic: block IC(0) per partition)
 //   output=<prefix>   writes <prefix>NNNNNNNN binary particle frames
+//   output_dir=<dir>  SaveDataToFile as the reference: <dir>/particles_NNNNNNNN,
+//                     the initial state as frame 0, then every frame
+//   debug=1           SaveDebugDataToFile into <output_dir>/debug at the same
+//                     points (the reference also writes inside each step)
 //   grid_output=<prefix>  also writes text grid dumps <prefix>{velocity,
 //                     pressure,marker}_NNNNNNNN each frame
 //   init=<file>       scene from an init file in the reference's grammar
@@ -225,7 +229,10 @@ int main(int argc, char **argv) {
     const int ppc = std::atoi(get(kv, "ppc", "8").c_str());
 
     fsl::FlipSimulation sim;
-    sim.InitializeMetadata(dims, parts, 0, 1.0f, ppc, gravity, flip);
+    const std::string out_dir = get(kv, "output_dir", "");
+    const bool debug = get(kv, "debug", "0") == "1";
+    sim.InitializeMetadata(dims, parts, 0, 1.0f, ppc, gravity, flip,
+                           out_dir.empty() ? "output" : out_dir, debug);
     sim.InitializeVariables();
     sim.set_seed(uint32_t(std::strtoul(get(kv, "seed", "1").c_str(), nullptr, 10)));
     if (!init.empty()) {
@@ -258,6 +265,15 @@ int main(int argc, char **argv) {
     prm.cfl = std::atoi(get(kv, "cfl", "1").c_str());
     fsl::FlipDriver driver(&sim, &solver, prm);
     const std::string out = get(kv, "output", "");
+    // FlipDriver::SaveData (flip_app_sim.cc:204, 259-274): the state after
+    // initialisation is frame 0, then one frame after every frame.
+    auto save = [&] {
+      if (debug) sim.SaveDebugDataToFile();
+      if (out_dir.empty()) return;
+      sim.SaveDataToFile();
+      sim.NextFrame();
+    };
+    save();
 
     for (int f = 0; f < prm.frames; ++f) {
       const int s0 = driver.steps();
@@ -278,6 +294,7 @@ int main(int argc, char **argv) {
         sim.WriteGrid(gout + "pressure_" + num, 10 + FSL_GRID_PRESSURE);
         sim.WriteGrid(gout + "marker_" + num, 10 + FSL_GRID_MARKER);
       }
+      save();
       std::printf("{\"frame\": %d, \"steps\": %d, \"particles\": %zu, \"last_iterations\": %d, "
                   "\"seconds\": %.6f}\n",
                   driver.frame(), driver.steps() - s0, sim.ParticleCount(),

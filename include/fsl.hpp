@@ -29,6 +29,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <filesystem>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -164,8 +165,10 @@ class FlipSimulation {
                           float voxel_len = 1, int num_particles_per_cell = 8,
                           float gravity = 9.8f, float flip_factor = 1,
                           std::string output_dir = "output", bool debug_flag = false) {
-    (void)output_dir;
-    (void)debug_flag;
+    output_dir_ = output_dir;
+    debug_ = debug_flag;
+    frame_ = 0;
+    debug_step_ = 0;
     if (std::fabs(voxel_len - 1.0f) > 1e-2f)
       throw Error(FSL_ERR_ARG, "voxel_len must be 1 (flip_simulation.cc:99)");
     dims_ = global_dims;
@@ -423,6 +426,42 @@ class FlipSimulation {
     return (((ri * 2048 + rj) * 2048 + rk) << 18) | (leaf << 9) | vox;
   }
 
+  /* SaveDataToFile (flip_simulation.cc:1284-1292): the binary particle
+   * frame <output_dir>/particles_<frame, %08d> (the reference's prefix,
+   * flip_simulation.cc:61-66; the directory is created on first use). The
+   * velocity and phi grids are not written there (commented out in the
+   * reference). */
+  void SaveDataToFile() {
+    std::filesystem::create_directories(output_dir_);
+    WriteParticles(output_dir_ + "/particles_" + num8(frame_), false);
+  }
+  /* SaveDebugDataToFile (flip_simulation.cc:1295-1317), debug_flag set: in
+   * <output_dir>/debug, particles_<step> (the text particle format, the
+   * default of Particles::Write), grid_velocity_<step> and
+   * grid_velocity_<step>.txt (both the text dump, as the reference's
+   * ScalarGrid/VectorGrid::Write print text in either mode), and the
+   * pressure grid as grid_phi_<step>.txt (the reference writes it under the
+   * phi prefix). The phi grid itself (level set, not built) is not written.
+   * Advances the debug step. */
+  void SaveDebugDataToFile() {
+    if (!debug_) throw Error(FSL_ERR_STATE, "SaveDebugDataToFile needs debug_flag");
+    const std::string dir = output_dir_ + "/debug", step = num8(debug_step_);
+    std::filesystem::create_directories(dir);
+    WriteParticles(dir + "/particles_" + step, true);
+    WriteGrid(dir + "/grid_velocity_" + step, FSL_GRID_VELOCITY);
+    WriteGrid(dir + "/grid_velocity_" + step + ".txt", FSL_GRID_VELOCITY);
+    WriteGrid(dir + "/grid_phi_" + step + ".txt", 10 + FSL_GRID_PRESSURE);
+    ++debug_step_;
+  }
+  /* NextFrame (flip_simulation.cc:1320-1322): the frame number of the next
+   * SaveDataToFile. */
+  void NextFrame() { ++frame_; }
+  int frame() const { return frame_; }
+  /* Empty in the reference (flip_simulation.cc:518-521, 642-644, 1192-1194). */
+  void AdvectPhi() {}
+  void ExtendVelocity() {}
+  void ResampleParticles() {}
+
   Coord global_dims() const { return dims_; }
   Coord partitions() const { return partitions_; }
   float flip_factor() const { return flip_; }
@@ -433,6 +472,11 @@ class FlipSimulation {
   }
 
  private:
+  static std::string num8(int n) {
+    char buf[16];
+    std::snprintf(buf, sizeof(buf), "%08d", n);
+    return buf;
+  }
   size_t box_cells() const {
     return size_t(dims_.x + 2) * (dims_.y + 2) * (dims_.z + 2);
   }
@@ -462,6 +506,9 @@ class FlipSimulation {
   uint32_t source_stream_ = 0;  // Philox key word of the next source call
   std::vector<Source> sources_;
   std::vector<uint8_t> mask_;
+  std::string output_dir_ = "output";
+  bool debug_ = false;
+  int frame_ = 0, debug_step_ = 0;
 };
 
 /* PCGDriver (solver_driver.h:30-106): Jacobi PCG on the FLUID cells of the

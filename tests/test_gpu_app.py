@@ -133,3 +133,37 @@ def test_water_app_grid_dumps(tmp_path):
     on = on[np.argsort(vdb_key(on[:, 0], on[:, 1], on[:, 2]), kind="stable")]
     assert np.array_equal(ijk, on)
     assert np.array_equal(vals[:, 0], ms[on[:, 0], on[:, 1], on[:, 2]].astype(np.float64))
+
+
+def test_water_app_save_data_to_file(tmp_path):
+    """SaveDataToFile / NextFrame (flip_simulation.cc:1284-1322) as the
+    reference's FlipDriver calls them (flip_app_sim.cc:204, 259-274): frame 0
+    is the initial state, frame 1 the state after the first frame, equal to
+    the output= frame of the same run; debug=1 adds the debug dumps."""
+    n = 32
+    out_dir = str(tmp_path / "out")
+    prefix = str(tmp_path / "frames_")
+    r = subprocess.run([APP, f"n={n}", "scene=sphere_drop", "frames=1", "seed=1",
+                        f"output_dir={out_dir}", f"output={prefix}", "debug=1"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    _, _, f0 = read_frame(os.path.join(out_dir, "particles_00000000"))
+    _, _, f1 = read_frame(os.path.join(out_dir, "particles_00000001"))
+    dims = (n, n, n)
+    mask = scenes.sphere_drop_mask(dims, 8, (16, 23, 16), 5)
+    assert f0.shape[0] == int(mask.sum()) * 8
+    with open(os.path.join(out_dir, "particles_00000001"), "rb") as a, \
+            open(prefix + "00000001", "rb") as b:
+        assert a.read() == b.read()
+    # One step: the move uses the previous G2P's grid velocity (0 at first),
+    # so positions are unchanged and velocities are not.
+    assert not np.array_equal(f0["vel"], f1["vel"])
+    dbg = os.path.join(out_dir, "debug")
+    for step in ("00000000", "00000001"):
+        for name in (f"particles_{step}", f"grid_velocity_{step}",
+                     f"grid_velocity_{step}.txt", f"grid_phi_{step}.txt"):
+            assert os.path.getsize(os.path.join(dbg, name)) > 0, name
+    with open(os.path.join(dbg, "particles_00000001")) as fh:
+        head = [fh.readline() for _ in range(4)]
+    assert head[0].split() == ["0", "0", "0"] and int(head[2]) == f1.shape[0]
+    assert head[3].startswith("0 : ")
