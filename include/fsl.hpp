@@ -532,6 +532,16 @@ class PCGDriver {
    * grids belong to the simulation's context. */
   void SetVariables(FlipSimulation *simulation) { sim_ = simulation; }
 
+  /* DeclareVariables, AssignWorkerForGlobals, AssignWorkersForPartitions and
+   * SetMigratableVariables (solver_driver.h:96-102) declare the solver's
+   * Canary variables and place them on workers. The simulation's context
+   * holds every solver vector on its device, so there is nothing to declare
+   * or place; the calls are kept so reference call sites compile. */
+  void DeclareVariables() {}
+  void AssignWorkerForGlobals(int idx) { (void)idx; }
+  void AssignWorkersForPartitions(const std::vector<int> &idxs) { (void)idxs; }
+  void SetMigratableVariables() {}
+
   /* Solve (solver_driver.cc:13-69): the reference's stopping rule is
    * r.r < threshold (FP32); fixed_iterations runs exactly max_iterations. */
   void Solve() {
