@@ -333,6 +333,8 @@ def main():
     iters_t = [s["iterations"] for s in stats_t]
     rows_t = [s["n_rows"] for s in stats_t]
     rows_w = float(np.average(rows_t, weights=np.maximum(iters_t, 1)))
+    # Leaf occupancy: FLUID rows over the cells of the leaves holding them.
+    occupancy = float(np.mean([s["n_rows"] / max(512 * s["n_fluid_leaves"], 1) for s in stats_t]))
     cg_ms = tm["fused"] / max(tm["fused_n"], 1)
     achieved = SR_BYTES[fsl.PCG_MIXED] * rows_w / (cg_ms * 1e-3) / 1e9
     pcg_time = tm["solve"] * 1e-3
@@ -382,6 +384,7 @@ def main():
             "dims": list(dims),
             "particles": n_particles,
             "fluid_rows_mean": rows_w,
+            "fluid_leaf_occupancy": occupancy,
             "pcg_iterations_per_step": iters,
             "parallelism": "1 GPU, tiles as work units",
         },
