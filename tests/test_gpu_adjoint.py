@@ -301,3 +301,29 @@ def test_autograd_projection_matches_c_abi():
     ref, _ = ctx.project_backward(tight(), w.cpu().numpy())
     assert np.allclose(u.grad.cpu().numpy(), ref, rtol=0, atol=1e-6 * np.abs(ref).max())
     ctx.close()
+
+
+def test_projection_with_ic_matches_jacobi():
+    """The block IC(0) preconditioner changes the solver's path, not the
+    system: the projection and its transpose match the Jacobi ones."""
+    n = (32, 32, 32)
+    marker = forward_labels(n, scenes.sphere_drop_mask(n, 8, (16, 23, 16), 5), 5)
+    ctx = projection_ctx(n, marker)
+    rng = np.random.default_rng(3)
+    u = rng.standard_normal(ctx.box + (3,)).astype(np.float32)
+    g = rng.standard_normal(ctx.box + (3,)).astype(np.float32)
+    ic = dict(TIGHT, preconditioner=fsl.PRECOND_IC)
+    out = {}
+    for name, prm in (("jacobi", tight()), ("ic", fsl.solve_params(**ic))):
+        ctx.set_vec(fsl.GRID_VELOCITY, u, np.ones(ctx.box, np.uint8))
+        st_f = ctx.project_forward(prm)
+        pu = ctx.get_vec(fsl.GRID_VELOCITY)[0].astype(np.float64)
+        ptg, st_b = ctx.project_backward(prm, g)
+        print(f"{name}: forward {st_f['iterations']} iterations, backward "
+              f"{st_b['iterations']}")
+        out[name] = (pu, ptg.astype(np.float64), st_f["iterations"], st_b["iterations"])
+    ctx.close()
+    (pj, gj, fj, bj), (pi, gi, fi, bi) = out["jacobi"], out["ic"]
+    assert fi < fj and bi < bj
+    assert np.linalg.norm(pi - pj) <= 1e-6 * np.linalg.norm(pj)
+    assert np.linalg.norm(gi - gj) <= 1e-6 * np.linalg.norm(gj)
