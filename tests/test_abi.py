@@ -47,3 +47,32 @@ def test_binding_loads_and_rejects_bad_dims():
     assert lib.fsl_create(ctypes.byref(h), dims, None, 0, 0) == -1
     assert lib.fsl_destroy(None) == -1
     assert lib.fsl_last_error(None) == b"null context"
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """The ctypes mirrors of fsl_solve_params and fsl_solve_stats have the C
+    header's field offsets and sizes (a C program compiled against
+    include/fsl.h prints them)."""
+    import ctypes as C
+    import subprocess
+
+    import fsl
+
+    structs = {"fsl_solve_params": fsl.SolveParams, "fsl_solve_stats": fsl.SolveStats}
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "fsl.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for field, _ in py._fields_:
+            lines.append(f'  printf("{cname} {field} %zu\\n", offsetof({cname}, {field}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    inc = os.path.join(ROOT, "include")
+    subprocess.run(["gcc", "-std=c99", "-I", inc, str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out.splitlines()}
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == C.sizeof(py), cname
+        for field, _ in py._fields_:
+            assert got[(cname, field)] == getattr(py, field).offset, (cname, field)
