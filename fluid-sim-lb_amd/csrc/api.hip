@@ -137,6 +137,23 @@ int fsl_create(fsl_ctx **out, const int dims[3], const int tiles[3], int device,
   if (!rc) rc = dev_alloc(ctx, &ctx->act_list, nl);
   if (!rc) rc = dev_alloc(ctx, &ctx->morton, nl);
   if (!rc) rc = dev_alloc(ctx, &ctx->lflag_m, nl);
+  if (!rc) rc = dev_alloc(ctx, &ctx->lrank, nl);
+  std::vector<int32_t> lrank(nl);
+  if (!rc) {
+    // Leaves in VDB traversal order of their origins (the particle order).
+    std::vector<std::pair<uint64_t, int32_t>> keyed(nl);
+    for (int64_t l = 0; l < ctx->nleaves; ++l) {
+      int o[3];
+      fsl::leaf_origin(ctx->dom, l, o);
+      keyed[size_t(l)] = {fsl::vdb_leaf_key(o[0], o[1], o[2]), int32_t(l)};
+    }
+    std::sort(keyed.begin(), keyed.end());
+    for (size_t q = 0; q < nl; ++q) lrank[size_t(keyed[q].second)] = int32_t(q);
+    if (hipMemcpyAsync(ctx->lrank, lrank.data(), nl * sizeof(int32_t), hipMemcpyHostToDevice,
+                       ctx->stream) != hipSuccess)
+      rc = FSL_ERR_HIP;
+    ctx->dom.lrank = ctx->lrank;
+  }
   if (!rc) {
     // Leaves sorted by the Morton code of their coordinates.
     std::vector<std::pair<uint64_t, int32_t>> keyed(nl);
@@ -197,6 +214,7 @@ int fsl_destroy(fsl_ctx *ctx) {
                   ctx->marker, ctx->solid, ctx->lab, ctx->div, ctx->pres, ctx->x, ctx->r, ctx->z,
                   ctx->p[0], ctx->p[1], ctx->Ap, ctx->x64, ctx->partials, ctx->pcg, ctx->lhas,
                   ctx->lact, ctx->lfluid, ctx->lscan, ctx->act_list, ctx->fluid_list, ctx->morton,
+                  ctx->lrank,
                   ctx->lflag_m, ctx->key, ctx->skey, ctx->p2g_w, ctx->p2g_ring,
                   ctx->perm, ctx->vcount, ctx->vstart, ctx->scan_tmp, ctx->flags};
   for (void *b : bufs) if (b) (void)hipFree(b);
@@ -814,6 +832,21 @@ int fsl_pcg_prepare(fsl_ctx *ctx, const fsl_solve_params *params, double *rr, do
   if (rr) *rr = ctx->pcg_host->rr;
   if (rz) *rz = ctx->pcg_host->rz;
   return rc;
+}
+
+int fsl_pcg_local1(fsl_ctx *ctx, double *pAp) {
+  CHECK_CTX();
+  return fsl::pcg_local1(ctx, pAp);
+}
+
+int fsl_pcg_local2(fsl_ctx *ctx, double *alpha, double *rr, double *rz) {
+  CHECK_CTX();
+  return fsl::pcg_local2(ctx, alpha, rr, rz);
+}
+
+int fsl_pcg_local3(fsl_ctx *ctx, double *beta) {
+  CHECK_CTX();
+  return fsl::pcg_local3(ctx, beta);
 }
 
 int fsl_pcg_iterate(fsl_ctx *ctx, double *pAp, double *alpha, double *rr, double *rz,

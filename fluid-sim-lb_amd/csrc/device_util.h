@@ -97,10 +97,44 @@ __device__ inline T block_sum(T v, T *lds /* >= 16 entries */) {
 // Neumaier-compensated accumulation (Neumaier 1974): running sum s with
 // correction c; s + c is the compensated total.
 __device__ inline void neumaier_add(double &s, double &c, double x) {
+#pragma clang fp contract(off)
   const double t = s + x;
   if (fabs(s) >= fabs(x)) c += (s - t) + x;
   else c += (x - t) + s;
   s = t;
+}
+
+// Knuth's TwoSum: s = fl(a + b) and the exact error e, a + b = s + e.
+__device__ inline void two_sum(double a, double b, double &s, double &e) {
+#pragma clang fp contract(off)
+  s = a + b;
+  const double bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+// Adds the compensated pair (s2, c2) to (s, c).
+__device__ inline void comp_add(double &s, double &c, double s2, double c2) {
+#pragma clang fp contract(off)
+  double t, e;
+  two_sum(s, s2, t, e);
+  s = t;
+  c = (c + c2) + e;
+}
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ inline void comp_dpp_step(double &s, double &c) {
+  comp_add(s, c, dpp_move<CTRL, ROW_MASK>(s), dpp_move<CTRL, ROW_MASK>(c));
+}
+// Compensated sum of the pairs (s, c) over a 64-lane wavefront, in the DPP
+// order of wave_sum: every combination is a TwoSum whose error joins the
+// correction term. Returns the total pair to every lane (fold with s + c).
+__device__ inline void wave_sum_c(double &s, double &c) {
+  comp_dpp_step<0xB1>(s, c);
+  comp_dpp_step<0x4E>(s, c);
+  comp_dpp_step<0x124>(s, c);
+  comp_dpp_step<0x128>(s, c);
+  comp_dpp_step<0x142, 0xA>(s, c);
+  comp_dpp_step<0x143, 0xC>(s, c);
+  s = read_lane(s, 63);
+  c = read_lane(c, 63);
 }
 
 }  // namespace fsl

@@ -31,6 +31,11 @@ struct Dom {
   int n[3];   // cells per axis
   int L[3];   // leaves per axis, n/8 + 2
   int T[3];   // tiles (reference partitions) per axis
+  // Rank of each leaf in VDB traversal order (root tiles of 64^3 by origin,
+  // then the leaf's offset in its tile): the particle sort orders cells by
+  // (rank, voxel offset), the reference's particle traversal order, and the
+  // per-cell particle ranges vstart are indexed that way (sort_index).
+  const int32_t *lrank;
 };
 
 __host__ __device__ inline int64_t vox_of(const Dom &d, int i, int j, int k) {
@@ -38,6 +43,24 @@ __host__ __device__ inline int64_t vox_of(const Dom &d, int i, int j, int k) {
   const int64_t leaf = (int64_t(a >> 3) * d.L[1] + (b >> 3)) * d.L[2] + (c >> 3);
   return (leaf << 9) | ((a & 7) << 6) | ((b & 7) << 3) | (c & 7);
 }
+// Index of voxel v (storage order) in the particle sort order.
+__device__ inline int64_t sort_index(const Dom &d, int64_t v) {
+  return (int64_t(d.lrank[v >> 9]) << 9) | (v & 511);
+}
+// First sort index of a leaf's voxels (its range ends at sort_begin + 512).
+__device__ inline int64_t sort_begin(const Dom &d, int64_t leaf) {
+  return int64_t(d.lrank[leaf]) << 9;
+}
+// VDB traversal key of a leaf origin (cell coordinates, multiples of 8):
+// root tile of 64^3 in lexicographic order of its origin, then the leaf's
+// offset in the tile, x-major.
+inline uint64_t vdb_leaf_key(int i, int j, int k) {
+  const uint64_t ri = uint64_t((i >> 6) + 1024), rj = uint64_t((j >> 6) + 1024),
+                 rk = uint64_t((k >> 6) + 1024);
+  const uint64_t off = uint64_t(((i >> 3) & 7) << 6 | ((j >> 3) & 7) << 3 | ((k >> 3) & 7));
+  return ((ri * 2048 + rj) * 2048 + rk) << 9 | off;
+}
+
 __host__ __device__ inline bool in_store(const Dom &d, int i, int j, int k) {
   return i >= -8 && j >= -8 && k >= -8 && i < d.n[0] + 8 && j < d.n[1] + 8 && k < d.n[2] + 8;
 }
@@ -74,6 +97,11 @@ struct PcgState {
   int it, done, status, max_iter;
   int stop_mode, precision, refresh_due;
   int pc_fail;    // the IC(0) build met a pivot below 1e-5
+  // Single-reduction path: the iteration at which a k_cg launch stopped
+  // (INT_MAX while running). Launches of later iterations exit on it; the
+  // launch that stops still runs in every workgroup, since a workgroup may
+  // start after the stopping decision was stored.
+  int stop_k;
   unsigned cnt_a, cnt_b, cnt_r, cnt_x;
 };
 
@@ -131,6 +159,9 @@ struct fsl_ctx {
   int pcg_precision = FSL_PCG_FP32, pcg_stop = FSL_STOP_ABS_RR, pcg_refresh = 32;
   int pcg_pbuf = 0;            // which p buffer holds the current direction
   bool pcg_ready = false;
+  bool pcg_sr = false;         // the last solve ran the single-reduction iteration
+  bool pcg_roworder = false;   // dot products in the oracle's row order (two-pass)
+  int pcg_stage = 0;           // stage API: next stage of the iteration (0, 1, 2 = Local1..3)
   // Block IC(0) preconditioner (allocated by the first IC solve): L's
   // diagonal d and -1/d per voxel (sized for doubles), the FLUID leaves
   // sorted by tile-local level, and the level offsets into that list.
@@ -145,6 +176,7 @@ struct fsl_ctx {
   uint32_t *lscan = nullptr;  // scan scratch over leaves (nleaves + 1)
   int32_t *act_list = nullptr, *fluid_list = nullptr;
   int32_t *morton = nullptr;   // leaf indices in Morton (Z-curve) order of leaf coordinates
+  int32_t *lrank = nullptr;    // leaf -> rank in VDB traversal order (dom.lrank)
   uint32_t *lflag_m = nullptr; // flags gathered in Morton order (scan input)
   int64_t n_act = 0, n_fluid = 0;
   bool bc_applied = false, grid_ready = false, outside_deleted = false;
@@ -222,6 +254,9 @@ size_t pcg_partials_bytes();
 int pcg_prepare(fsl_ctx *c, const fsl_solve_params *prm);
 int pcg_run(fsl_ctx *c, const fsl_solve_params *prm, fsl_solve_stats *st);
 int pcg_iterate_once(fsl_ctx *c);
+int pcg_local1(fsl_ctx *c, double *pAp);
+int pcg_local2(fsl_ctx *c, double *alpha, double *rr, double *rz);
+int pcg_local3(fsl_ctx *c, double *beta);
 int pcg_finish(fsl_ctx *c, fsl_solve_stats *st);
 int pcg_true_residual(fsl_ctx *c, double *rr_true);
 

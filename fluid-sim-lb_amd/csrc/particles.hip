@@ -1,12 +1,15 @@
 // Particle-side kernels: seeding, advection + counting sort by cell, leaf
 // activation, particle-to-grid (P2G) and grid-to-particle (G2P).
 //
-// Particles live in nine SoA float arrays sorted by cell (voxel key in the
-// leaf-major layout), so a leaf's particles are one contiguous range and a
-// cell's particles are a contiguous sub-range [vstart[v], vstart[v+1]).
-// Within a cell, particles keep their previous order (the sort is stable),
-// as the reference's StepInGrid re-adds particles in traversal order
-// (particles.cc:489-544); results are therefore deterministic run to run.
+// Particles live in nine SoA float arrays sorted by cell. Cells are ordered
+// as the reference's particle tree is traversed (VDB order: root tiles of
+// 64^3 by origin, leaves by offset in the tile, voxels x-major; sort_index),
+// so a leaf's particles are one contiguous range and a cell's particles the
+// sub-range [vstart[s], vstart[s+1]) at its sort index s. Within a cell,
+// particles keep their previous order (the sort is stable), as the
+// reference's StepInGrid re-adds particles in traversal order
+// (particles.cc:489-544): the device order is the reference's particle
+// order, and results are deterministic run to run.
 #include "device_util.h"
 #include "fsl_internal.h"
 
@@ -175,7 +178,10 @@ __global__ void k_advect(Dom d, int64_t np, Particles P, float dt, const uint8_t
         killed = killed || (p[0] >= kill.lo[b][0] && p[0] <= kill.hi[b][0] &&
                             p[1] >= kill.lo[b][1] && p[1] <= kill.hi[b][1] &&
                             p[2] >= kill.lo[b][2] && p[2] <= kill.hi[b][2]);
-      if (!solid[v] && !killed) k = uint32_t(v);
+      // The sort key ranks cells in VDB traversal order (the leaf's rank,
+      // then the voxel offset), so the sorted order is the reference's
+      // particle order.
+      if (!solid[v] && !killed) k = uint32_t(sort_index(d, v));
     }
     key[q] = k;
   }
@@ -269,10 +275,12 @@ __global__ void k_gather(uint32_t n, const uint32_t *perm, Particles in, Particl
 }
 
 // ------------------------------------------------------------- leaf lists
-__global__ void k_leaf_has(int64_t nleaves, const uint32_t *vstart, uint32_t *lhas) {
+__global__ void k_leaf_has(int64_t nleaves, const int32_t *lrank, const uint32_t *vstart,
+                           uint32_t *lhas) {
   const int64_t l = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (l >= nleaves) return;
-  lhas[l] = vstart[(l + 1) * kLeaf] > vstart[l * kLeaf] ? 1u : 0u;
+  const int64_t r = lrank[l];
+  lhas[l] = vstart[(r + 1) * kLeaf] > vstart[r * kLeaf] ? 1u : 0u;
 }
 // A leaf is active when it or one of its 26 neighbours holds particles:
 // everything the reference writes within a step lies within two cells of a
@@ -505,7 +513,8 @@ __global__ __launch_bounds__(512) void k_p2g_own(Dom d, const int32_t *list, int
   const int c[3] = {o[0] + lx, o[1] + ly, o[2] + lz};
   const int r = (lx + 1) * 100 + (ly + 1) * 10 + (lz + 1);
   const int64_t v = (leaf << 9) | t;
-  const uint32_t beg = vstart[v], end = vstart[v + 1];
+  const int64_t sv = sort_begin(d, leaf) | t;
+  const uint32_t beg = vstart[sv], end = vstart[sv + 1];
   __syncthreads();
   p2g_own(P, c, r, beg, end, s_vw, s_w, s_touch);
   vx[v] = s_vw[0][r]; vy[v] = s_vw[1][r]; vz[v] = s_vw[2][r];
@@ -594,7 +603,8 @@ __global__ __launch_bounds__(512) void k_p2g_combine(Dom d, const int32_t *list,
   int o[3];
   leaf_origin(d, leaf, o);
   uint8_t m = kAir;
-  if (in_bbox(d, o[0] + lx, o[1] + ly, o[2] + lz) && vstart[v + 1] > vstart[v]) m = kFluid;
+  const int64_t sv = sort_index(d, v);
+  if (in_bbox(d, o[0] + lx, o[1] + ly, o[2] + lz) && vstart[sv + 1] > vstart[sv]) m = kFluid;
   marker[v] = m;
 }
 
@@ -656,7 +666,7 @@ __global__ __launch_bounds__(256) void k_g2p(Dom d, const int32_t *list, int64_t
     s_u[0][idx] = a[3]; s_u[1][idx] = a[4]; s_u[2][idx] = a[5];
   }
   __syncthreads();
-  const uint32_t b = vstart[leaf << 9], e = vstart[(leaf + 1) << 9];
+  const uint32_t b = vstart[sort_begin(d, leaf)], e = vstart[sort_begin(d, leaf) + kLeaf];
   // Four particles per trip (q, q + 256, ...), their loads issued together
   // (indices clamped to the leaf's last particle; clamped copies are not
   // processed).
@@ -726,7 +736,7 @@ __global__ __launch_bounds__(256) void k_p2g_transpose(
     s_g[0][idx] = a[0]; s_g[1][idx] = a[1]; s_g[2][idx] = a[2];
   }
   __syncthreads();
-  const uint32_t b = vstart[leaf << 9], e = vstart[(leaf + 1) << 9];
+  const uint32_t b = vstart[sort_begin(d, leaf)], e = vstart[sort_begin(d, leaf) + kLeaf];
   for (uint32_t q = b + threadIdx.x; q < e; q += blockDim.x) {
     const float pos[3] = {P.a[0][q], P.a[1][q], P.a[2][q]};
     const int c[3] = {int(floorf(pos[0])), int(floorf(pos[1])), int(floorf(pos[2]))};
@@ -746,7 +756,7 @@ __global__ void k_cell_counts_box(Dom d, const uint32_t *vstart, int32_t *out) {
   const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (q >= int64_t(bx) * by * bz) return;
   const int i = int(q / (int64_t(by) * bz)) - 1, j = int((q / bz) % by) - 1, k = int(q % bz) - 1;
-  const int64_t v = vox_of(d, i, j, k);
+  const int64_t v = sort_index(d, vox_of(d, i, j, k));
   out[q] = int32_t(vstart[v + 1] - vstart[v]);
 }
 
@@ -814,7 +824,8 @@ hipError_t launch_sort_tail(fsl_ctx *c, uint32_t n_alive, bool keep_gv) {
 // Active leaves: holds particles or neighbours a leaf that does.
 hipError_t launch_leaf_flags(fsl_ctx *c) {
   const int64_t nl = c->nleaves;
-  k_leaf_has<<<blocks_for(nl, kThreads), kThreads, 0, c->stream>>>(nl, c->vstart, c->lhas);
+  k_leaf_has<<<blocks_for(nl, kThreads), kThreads, 0, c->stream>>>(nl, c->lrank, c->vstart,
+                                                                   c->lhas);
   k_leaf_dilate<<<blocks_for(nl, kThreads), kThreads, 0, c->stream>>>(c->dom, c->lhas, c->lact);
   return launch_compact(c, c->lact, c->act_list);
 }

@@ -26,8 +26,9 @@
 //   FP32  - the reference's precision: float vectors, float dot products,
 //           the stencil summed in the reference's column order without FMA.
 //   FP64  - double everywhere.
-//   MIXED - float vectors and stencil, double dot products (exact products,
-//           Neumaier-compensated across workgroups), and every `refresh`
+//   MIXED - float vectors and stencil, double dot products (exact products
+//           accumulated by FMA in each thread; TwoSum/Neumaier-compensated
+//           through the wave, workgroup and cross-workgroup sums), and every `refresh`
 //           iterations the float iterate is folded into a double solution and
 //           the true residual b - A x is recomputed in double (a reliable
 //           update; CG continues with the same direction).
@@ -175,25 +176,39 @@ __device__ inline Walk walk(int64_t nf) {
 __device__ inline int gtid() { return threadIdx.x % kG; }
 
 // Block-reduce K values and publish this workgroup's partial sums: wave
-// sums, one barrier, then thread 0 adds the waves' sums in order.
+// sums, one barrier, then thread 0 adds the waves' sums in order. Double
+// values are summed with TwoSum at every step (the error terms are carried
+// and folded in at the end).
 template <typename A, int K>
 __device__ inline void publish(A vals[K], A *partials) {
   __shared__ A s_red[K][16];
+  __shared__ A s_cor[K][16];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nw = (blockDim.x + 63) >> 6;
+  A cor[K];
 #pragma unroll
-  for (int k = 0; k < K; ++k) vals[k] = wave_sum(vals[k]);
+  for (int k = 0; k < K; ++k) {
+    cor[k] = A(0);
+    if constexpr (sizeof(A) == 8) wave_sum_c(vals[k], cor[k]);
+    else vals[k] = wave_sum(vals[k]);
+  }
   if (lane == 0) {
 #pragma unroll
-    for (int k = 0; k < K; ++k) s_red[k][wave] = vals[k];
+    for (int k = 0; k < K; ++k) {
+      s_red[k][wave] = vals[k];
+      s_cor[k][wave] = cor[k];
+    }
   }
   __syncthreads();
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      A t = A(0);
-      for (int w = 0; w < nw; ++w) t += s_red[k][w];
-      vals[k] = t;
+      A t = A(0), c = A(0);
+      for (int w = 0; w < nw; ++w) {
+        if constexpr (sizeof(A) == 8) comp_add(t, c, s_red[k][w], s_cor[k][w]);
+        else t += s_red[k][w];
+      }
+      vals[k] = t + c;
     }
 #pragma unroll
     for (int k = 0; k < K; ++k) partials[int64_t(blockIdx.x) * K + k] = vals[k];
@@ -218,34 +233,46 @@ __device__ inline double total(const A *partials, int n, int k) {
 }
 
 // Totals of all K values of the n published (stride K) partials in one pass:
-// one load round and one combined workgroup reduction, in A's precision, the
-// same in every workgroup. A thread adds at most a few partials (n is at most
-// a few thousand workgroups), so plain sums in a fixed order suffice: the
-// rounding error is a few units of 1e-16 relative to the sum of magnitudes.
+// one load round and one combined workgroup reduction, the same in every
+// workgroup. Double partials are summed with compensation: Neumaier within a
+// thread's strided partials, TwoSum through the wave and across the waves.
 template <typename A, int K>
 __device__ inline void total_all(const A *partials, int n, double out[K]) {
   __shared__ A s_t[K][16];
-  A v[K];
+  __shared__ A s_c[K][16];
+  A v[K], cv[K];
 #pragma unroll
-  for (int j = 0; j < K; ++j) v[j] = A(0);
+  for (int j = 0; j < K; ++j) v[j] = cv[j] = A(0);
   for (int q = threadIdx.x; q < n; q += blockDim.x) {
 #pragma unroll
-    for (int j = 0; j < K; ++j) v[j] += partials[K * q + j];
+    for (int j = 0; j < K; ++j) {
+      if constexpr (sizeof(A) == 8) neumaier_add(v[j], cv[j], partials[K * q + j]);
+      else v[j] += partials[K * q + j];
+    }
   }
 #pragma unroll
-  for (int j = 0; j < K; ++j) v[j] = wave_sum(v[j]);
+  for (int j = 0; j < K; ++j) {
+    if constexpr (sizeof(A) == 8) wave_sum_c(v[j], cv[j]);
+    else v[j] = wave_sum(v[j]);
+  }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nw = (blockDim.x + 63) >> 6;
   if (lane == 0) {
 #pragma unroll
-    for (int j = 0; j < K; ++j) s_t[j][wave] = v[j];
+    for (int j = 0; j < K; ++j) {
+      s_t[j][wave] = v[j];
+      s_c[j][wave] = cv[j];
+    }
   }
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < K; ++j) {
-    A t = A(0);
-    for (int w = 0; w < nw; ++w) t += s_t[j][w];
-    out[j] = double(t);
+    A t = A(0), c = A(0);
+    for (int w = 0; w < nw; ++w) {
+      if constexpr (sizeof(A) == 8) comp_add(t, c, s_t[j][w], s_c[j][w]);
+      else t += s_t[j][w];
+    }
+    out[j] = double(t + c);
   }
 }
 
@@ -890,7 +917,10 @@ __global__ __launch_bounds__(kG * GR) void k_cg(Dom d, const int32_t *flist, int
   __shared__ int32_t s_leaf[GR][kSched];
   // 1 / diag by label (label = FLUID | diag << 2; 0 for every non-row label).
   __shared__ V s_dinv[32];
-  if (st->done) return;
+  // Exit only on a stop stored by an earlier launch: workgroup 0 of this
+  // launch may store its own stop before other workgroups start, and those
+  // must still reach the same decision and apply the pending x update.
+  if (st->stop_k < k) return;
   const int g = threadIdx.x / kG, t = gtid();
   const int w = __builtin_amdgcn_readfirstlane(t >> 6), l = t & 63;
   if (threadIdx.x < 32) {
@@ -996,7 +1026,10 @@ __global__ __launch_bounds__(kG * GR) void k_cg(Dom d, const int32_t *flist, int
     st->rzh[k & 1] = double(gam);
     st->alph[k & 1] = double(alpha_a);
     if (status) st->status = status;
-    if (stop) st->done = 1;
+    if (stop) {
+      st->stop_k = k;
+      st->done = 1;
+    }
   }
   // alpha_{k-1}: x lacks alpha_{k-1} p_{k-1} when bit 1 of mode is set.
   const V alpha_prev = V(st->alph[(k + 1) & 1]);
@@ -1373,6 +1406,8 @@ __global__ __launch_bounds__(1024) void k_refresh_scalars_sr(const double *part_
   st->rr = tot[2];
   if (isnan(tot[2])) st->status = FSL_ERR_NAN;
   st->done = (st->status != 0 || converged(st, tot[2]) || st->it >= st->max_iter) ? 1 : 0;
+  // The solve resumes from iteration it: its k_cg launch must run.
+  if (!st->done) st->stop_k = 0x7fffffff;
 }
 
 // Stage API: after B(k), the scalars and the stopping test of iteration k+1
@@ -1393,6 +1428,73 @@ __global__ __launch_bounds__(1024) void k_stage_scalars(const A *part_b_new, int
   if (isnan(rr)) st->status = FSL_ERR_NAN;
   else if (!(double(b) <= 1e5)) st->status = FSL_ERR_BETA;
   st->done = (st->status != 0 || converged(st, rr) || k + 1 >= st->max_iter) ? 1 : 0;
+}
+
+// Stage API: pAp of A(k) (ComputeLocal1's output, solver.cc:304-308).
+template <typename A>
+__global__ __launch_bounds__(1024) void k_stage_pap(const A *part_a, int g, PcgState *st) {
+  const double pAp = total<A, 1>(part_a, g, 0);
+  if (threadIdx.x == 0 && !st->done) st->pAp = double(A(pAp));
+}
+
+// Dot products summed in the oracle's row order (FSL_DOT_ROW_ORDER): over
+// the FLUID rows in lexicographic (i, j, k) order, the row index order of
+// CreateIndexAndMaskTree (solver.cc:114-151), one rounding per product and
+// per sum, s = s + a_i b_i, as a sequential loop. K = 1: a0.b0; K = 2: also
+// a1.b1. The totals replace a pass's published partials: entry 0 of `part`
+// holds them and every other workgroup slot holds 0, so the pass's own
+// total over g partials returns them unchanged. One workgroup: chunks of
+// 256 cells are loaded and multiplied in parallel, and thread 0 adds them in
+// order (double-buffered, one barrier per chunk). For checks of the stage
+// arithmetic on small systems, not for speed.
+template <typename V, typename A, int K>
+__global__ __launch_bounds__(256) void k_rowdot(Dom d, const uint8_t *lab, const V *a0,
+                                                const V *b0, const V *a1, const V *b1, A *part,
+                                                int g, const PcgState *st, int force) {
+#pragma clang fp contract(off)
+  __shared__ A s_m[2][K][256];
+  __shared__ uint8_t s_f[2][256];
+  if ((!force && st->done) || st->status) return;
+  const int t = threadIdx.x;
+  const int64_t n12 = int64_t(d.n[1]) * d.n[2];
+  const int64_t ncell = int64_t(d.n[0]) * n12;
+  A sum[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) sum[j] = A(0);
+  int buf = 0;
+  for (int64_t q0 = 0; q0 < ncell; q0 += 256, buf ^= 1) {
+    const int64_t q = q0 + t;
+    uint8_t f = 0;
+    A m[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) m[j] = A(0);
+    if (q < ncell) {
+      const int i = int(q / n12), jj = int((q / d.n[2]) % d.n[1]), k = int(q % d.n[2]);
+      const int64_t v = vox_of(d, i, jj, k);
+      f = is_fluid(lab[v]) ? 1 : 0;
+      if (f) {
+        m[0] = A(a0[v]) * A(b0[v]);
+        if (K == 2) m[K - 1] = A(a1[v]) * A(b1[v]);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < K; ++j) s_m[buf][j][t] = m[j];
+    s_f[buf][t] = f;
+    __syncthreads();
+    if (t == 0) {
+      for (int u = 0; u < 256; ++u) {
+        if (!s_f[buf][u]) continue;
+#pragma unroll
+        for (int j = 0; j < K; ++j) sum[j] = sum[j] + s_m[buf][j][u];
+      }
+    }
+  }
+  // Thread 0 holds the sums.
+  if (t == 0) {
+#pragma unroll
+    for (int j = 0; j < K; ++j) part[j] = sum[j];
+  }
+  for (int e = K + t; e < K * g; e += 256) part[e] = A(0);
 }
 
 // After a forced refresh: the true r.r decides whether the solve is done.
@@ -1530,6 +1632,7 @@ struct Kernels {
         c->dom, c->fluid_list, c->n_fluid, c->lab, c->div, c->pres, warm, (V *)c->x, (V *)c->r,
         (V *)c->z, c->x64, (A *)P.b[1], (A *)P.bb);
     if (c->pcg_ic) ic_apply(c, P.b[1], 0);
+    if (c->pcg_roworder) rowdot_b(c, P.b[1], 1);
     k_prepare_scalars<A><<<1, 1024, 0, c->stream>>>((const A *)P.b[1], (const A *)P.bb, g, c->pcg);
   }
   // Block IC(0) in place of B's Jacobi z: z = M^-1 r, then (r.r, r.z) into
@@ -1547,6 +1650,7 @@ struct Kernels {
     pass_b(c, k, e0, e1);
     const bool refresh_due = c->pcg_precision == FSL_PCG_MIXED && (k + 1) % c->pcg_refresh == 0;
     if (c->pcg_ic && !refresh_due) ic_apply(c, parts(c).b[k & 1], 0);
+    if (c->pcg_roworder) rowdot_b(c, parts(c).b[k & 1], 0);
   }
   // With events, hipExtLaunchKernelGGL records the dispatch's own start and
   // end (the kernel duration, as a profiler reports it).
@@ -1558,6 +1662,17 @@ struct Kernels {
                           (const uint8_t *)c->lab, (const V *)c->z,
                           (const V *)c->p[(k + 1) & 1], (V *)c->p[k & 1], (V *)c->Ap,
                           (const A *)P.b[(k + 1) & 1], (A *)P.a, g, k, c->pcg);
+    if (c->pcg_roworder)
+      k_rowdot<V, A, 1><<<1, 256, 0, c->stream>>>(c->dom, c->lab, (const V *)c->p[k & 1],
+                                                  (const V *)c->Ap, nullptr, nullptr, (A *)P.a,
+                                                  g, c->pcg, 0);
+  }
+  // (r.r, r.z) of B(k) (or of the preconditioner apply completing it) in the
+  // oracle's row order, in place of the pass's partials.
+  static void rowdot_b(fsl_ctx *c, void *part, int force) {
+    k_rowdot<V, A, 2><<<1, 256, 0, c->stream>>>(c->dom, c->lab, (const V *)c->r, (const V *)c->r,
+                                                (const V *)c->r, (const V *)c->z, (A *)part,
+                                                grid_for(c->n_fluid), c->pcg, force);
   }
   static void pass_b(fsl_ctx *c, int k, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
     const int g = grid_for(c->n_fluid);
@@ -1566,6 +1681,9 @@ struct Kernels {
                           (const int32_t *)c->fluid_list, c->n_fluid, (const uint8_t *)c->lab,
                           (const V *)c->p[k & 1], (const V *)c->Ap, (V *)c->x, (V *)c->r,
                           (V *)c->z, (const A *)P.a, (A *)P.b[k & 1], g, k, c->pcg);
+  }
+  static void stage_pap(fsl_ctx *c) {
+    k_stage_pap<A><<<1, 1024, 0, c->stream>>>((const A *)parts(c).a, grid_for(c->n_fluid), c->pcg);
   }
   static void stage_scalars(fsl_ctx *c, int k) {
     const int g = grid_for(c->n_fluid);
@@ -1683,6 +1801,8 @@ bool pcg_single_reduction(const fsl_solve_params *prm) {
   // single-reduction recurrence folds into one kernel: two passes only.
   if (prm->preconditioner == FSL_PRECOND_IC) return false;
   if (prm->variant == FSL_PCG_SINGLE_REDUCTION) return true;
+  // Row-order dot products are a two-pass feature.
+  if (prm->dot_order == FSL_DOT_ROW_ORDER) return false;
   if (prm->variant == FSL_PCG_TWO_PASS) return false;
   return prm->precision != FSL_PCG_FP32;
 }
@@ -1699,6 +1819,16 @@ static int pcg_setup(fsl_ctx *ctx, const fsl_solve_params *prm) {
     ctx->err = "the IC preconditioner runs with the two-pass variant only";
     return FSL_ERR_ARG;
   }
+  if (prm->dot_order != FSL_DOT_TREE && prm->dot_order != FSL_DOT_ROW_ORDER) {
+    ctx->err = "invalid dot_order";
+    return FSL_ERR_ARG;
+  }
+  if (prm->dot_order == FSL_DOT_ROW_ORDER &&
+      (prm->variant == FSL_PCG_SINGLE_REDUCTION || prm->precision == FSL_PCG_MIXED)) {
+    ctx->err = "row-order dot products run with the two-pass FP32 or FP64 solve only";
+    return FSL_ERR_ARG;
+  }
+  ctx->pcg_roworder = prm->dot_order == FSL_DOT_ROW_ORDER;
   ctx->pcg_ic = prm->preconditioner == FSL_PRECOND_IC;
   ctx->pcg_precision = prm->precision;
   ctx->pcg_stop = prm->stop_mode;
@@ -1707,6 +1837,7 @@ static int pcg_setup(fsl_ctx *ctx, const fsl_solve_params *prm) {
   ctx->pcg_refresh = prm->refresh_every > 0 ? prm->refresh_every + (prm->refresh_every & 1) : 32;
   PcgState *h = ctx->pcg_host;
   *h = PcgState{};
+  h->stop_k = 0x7fffffff;
   h->max_iter = prm->max_iterations;
   h->stop_mode = prm->stop_mode;
   h->precision = prm->precision;
@@ -1722,6 +1853,8 @@ static int pcg_setup(fsl_ctx *ctx, const fsl_solve_params *prm) {
 int pcg_prepare(fsl_ctx *ctx, const fsl_solve_params *prm) {
   int rc = pcg_setup(ctx, prm);
   if (rc) return rc;
+  ctx->pcg_sr = false;
+  ctx->pcg_stage = 0;
   if (ctx->pcg_ic) {
     rc = ic_factor(ctx);
     if (rc) return rc;
@@ -1737,9 +1870,76 @@ int pcg_prepare(fsl_ctx *ctx, const fsl_solve_params *prm) {
   return ctx->pcg_host->status;
 }
 
+// Stage API, one stage per call (each synchronised on its own).
+// ComputeLocal1 (solver.cc:296-309) of iteration it: p = z + beta p (the
+// direction update of the previous iteration's ComputeLocal3, applied here,
+// where the pass reads p), Ap and pAp.
+int pcg_local1(fsl_ctx *ctx, double *pAp) {
+  if (!ctx->pcg_ready) { ctx->err = "pcg not prepared"; return FSL_ERR_STATE; }
+  if (ctx->pcg_sr) {
+    ctx->err = "stage calls need a two-pass solve (fsl_pcg_prepare with the two-pass variant)";
+    return FSL_ERR_STATE;
+  }
+  if (ctx->pcg_stage != 0) { ctx->err = "ComputeLocal1 out of order"; return FSL_ERR_STATE; }
+  if (ctx->n_fluid == 0 || ctx->pcg_host->done) return ctx->pcg_host->status;
+  const int k = ctx->pcg_host->it;
+  dispatch(ctx->pcg_precision, [&](auto K) {
+    decltype(K)::pass_a(ctx, k);
+    decltype(K)::stage_pap(ctx);
+  });
+  FSL_HIP(hipGetLastError());
+  int rc = sync_state(ctx);
+  if (rc) return rc;
+  if (pAp) *pAp = ctx->pcg_host->pAp;
+  if (!ctx->pcg_host->done) ctx->pcg_stage = 1;
+  return ctx->pcg_host->status;
+}
+
+// ComputeLocal2 (solver.cc:323-346): alpha = rz / pAp (checked against 1e5),
+// x += alpha p, r -= alpha Ap, z = M^-1 r, r.r and r.z; then the reductions
+// of solver_driver.cc:318-343 and beta = rz / rz_old.
+int pcg_local2(fsl_ctx *ctx, double *alpha, double *rr, double *rz) {
+  if (!ctx->pcg_ready || ctx->pcg_stage != 1) {
+    ctx->err = "ComputeLocal2 out of order";
+    return FSL_ERR_STATE;
+  }
+  const int k = ctx->pcg_host->it;
+  dispatch(ctx->pcg_precision, [&](auto K) { decltype(K)::pass_b_pc(ctx, k); });
+  if (ctx->pcg_precision == FSL_PCG_MIXED && (k + 1) % ctx->pcg_refresh == 0)
+    launch_refresh(ctx, k, 0);
+  dispatch(ctx->pcg_precision, [&](auto K) { decltype(K)::stage_scalars(ctx, k); });
+  FSL_HIP(hipGetLastError());
+  int rc = sync_state(ctx);
+  if (rc) return rc;
+  const PcgState *h = ctx->pcg_host;
+  if (alpha) *alpha = h->alpha;
+  if (rr) *rr = h->rr;
+  if (rz) *rz = h->rz;
+  ctx->pcg_stage = 2;
+  // A beta above 1e5 is ComputeLocal3's check: reported there.
+  return h->status == FSL_ERR_BETA ? FSL_OK : h->status;
+}
+
+// ComputeLocal3 (solver.cc:355-366): beta = rz / rz_old, checked against
+// 1e5; p = z + beta p is applied by the next ComputeLocal1.
+int pcg_local3(fsl_ctx *ctx, double *beta) {
+  if (!ctx->pcg_ready || ctx->pcg_stage != 2) {
+    ctx->err = "ComputeLocal3 out of order";
+    return FSL_ERR_STATE;
+  }
+  if (beta) *beta = ctx->pcg_host->beta;
+  ctx->pcg_stage = 0;
+  return ctx->pcg_host->status;
+}
+
 // One While-loop body, launched and synchronised on its own (stage API).
 int pcg_iterate_once(fsl_ctx *ctx) {
   if (!ctx->pcg_ready) { ctx->err = "pcg not prepared"; return FSL_ERR_STATE; }
+  if (ctx->pcg_sr) {
+    ctx->err = "stage calls need a two-pass solve (fsl_pcg_prepare with the two-pass variant)";
+    return FSL_ERR_STATE;
+  }
+  if (ctx->pcg_stage != 0) { ctx->err = "iteration started with stage calls"; return FSL_ERR_STATE; }
   if (ctx->n_fluid == 0 || ctx->pcg_host->done) return ctx->pcg_host->status;
   const int k = ctx->pcg_host->it;
   dispatch(ctx->pcg_precision, [&](auto K) {
@@ -1819,6 +2019,7 @@ static int run_pipelined(fsl_ctx *ctx, int &k, int batch, bool adaptive, Issue &
 static int pcg_run_sr(fsl_ctx *ctx, const fsl_solve_params *prm, fsl_solve_stats *st) {
   int rc = pcg_setup(ctx, prm);
   if (rc) return rc;
+  ctx->pcg_sr = true;
   if (ctx->n_fluid > 0) {
     dispatch(ctx->pcg_precision,
              [&](auto K) { decltype(K)::sr_prepare(ctx, prm->warm_start ? 1 : 0); });

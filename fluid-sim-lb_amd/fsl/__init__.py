@@ -26,6 +26,7 @@ PCG_FP32, PCG_FP64, PCG_MIXED = 0, 1, 2
 PCG_AUTO, PCG_TWO_PASS, PCG_SINGLE_REDUCTION = 0, 1, 2
 STOP_ABS_RR, STOP_REL = 0, 1
 PRECOND_JACOBI, PRECOND_IC = 0, 1
+DOT_TREE, DOT_ROW_ORDER = 0, 1
 GRID_VELOCITY, GRID_VELOCITY_UPDATE = 0, 1
 GRID_DIVERGENCE, GRID_PRESSURE = 0, 1
 GRID_MARKER, GRID_SOLID, GRID_ROW_INDEX, GRID_CELL_COUNT = 0, 1, 2, 3
@@ -37,7 +38,8 @@ TIMING_NAMES = ("move", "clear", "p2g", "bc_save", "gravity_bc", "divergence", "
 class SolveParams(C.Structure):
     _fields_ = [("precision", C.c_int), ("stop_mode", C.c_int), ("threshold", C.c_double),
                 ("max_iterations", C.c_int), ("refresh_every", C.c_int), ("batch", C.c_int),
-                ("warm_start", C.c_int), ("variant", C.c_int), ("preconditioner", C.c_int)]
+                ("warm_start", C.c_int), ("variant", C.c_int), ("preconditioner", C.c_int),
+                ("dot_order", C.c_int)]
 
 
 class SolveStats(C.Structure):
@@ -97,6 +99,9 @@ SIGNATURES = {
     "fsl_get_active_leaves": [_P, _I, _P, _I64, C.POINTER(_I64)],
     "fsl_tile_rows": [_P, C.POINTER(_I64)],
     "fsl_pcg_prepare": [_P, C.POINTER(SolveParams), _DP, _DP],
+    "fsl_pcg_local1": [_P, _DP],
+    "fsl_pcg_local2": [_P, _DP, _DP, _DP],
+    "fsl_pcg_local3": [_P, _DP],
     "fsl_pcg_iterate": [_P, _DP, _DP, _DP, _DP, _DP],
     "fsl_pcg_finish": [_P, C.POINTER(SolveStats)],
     "fsl_set_timing": [_P, _I],
@@ -143,17 +148,20 @@ def _ptr(a: np.ndarray | None):
 
 def solve_params(precision=PCG_FP32, stop_mode=STOP_ABS_RR, threshold=None, dims=None,
                  max_iterations=1500, refresh_every=32, batch=0,
-                 variant=PCG_AUTO, preconditioner=PRECOND_JACOBI) -> SolveParams:
+                 variant=PCG_AUTO, preconditioner=PRECOND_JACOBI,
+                 dot_order=DOT_TREE) -> SolveParams:
     """Solver parameters; by default the reference's rule r.r < N^3 * 0.5e-8
     (flip_app.cc:326-328, flip_app.h:52-53). `variant` picks the iteration
     structure: PCG_AUTO (two-pass for FP32 or IC, single-reduction otherwise),
     PCG_TWO_PASS or PCG_SINGLE_REDUCTION. `preconditioner`: PRECOND_JACOBI
-    (what the reference runs) or PRECOND_IC (block IC(0) per tile)."""
+    (what the reference runs) or PRECOND_IC (block IC(0) per tile).
+    `dot_order`: DOT_TREE, or DOT_ROW_ORDER (two-pass FP32/FP64 only: dot
+    products summed sequentially in the solver's row order)."""
     if threshold is None:
         vol = int(np.prod(dims)) if dims is not None else 64 ** 3
         threshold = float(np.float32(np.float32(vol) * np.float32(0.5e-8)))
     return SolveParams(precision, stop_mode, threshold, max_iterations, refresh_every, batch, 0,
-                       variant, preconditioner)
+                       variant, preconditioner, dot_order)
 
 
 class Context:
@@ -381,6 +389,24 @@ class Context:
         v = [C.c_double() for _ in range(5)]
         self.call("fsl_pcg_iterate", *[C.byref(x) for x in v])
         return dict(zip(("pAp", "alpha", "rr", "rz", "beta"), (x.value for x in v)))
+
+    def pcg_local1(self) -> float:
+        """ComputeLocal1: Ap and the reduced pAp."""
+        v = C.c_double()
+        self.call("fsl_pcg_local1", C.byref(v))
+        return v.value
+
+    def pcg_local2(self):
+        """ComputeLocal2: (alpha, rr, rz)."""
+        v = [C.c_double() for _ in range(3)]
+        self.call("fsl_pcg_local2", *[C.byref(x) for x in v])
+        return tuple(x.value for x in v)
+
+    def pcg_local3(self) -> float:
+        """ComputeLocal3: beta."""
+        v = C.c_double()
+        self.call("fsl_pcg_local3", C.byref(v))
+        return v.value
 
     def pcg_finish(self) -> dict:
         st = SolveStats()

@@ -64,6 +64,13 @@ extern "C" {
 #define FSL_PRECOND_JACOBI 0
 #define FSL_PRECOND_IC 1
 
+/* Summation order of the PCG dot products (two-pass FP32 and FP64 solves). */
+#define FSL_DOT_TREE 0      /* per-thread sums, wave and workgroup trees, fixed order */
+#define FSL_DOT_ROW_ORDER 1 /* sequential over the rows in the row-index order of
+                               CreateIndexAndMaskTree (solver.cc:114-151), one rounding
+                               per product and per sum: the oracle's order. For stage
+                               parity on small systems (one workgroup sums; slow) */
+
 /* PCG stopping rule. */
 #define FSL_STOP_ABS_RR 0 /* stop when r.r < threshold (solver_driver.cc:330) */
 #define FSL_STOP_REL 1    /* stop when ||r|| < threshold * ||b|| */
@@ -92,6 +99,7 @@ typedef struct fsl_solve_params {
                          1: x0 = the current pressure grid */
   int variant;        /* FSL_PCG_AUTO / TWO_PASS / SINGLE_REDUCTION */
   int preconditioner; /* FSL_PRECOND_JACOBI (0, the reference's effective one) / IC */
+  int dot_order;      /* FSL_DOT_TREE (0) / FSL_DOT_ROW_ORDER */
 } fsl_solve_params;
 
 typedef struct fsl_solve_stats {
@@ -206,6 +214,21 @@ int fsl_tile_rows(fsl_ctx *ctx, int64_t *rows_per_tile);
 /* ---- PCG stage API (solver.cc:262-366, solver_driver.cc:318-343) ---- */
 /* Assembly + PreparePCGSolver: r = b - A x, z = M^-1 r, p = z, rz, rr. */
 int fsl_pcg_prepare(fsl_ctx *ctx, const fsl_solve_params *params, double *rr, double *rz);
+/* PCGSolver::ComputeLocal1 (solver.cc:296-309): Ap = A p and pAp (the
+ * global value, after the driver's reduction, solver_driver.cc:318-343).
+ * p = z + beta p of the previous iteration's ComputeLocal3 is applied here,
+ * where the pass reads p. The stage calls follow fsl_pcg_prepare in the
+ * order Local1, Local2, Local3 (FSL_ERR_STATE otherwise); the While test
+ * (solver_driver.cc:318-337) is the caller's. */
+int fsl_pcg_local1(fsl_ctx *ctx, double *pAp);
+/* PCGSolver::ComputeLocal2 (solver.cc:323-346): alpha = rz / pAp (alpha above
+ * 1e5 returns FSL_ERR_ALPHA), x += alpha p, r -= alpha Ap, z = M^-1 r, and the
+ * reduced r.r and r.z. */
+int fsl_pcg_local2(fsl_ctx *ctx, double *alpha, double *rr, double *rz);
+/* PCGSolver::ComputeLocal3 (solver.cc:355-366): beta = rz / rz_old (beta above
+ * 1e5 returns FSL_ERR_BETA); the direction update is applied by the next
+ * fsl_pcg_local1. */
+int fsl_pcg_local3(fsl_ctx *ctx, double *beta);
 /* One While-loop body: ComputeLocal1 (Ap, pAp), ComputeLocal2 (alpha, x, r,
  * z, rr, rz), ComputeLocal3 (beta; p = z + beta p is applied lazily by the
  * next ComputeLocal1). Returns the scalars after the iteration. */

@@ -579,6 +579,51 @@ class PCGDriver {
   fsl_solve_stats stats_{};
 };
 
+/* PCGSolver (solver.h:143-205): the solver's stages, one call each, on the
+ * simulation's grids (b = divergence, x = pressure, the FLUID rows of the
+ * marker). The reference driver's While loop (solver_driver.cc:318-343) is
+ *   PreparePCGSolver(); while (!(it >= max || rr() < threshold)) {
+ *     ComputeLocal1(); ComputeLocal2(); ComputeLocal3(); ++it; }
+ *   CopySolution();
+ * The scalars are the global (reduced) values. The stages run the two-pass
+ * iteration; params.dot_order = FSL_DOT_ROW_ORDER sums the dot products in
+ * the row order of CreateIndexAndMaskTree (solver.cc:114-151). */
+class PCGSolver {
+ public:
+  PCGSolver(FlipSimulation *simulation, const fsl_solve_params &params)
+      : sim_(simulation), params_(params) {
+    params_.variant = FSL_PCG_TWO_PASS;
+  }
+  void PreparePCGSolver() {
+    double rr = 0, rz = 0;
+    sim_->check(fsl_pcg_prepare(sim_->context(), &params_, &rr, &rz));
+    rr_ = rr;
+    rz_ = rz;
+  }
+  void ComputeLocal1() { sim_->check(fsl_pcg_local1(sim_->context(), &pAp_)); }
+  void ComputeLocal2() {
+    rz_old_ = rz_;
+    sim_->check(fsl_pcg_local2(sim_->context(), &alpha_, &rr_, &rz_));
+  }
+  void ComputeLocal3() { sim_->check(fsl_pcg_local3(sim_->context(), &beta_)); }
+  /* CopySolution (solver.cc:93-99) into the pressure grid. */
+  void CopySolution() { sim_->check(fsl_pcg_finish(sim_->context(), &stats_)); }
+
+  double rz() const { return rz_; }
+  double rz_old() const { return rz_old_; }
+  double rr() const { return rr_; }
+  double pAp() const { return pAp_; }
+  double alpha() const { return alpha_; }
+  double beta() const { return beta_; }
+  const fsl_solve_stats &stats() const { return stats_; }
+
+ private:
+  FlipSimulation *sim_;
+  fsl_solve_params params_;
+  double rz_ = 0, rz_old_ = 0, rr_ = 0, pAp_ = 0, alpha_ = 0, beta_ = 0;
+  fsl_solve_stats stats_{};
+};
+
 /* FlipDriver's time loop (flip_app_sim.cc:211-254, 357-462) as plain calls. */
 class FlipDriver {
  public:

@@ -35,6 +35,8 @@
 #include <cstring>
 #include <vector>
 
+#include <omp.h>
+
 // Scalar type of grids, particles and the API: float restates the reference;
 // the double build (-DORACLE_REAL=double, liboracle64.so) is the same
 // algorithm in FP64, for finite-difference checks of the adjoint.
@@ -123,6 +125,10 @@ struct Oracle {
   double rr_final = 0;
   Dense<int32_t> idx;            // solver row index (solver.cc:114-151)
   std::vector<double> rr_hist;
+  // Per-iteration scalars of the last solve, in the solver's precision
+  // (widened): rz after PreparePCGSolver and after each ComputeLocal2, and
+  // pAp, alpha, beta of each iteration.
+  std::vector<double> rz_hist, pap_hist, alpha_hist, beta_hist;
 
   void init(int x, int y, int z) {
     n[0] = x; n[1] = y; n[2] = z;
@@ -133,10 +139,40 @@ struct Oracle {
   bool in_bbox(int i, int j, int k) const {
     return i >= 0 && j >= 0 && k >= 0 && i < n[0] && j < n[1] && k < n[2];
   }
+  // Stable sort of the particles by VDB traversal key. Cells in [0, n] (all
+  // a sort meets after StepInGrid's clamps) are ranked densely (root tile,
+  // then leaf, then voxel, the order of vdb_key) and counting-sorted; any
+  // other cell falls back to a comparison sort. Both keep equal keys in
+  // buffer order.
   void sort_particles() {
-    std::stable_sort(parts.begin(), parts.end(), [](const Particle &a, const Particle &b) {
-      return vdb_key(a.cell[0], a.cell[1], a.cell[2]) < vdb_key(b.cell[0], b.cell[1], b.cell[2]);
-    });
+    const int64_t nn[3] = {n[0] / 64 + 1, n[1] / 64 + 1, n[2] / 64 + 1};
+    bool dense = true;
+    for (const Particle &p : parts)
+      for (int d = 0; d < 3; ++d)
+        if (p.cell[d] < 0 || p.cell[d] > n[d]) dense = false;
+    if (!dense) {
+      std::stable_sort(parts.begin(), parts.end(), [](const Particle &a, const Particle &b) {
+        return vdb_key(a.cell[0], a.cell[1], a.cell[2]) < vdb_key(b.cell[0], b.cell[1], b.cell[2]);
+      });
+      return;
+    }
+    auto rank = [&](const int c[3]) {
+      const int64_t root = ((int64_t(c[0] >> 6) * nn[1]) + (c[1] >> 6)) * nn[2] + (c[2] >> 6);
+      const int64_t leaf = ((c[0] >> 3) & 7) << 6 | ((c[1] >> 3) & 7) << 3 | ((c[2] >> 3) & 7);
+      const int64_t vox = (c[0] & 7) << 6 | (c[1] & 7) << 3 | (c[2] & 7);
+      return (root << 18) | (leaf << 9) | vox;
+    };
+    const size_t nb = size_t(nn[0] * nn[1] * nn[2]) << 18;
+    std::vector<uint32_t> start(nb + 1, 0);
+    std::vector<uint32_t> key(parts.size());
+    for (size_t q = 0; q < parts.size(); ++q) {
+      key[q] = uint32_t(rank(parts[q].cell));
+      ++start[key[q] + 1];
+    }
+    for (size_t b = 0; b < nb; ++b) start[b + 1] += start[b];
+    std::vector<Particle> out(parts.size());
+    for (size_t q = 0; q < parts.size(); ++q) out[start[key[q]]++] = parts[q];
+    parts.swap(out);
   }
 
   // FlipSimulation::SetBoxBoundary (flip_simulation.cc:255-308), one partition.
@@ -526,6 +562,11 @@ struct Oracle {
       A.start[r + 1] = int64_t(A.col.size());
     }
   }
+  // Threads of the solver's row-parallel loops (SpMV, axpys, Jacobi: each row
+  // is computed alone, so the results do not depend on the count) and of the
+  // FP64 dot products (fixed chunks, see dot()).
+  int threads = 1;
+
   template <typename T>
   static T row_dot(const Csr &A, int64_t r, const std::vector<T> &x) {
     T s = 0;
@@ -535,10 +576,28 @@ struct Oracle {
     }
     return s;
   }
+  // Dot product. In FP32 (the reference's precision) a sequential sum in row
+  // order. In FP64 the rows are cut into 256 fixed chunks, each summed in
+  // order, and the chunk sums added in order: the result does not depend on
+  // the thread count.
   template <typename T>
-  static T dot(const std::vector<T> &a, const std::vector<T> &b, size_t len) {
+  T dot(const std::vector<T> &a, const std::vector<T> &b, size_t len) const {
+    if (sizeof(T) == 4 || len < 4096) {
+      T s = 0;
+      for (size_t q = 0; q < len; ++q) { T m = a[q] * b[q]; s = s + m; }
+      return s;
+    }
+    constexpr int kChunks = 256;
+    T part[kChunks];
+#pragma omp parallel for num_threads(threads) schedule(static)
+    for (int c = 0; c < kChunks; ++c) {
+      const size_t q0 = len * size_t(c) / kChunks, q1 = len * size_t(c + 1) / kChunks;
+      T s = 0;
+      for (size_t q = q0; q < q1; ++q) { T m = a[q] * b[q]; s = s + m; }
+      part[c] = s;
+    }
     T s = 0;
-    for (size_t q = 0; q < len; ++q) { T m = a[q] * b[q]; s = s + m; }
+    for (int c = 0; c < kChunks; ++c) s = s + part[c];
     return s;
   }
   // ---- Block IC(0), the IC branch of BuildPreconditioner (solver.cc:103-106).
@@ -692,32 +751,48 @@ struct Oracle {
     // Relative mode uses <=, so a zero right-hand side (x0 = 0) stops at once.
     auto done = [&](T rr) { return stop_mode == 0 ? double(rr) < thr : std::sqrt(double(rr)) <= thr; };
     // PreparePCGSolver.
+#pragma omp parallel for num_threads(threads) schedule(static)
     for (int q = 0; q < n_rows; ++q) { T m = row_dot(A, q, x); r[q] = b[q] - m; }
     auto precondition = [&]() {
       if (precond == 1) ic_apply<T>(L, U, r, z, ic_tmp);
-      else for (int q = 0; q < n_rows; ++q) z[q] = dinv[q] * r[q];
+      else {
+#pragma omp parallel for num_threads(threads) schedule(static)
+        for (int q = 0; q < n_rows; ++q) z[q] = dinv[q] * r[q];
+      }
     };
     precondition();
     for (int q = 0; q < n_rows; ++q) p[q] = z[q];
     T rz = dot(r, z, n_rows), rr = dot(r, r, n_rows), rz_old = 0;
     rr_hist.clear();
     rr_hist.push_back(double(rr));
+    rz_hist.assign(1, double(rz));
+    pap_hist.clear();
+    alpha_hist.clear();
+    beta_hist.clear();
     iters = 0;
     while (!(iters >= max_iter || done(rr))) {
       if (std::isnan(double(rr))) return -3;
+#pragma omp parallel for num_threads(threads) schedule(static)
       for (int q = 0; q < n_rows; ++q) Ap[q] = row_dot(A, q, p);
       T pAp = 0;
       for (int q = 0; q < n_rows; ++q) { T m = p[q] * Ap[q]; pAp = pAp + m; }
       T alpha = rz / pAp;
+      pap_hist.push_back(double(pAp));
+      alpha_hist.push_back(double(alpha));
       if (!(double(alpha) <= 1e5)) return -4;  // CHECK_LE(alpha, LARGE_FLOAT)
+#pragma omp parallel for num_threads(threads) schedule(static)
       for (int q = 0; q < n_cols; ++q) { T m = alpha * p[q]; x[q] = m + x[q]; }
+#pragma omp parallel for num_threads(threads) schedule(static)
       for (int q = 0; q < n_rows; ++q) { T m = -alpha * Ap[q]; r[q] = m + r[q]; }
       precondition();
       rr = dot(r, r, n_rows);
       rz_old = rz;
       rz = dot(r, z, n_rows);
       T beta = rz / rz_old;
+      rz_hist.push_back(double(rz));
+      beta_hist.push_back(double(beta));
       if (!(double(beta) <= 1e5)) return -5;  // CHECK_LE(beta, LARGE_FLOAT)
+#pragma omp parallel for num_threads(threads) schedule(static)
       for (int q = 0; q < n_rows; ++q) { T m = beta * p[q]; p[q] = z[q] + m; }
       ++iters;
       rr_hist.push_back(double(rr));
@@ -745,6 +820,7 @@ extern "C" {
 void *or_create(int nx, int ny, int nz) {
   Oracle *o = new Oracle();
   o->init(nx, ny, nz);
+  o->threads = omp_get_max_threads();
   return o;
 }
 void or_destroy(void *h) { delete static_cast<Oracle *>(h); }
@@ -841,6 +917,24 @@ int or_solver_sizes(void *h, int *n_rows, int *n_cols) {
   *n_rows = o->n_rows; *n_cols = o->n_cols;
   return 0;
 }
+// Scalar history of the last solve: which 0 r.r (after prepare and after
+// each iteration), 1 pAp, 2 alpha (per iteration), 3 r.z (after prepare and
+// after each iteration), 4 beta (per iteration).
+int or_scalar_history(void *h, int which, double *out, int cap) {
+  Oracle *o = static_cast<Oracle *>(h);
+  const std::vector<double> *v[5] = {&o->rr_hist, &o->pap_hist, &o->alpha_hist, &o->rz_hist,
+                                     &o->beta_hist};
+  if (which < 0 || which > 4) return -1;
+  const int m = int(v[which]->size());
+  for (int q = 0; q < m && q < cap; ++q) out[q] = (*v[which])[size_t(q)];
+  return m;
+}
+
+// Threads of the solver's parallel loops (1: everything sequential).
+void or_set_threads(void *h, int threads) {
+  static_cast<Oracle *>(h)->threads = threads < 1 ? 1 : threads;
+}
+
 int or_residual_history(void *h, double *out, int cap) {
   Oracle *o = static_cast<Oracle *>(h);
   int m = int(o->rr_hist.size());

@@ -15,6 +15,11 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 LIB64_PATH = os.path.join(HERE, "liboracle64.so")
+# Timed CPU-baseline builds, made on the machine that runs them
+# (`make baseline`, -march=native): same source, -O3 (bit-identical to the
+# checker) and -Ofast (the reference's flags).
+VARIANT_PATHS = {"o3": os.path.join(HERE, "liboracle_o3.so"),
+                 "ofast": os.path.join(HERE, "liboracle_ofast.so")}
 
 
 def build() -> str:
@@ -25,13 +30,20 @@ def build() -> str:
 _libs = {}
 
 
-def load(fp64=False):
+def build_baseline():
+    """Builds the -march=native baseline variants here (make baseline)."""
+    subprocess.run(["make", "-s", "-C", HERE, "baseline"], check=True)
+
+
+def load(fp64=False, variant=None):
     """The FP32 restatement (liboracle.so), or with fp64 the same source built
-    with double grids, particles and scalars (liboracle64.so)."""
-    if fp64 not in _libs:
-        path = LIB64_PATH if fp64 else LIB_PATH
+    with double grids, particles and scalars (liboracle64.so); variant "o3"
+    or "ofast" loads a baseline build (FP32)."""
+    key = variant or fp64
+    if key not in _libs:
+        path = VARIANT_PATHS[variant] if variant else (LIB64_PATH if fp64 else LIB_PATH)
         if not os.path.exists(path):
-            build()
+            build_baseline() if variant else build()
         lib = C.CDLL(path)
         P, I, D, I64 = C.c_void_p, C.c_int, C.c_double, C.c_int64
         F = C.c_double if fp64 else C.c_float
@@ -50,6 +62,7 @@ def load(fp64=False):
             "or_ic_factor": ([P, P, P, P, P], I),
             "or_solver_sizes": ([P, C.POINTER(I), C.POINTER(I)], I),
             "or_residual_history": ([P, P, I], I), "or_solution": ([P, P, I], I),
+            "or_scalar_history": ([P, I, P, I], I), "or_set_threads": ([P, I], None),
             "or_pressure_update": ([P], None), "or_velocity_update": ([P], I),
             "or_extrapolate": ([P], None), "or_g2p": ([P], None),
             "or_run_one_step": ([P, F, I, I, D, I, C.POINTER(I)], I),
@@ -63,8 +76,8 @@ def load(fp64=False):
             fn = getattr(lib, name)
             fn.argtypes = args
             fn.restype = res
-        _libs[fp64] = lib
-    return _libs[fp64]
+        _libs[key] = lib
+    return _libs[key]
 
 
 def _ptr(a):
@@ -74,13 +87,19 @@ def _ptr(a):
 class Oracle:
     """The reference's one-partition FLIP step on dense CPU grids."""
 
-    def __init__(self, dims, fp64=False):
-        self.lib = load(fp64)
+    def __init__(self, dims, fp64=False, variant=None, threads=None):
+        self.lib = load(fp64, variant)
         self.real = np.float64 if fp64 else np.float32
         self.creal = C.c_double if fp64 else C.c_float
         self.dims = tuple(int(d) for d in dims)
         self.box = tuple(d + 2 for d in self.dims)
         self.h = self.lib.or_create(*self.dims)
+        if threads is not None:
+            self.set_threads(threads)
+
+    def set_threads(self, threads):
+        """Threads of the solver's parallel loops (default: OpenMP's count)."""
+        self.lib.or_set_threads(self.h, int(threads))
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -178,6 +197,16 @@ class Oracle:
         out = np.empty(4096, np.float64)
         m = self.lib.or_residual_history(self.h, _ptr(out), out.size)
         return out[: min(m, out.size)].copy()
+
+    def scalar_history(self):
+        """Scalars of the last solve: dict of rr and rz (after prepare, then
+        after each iteration) and pAp, alpha, beta (one per iteration)."""
+        out = {}
+        for w, name in enumerate(("rr", "pAp", "alpha", "rz", "beta")):
+            buf = np.empty(20000, np.float64)
+            m = self.lib.or_scalar_history(self.h, w, _ptr(buf), buf.size)
+            out[name] = buf[: min(m, buf.size)].copy()
+        return out
 
     def solution(self):
         n, _ = self.solver_sizes()

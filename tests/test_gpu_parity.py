@@ -2,7 +2,8 @@
 
 Bars (from the projection core's parity contract):
   * labels, velocity-grid activity, per-cell particle counts, the fluid-leaf
-    index and positions after advection: bit-exact;
+    index and positions after advection: bit-exact, with the particles in the
+    same (the reference's traversal) order on both sides;
   * velocity, divergence, pressure: relative L2 <= 1e-5 (pressure from an
     FP64 solve to 1e-10 on both sides);
   * PCG iterations within +-2 of the oracle under the reference's rule, and
@@ -77,6 +78,8 @@ def run_stage_parity(n, mask, ppc, seed, steps=2, tank=False):
         gp, gvel, ggv = ctx.get_particles()
         op, ovel, ogv, _ = orc.get_particles()
         assert gp.shape == op.shape
+        # The device keeps the particles in the reference's traversal order.
+        assert np.array_equal(gp.view(np.uint32), op.view(np.uint32)), "particle order"
         gp, gvel = sort_particles(gp, gvel)
         op, ovel = sort_particles(op, ovel)
         assert np.array_equal(gp, op), "positions after advection"

@@ -18,6 +18,12 @@ computation or the published known answers of an algorithm it uses:
   advection (a4/5)  clamp rules and deletion in solids / outside the box
   extrapolation     two-hop corner value (order dependence) (a23)
   hydrostatics      a pool at rest stays at rest
+  free fall         a blob at rest ends its first step at (0, -g dt, 0)
+                    exactly (gravity, save / update, blend); with an initial
+                    velocity, at (u0, v0 - g dt, w0) to rounding
+  G2P blend         v + flip (dv + vel - v) on linear fields, two flips
+  boundary cond.    the zeroed faces of a hand-built solid configuration
+  velocity update   dv = v_new - v_saved bit for bit; the unsaved-face check
   block IC(0)       L L^T == A on A's block pattern; exact Cholesky on a chain;
                     scipy triangular solves; independent tiles
   fixtures          tests/golden/projection_systems.npz (make_golden.py): FP32
@@ -363,6 +369,179 @@ def test_hydrostatic_pool_stays_at_rest():
     # surface's sub-cell roughness.
     assert np.abs(v).max() < 0.2 * 9.8 / 30
     assert np.median(np.abs(v)) < 1e-3 * 9.8 / 30
+
+
+def free_fall_blob(vel0, flip=0.95):
+    """A cube of fluid (cells 12..19 on every axis, 8 particles per cell) in
+    a 32^3 box, far from every wall, after one step of dt = 1/30."""
+    n = (32, 32, 32)
+    mask = np.zeros(n, np.uint8)
+    mask[12:20, 12:20, 12:20] = 1
+    pos, vel = scenes.seed_particles(mask, 8, 5)
+    vel[:] = vel0
+    o = Oracle(n)
+    o.set_params(9.8, (0, 9.8, 0), flip)
+    o.set_box_boundary()
+    o.set_particles(pos, vel)
+    dt = np.float32(1.0 / 30)
+    rc, _ = o.run_one_step(dt, precision=1, stop_mode=1, threshold=1e-12, max_iter=2000)
+    assert rc == 0
+    return o, pos, dt
+
+
+def test_free_fall_at_rest_closed_form():
+    """AddGravity (flip_simulation.cc:505-515), SaveGridVelocities /
+    GetVelocityUpdate (:490-496, :780-801) and the FLIP blend (:1187): a blob
+    at rest, far from the walls, falls freely during its first step. P2G
+    gives 0 on every active face, gravity -g dt, the divergence of that
+    uniform field is 0 on every FLUID cell, so the pressure is 0, dv = -g dt,
+    and v + flip (dv + vel - v) = -g dt. Away from the blob's surface (where
+    the G2P stencil also reads inactive faces) every particle ends with
+    velocity (0, fl(0 - fl(dt g)), 0) exactly, at its old position. A flipped
+    gravity sign, a dropped dt, dv = v_new (no subtraction) with a
+    nonzero saved velocity (next test) or a misplaced flip term fail."""
+    o, pos0, dt = free_fall_blob((0.0, 0.0, 0.0))
+    p, v, gv, _ = o.get_particles()
+    assert p.shape == pos0.shape
+    g_dt = np.float32(0.0) - np.float32(dt * np.float32(9.8))
+    assert g_dt < 0
+    cell = np.floor(p).astype(int)
+    inner = np.all((cell >= 13) & (cell <= 18), axis=1)
+    assert inner.sum() == 6 ** 3 * 8
+    assert np.array_equal(v[:, 0], np.zeros(len(v), np.float32))
+    assert np.array_equal(v[:, 2], np.zeros(len(v), np.float32))
+    assert np.array_equal(v[inner, 1], np.full(inner.sum(), g_dt, np.float32))
+    assert np.array_equal(gv[inner, 1], np.full(inner.sum(), g_dt, np.float32))
+    # Particles sat still during the step's advection (grid velocity 0).
+    order_in = np.lexsort(pos0.T[::-1])
+    order_out = np.lexsort(p.T[::-1])
+    assert np.array_equal(p[order_out], pos0[order_in])
+    # Nothing anywhere moved upwards or sideways.
+    assert (v[:, 1] <= 0).all() and (v[:, 1] >= g_dt).all()
+
+
+def test_free_fall_with_initial_velocity_closed_form():
+    """The same blob moving at (u0, v0, w0): the saved velocity is
+    (u0, v0, w0), the step's update dv = v_new - v_saved = (0, -g dt, 0), and
+    the particles end at (u0, v0 - g dt, w0) to rounding (P2G's weighted
+    average of equal values rounds). With dv = v_new instead, the blend would
+    add 0.95 (u0, v0, w0) again; with PIC only (flip ignored) the result would
+    be the same here, so the next test pins the blend itself."""
+    u = np.array([0.7, -1.3, 0.4], np.float32)
+    o, pos0, dt = free_fall_blob(tuple(u))
+    p, v, _, _ = o.get_particles()
+    cell = np.floor(p).astype(int)
+    inner = np.all((cell >= 14) & (cell <= 17), axis=1)
+    assert inner.sum() > 0
+    expect = np.array([u[0], u[1] - dt * np.float32(9.8), u[2]], np.float64)
+    err = np.abs(v[inner].astype(np.float64) - expect).max()
+    assert err <= 1e-5, err
+    # The particles moved by u dt during the step's advection (grid velocity
+    # = initial velocity).
+    assert np.allclose(np.sort(p[:, 0]), np.sort(pos0[:, 0] + u[0] * dt), atol=1e-5)
+
+
+def test_g2p_flip_blend_closed_form():
+    """UpdateSingleParticleFromGrid's blend (flip_simulation.cc:1187):
+    vel' = v(x) + flip (dv(x) + vel - v(x)), with v and dv linear fields (the
+    trilinear interpolation reproduces them to rounding) and arbitrary
+    particle velocities; flip = 0.95 and flip = 0.3. A swapped sign on dv or
+    on v inside the bracket, or the blend weights exchanged, fail."""
+    n = (16, 16, 16)
+    rng = np.random.default_rng(7)
+    pos = (rng.random((300, 3)) * 12 + 2.0).astype(np.float32)
+    vel = rng.standard_normal((300, 3)).astype(np.float32)
+    ii = np.indices(tuple(x + 2 for x in n)).astype(np.float64) - B
+    cv = np.array([[0.5, 0.1, -0.2, 0.3], [-1.0, 0.05, 0.2, -0.1], [2.0, -0.3, 0.1, 0.15]])
+    cu = np.array([[-0.2, 0.02, 0.01, -0.03], [0.3, -0.01, 0.04, 0.0], [0.1, 0.0, -0.02, 0.05]])
+    fv = np.zeros(ii.shape[1:] + (3,))
+    fu = np.zeros(ii.shape[1:] + (3,))
+    for d in range(3):
+        loc = [ii[a] + (0.5 if a != d else 0.0) for a in range(3)]
+        fv[..., d] = cv[d, 0] + sum(cv[d, 1 + a] * loc[a] for a in range(3))
+        fu[..., d] = cu[d, 0] + sum(cu[d, 1 + a] * loc[a] for a in range(3))
+    act = np.ones(ii.shape[1:], np.uint8)
+    for flip in (0.95, 0.3):
+        o = Oracle(n)
+        o.set_params(9.8, (0, 9.8, 0), flip)
+        o.set_particles(pos, vel)
+        o.set_vec(0, fv.astype(np.float32), act)
+        o.set_vec(1, fu.astype(np.float32), act)
+        o.g2p()
+        p, v, _, _ = o.get_particles()
+        order_in = np.lexsort(pos.T[::-1])
+        order_out = np.lexsort(p.T[::-1])
+        x = p[order_out].astype(np.float64)
+        vx = cv[:, 0][None, :] + x @ cv[:, 1:].T
+        ux = cu[:, 0][None, :] + x @ cu[:, 1:].T
+        expect = vx + flip * (ux + vel[order_in].astype(np.float64) - vx)
+        assert np.abs(v[order_out] - expect).max() <= 3e-5, flip
+
+
+def test_boundary_conditions_hand_listed_faces():
+    """ApplyBoundaryConditions (flip_simulation.cc:656-713) on a hand-built
+    case: solid cells (3,3,3) and (4,3,3) inside an 8^3 box, faces active
+    (value 1 in every component) on the voxels of [2,4]^3 only. A solid cell
+    zeroes the normal component on each face it shares with a non-solid
+    neighbour: its lower faces are stored at the cell, its upper faces at
+    the neighbour (c + e_d); the face between the two solid cells is kept;
+    an inactive face (the upper x face of (4,3,3), stored at (5,3,3)) stays
+    inactive. The expected zeros are listed by hand."""
+    n = (8, 8, 8)
+    o = Oracle(n)
+    o.set_params(9.8, (0, 9.8, 0), 0.95)
+    o.set_box_boundary()
+    o.add_solid_box((3, 3, 3), (4, 3, 3))
+    box = tuple(x + 2 for x in n)
+    act = np.zeros(box, np.uint8)
+    act[2 + B:5 + B, 2 + B:5 + B, 2 + B:5 + B] = 1
+    vals = np.where(act[..., None] == 1, 1.0, 0.0).astype(np.float32) * np.ones(3, np.float32)
+    o.set_vec(0, vals, act)
+    o.apply_bc()
+    v, a = o.get_vec(0)
+    assert np.array_equal(a, act)
+    zeros = {((3, 3, 3), 0), ((3, 3, 3), 1), ((3, 3, 3), 2),  # lower faces of (3,3,3)
+             ((3, 4, 3), 1), ((3, 3, 4), 2),                  # its upper y and z faces
+             ((4, 3, 3), 1), ((4, 3, 3), 2),                  # lower y, z faces of (4,3,3)
+             ((4, 4, 3), 1), ((4, 3, 4), 2)}                  # its upper y and z faces
+    for i in range(2, 5):
+        for j in range(2, 5):
+            for k in range(2, 5):
+                for d in range(3):
+                    want = 0.0 if ((i, j, k), d) in zeros else 1.0
+                    assert v[i + B, j + B, k + B, d] == want, ((i, j, k), d)
+    m = o.get_int(0)
+    assert m[3 + B, 3 + B, 3 + B] == 2 and m[4 + B, 3 + B, 3 + B] == 2
+    assert m[2 + B, 3 + B, 3 + B] != 2
+
+
+def test_velocity_update_is_new_minus_saved():
+    """SaveGridVelocities then GetVelocityUpdate (flip_simulation.cc:490-496,
+    780-801): dv = v_new - v_saved on every face active in both, one FP32
+    subtraction each (so bit-equal to numpy's); a face active in v but never
+    saved is the reference's CHECK failure."""
+    n = (8, 8, 8)
+    rng = np.random.default_rng(3)
+    box = tuple(x + 2 for x in n)
+    act = (rng.random(box) < 0.7).astype(np.uint8)
+    act[0] = act[:, 0] = act[:, :, 0] = 0  # the update loops over [0, n]
+    v1 = rng.standard_normal(box + (3,)).astype(np.float32) * act[..., None]
+    v2 = rng.standard_normal(box + (3,)).astype(np.float32) * act[..., None]
+    o = Oracle(n)
+    o.set_vec(0, v1, act)
+    o.save_velocities()
+    o.set_vec(0, v2, act)
+    assert o.velocity_update() == 0
+    du, da = o.get_vec(1)
+    on = act == 1
+    assert np.array_equal(da, act)
+    assert np.array_equal(du[on].view(np.uint32), (v2[on] - v1[on]).view(np.uint32))
+    # A face active in v but not saved.
+    extra = act.copy()
+    idx = tuple(np.argwhere(act[1:, 1:, 1:] == 0)[0] + 1)
+    extra[idx] = 1
+    o.set_vec(0, v2, extra)
+    assert o.velocity_update() != 0
 
 
 def test_sealed_pocket_and_enclosed_cell_errors():
