@@ -520,6 +520,7 @@ extern "C" {
 
 int fsl_rollout_begin(fsl_ctx *ctx, int max_substeps) {
   if (!ctx || max_substeps < 1) return FSL_ERR_ARG;
+  if (ctx->counts_on_device && fsl::host_counts(ctx)) return FSL_ERR_HIP;
   int rc;
   if ((rc = fsl::ensure_state(ctx))) return rc;
   if ((rc = fsl::free_checkpoints(ctx))) return rc;
@@ -535,6 +536,7 @@ int fsl_rollout_begin(fsl_ctx *ctx, int max_substeps) {
 int fsl_rollout_step(fsl_ctx *ctx, float dt, const fsl_solve_params *params, const float *u0_box,
                      float *u_post_box, fsl_solve_stats *stats) {
   if (!ctx || !params) return FSL_ERR_ARG;
+  if (ctx->counts_on_device && fsl::host_counts(ctx)) return FSL_ERR_HIP;
   if (!ctx->adj || ctx->adj->ck.empty()) {
     ctx->err = "fsl_rollout_begin first";
     return FSL_ERR_STATE;
@@ -565,6 +567,7 @@ int fsl_rollout_step(fsl_ctx *ctx, float dt, const fsl_solve_params *params, con
 int fsl_rollout_replay(fsl_ctx *ctx, const fsl_solve_params *params, const float *u0_box,
                        const float force[3], float *u_post_box, fsl_solve_stats *stats) {
   if (!ctx || !params || !force) return FSL_ERR_ARG;
+  if (ctx->counts_on_device && fsl::host_counts(ctx)) return FSL_ERR_HIP;
   if (!ctx->adj || ctx->adj->steps == 0) {
     ctx->err = "no recorded rollout";
     return FSL_ERR_STATE;
@@ -603,6 +606,7 @@ int fsl_rollout_replay(fsl_ctx *ctx, const fsl_solve_params *params, const float
 int fsl_rollout_backward(fsl_ctx *ctx, const fsl_solve_params *params, const float *loss_grad,
                          float *grad_u0, double grad_force[3], fsl_solve_stats *stats) {
   if (!ctx || !params || !loss_grad) return FSL_ERR_ARG;
+  if (ctx->counts_on_device && fsl::host_counts(ctx)) return FSL_ERR_HIP;
   if (!ctx->adj || ctx->adj->steps == 0) {
     ctx->err = "no recorded rollout";
     return FSL_ERR_STATE;
@@ -687,6 +691,7 @@ int fsl_rollout_backward(fsl_ctx *ctx, const fsl_solve_params *params, const flo
 int fsl_rollout_checkpoint(fsl_ctx *ctx, int k, float *pos, float *vel, uint32_t *perm,
                            int64_t *n, int64_t *n_prev, float *dt) {
   if (!ctx || !ctx->adj || k < 0 || k >= ctx->adj->steps) return FSL_ERR_ARG;
+  if (ctx->counts_on_device && fsl::host_counts(ctx)) return FSL_ERR_HIP;
   const Checkpoint &c = ctx->adj->ck[k];
   if (n) *n = c.n;
   if (n_prev) *n_prev = c.n_prev;

@@ -10,11 +10,35 @@
 
 using fsl::Dom;
 
+namespace fsl {
+
+// Reads the device-resident counts back: the exact particle count and
+// leaf-list lengths replace the bounds a device-resident step left.
+int host_counts(fsl_ctx *ctx) {
+  if (!ctx->counts_on_device) return FSL_OK;
+  FSL_HIP(hipMemcpyAsync(ctx->host_u32, ctx->dcount, 3 * sizeof(uint32_t),
+                         hipMemcpyDeviceToHost, ctx->stream));
+  FSL_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->np = ctx->host_u32[kCountParticles];
+  ctx->n_act = ctx->host_u32[kCountActive];
+  ctx->n_fluid = ctx->host_u32[kCountFluid];
+  ctx->counts_on_device = false;
+  return FSL_OK;
+}
+
+}  // namespace fsl
+
 namespace {
 
-#define CHECK_CTX()                        \
-  do {                                     \
-    if (ctx == nullptr) return FSL_ERR_ARG; \
+// Every entry point except the device-resident step works with the counts
+// on the host: after a device-resident step they are read back first.
+#define CHECK_CTX()                                    \
+  do {                                                 \
+    if (ctx == nullptr) return FSL_ERR_ARG;            \
+    if (ctx->counts_on_device) {                       \
+      const int rc_ = fsl::host_counts(ctx);           \
+      if (rc_) return rc_;                             \
+    }                                                  \
   } while (0)
 
 template <typename T>
@@ -49,7 +73,7 @@ int sync_flags(fsl_ctx *ctx) {
 // sort by cell.
 int sort_particles(fsl_ctx *ctx, float dt, bool drop_outside, bool keep_gv,
                    const fsl::KillBoxes &kill = fsl::KillBoxes()) {
-  FSL_HIP(fsl::launch_advect(ctx, dt, drop_outside, kill));
+  FSL_HIP(fsl::launch_advect(ctx, dt, nullptr, drop_outside, kill));
   FSL_HIP(fsl::exclusive_scan(ctx->vcount, ctx->vstart, ctx->nvox, ctx->scan_tmp, ctx->stream));
   uint32_t alive = 0;
   int rc = read_u32(ctx, ctx->vstart + ctx->nvox, &alive);
@@ -64,9 +88,134 @@ int all_leaves(fsl_ctx *ctx) {
   if (ctx->n_act != ctx->nleaves) {
     FSL_HIP(fsl::launch_all_leaves(ctx));
     ctx->n_act = ctx->nleaves;
-    FSL_HIP(fsl::launch_clear_leaves(ctx, ctx->act_list, ctx->n_act));
+    FSL_HIP(fsl::launch_clear_leaves(ctx, fsl::act_leaves(ctx)));
   }
   ctx->grid_ready = true;
+  return FSL_OK;
+}
+
+// The velocity update's consistency flag of the last step (copied to the
+// host at the step's end): FSL_ERR_VELOCITY_UPDATE when it is set.
+int check_step_flags(fsl_ctx *ctx) {
+  FSL_HIP(hipStreamSynchronize(ctx->stream));
+  if (ctx->host_u32[16]) {
+    ctx->err = "face active in grid_velocity_ but not in grid_velocity_update_";
+    return FSL_ERR_VELOCITY_UPDATE;
+  }
+  return FSL_OK;
+}
+
+// FlipDriver::RunOneStep (flip_app_sim.cc:211-254) with the counts on the
+// device: particle count, active leaves and FLUID leaves stay on the device
+// from the advection to the divergence, and the host reads them (with the
+// solver flags and the device dt) once, after the divergence, to size the
+// solve. dt_dev, when set, holds dt on the device (fsl_advance_frame). The
+// velocity update's flag is copied to host_u32[16] and checked by the
+// caller (check_step_flags) once the stream has drained.
+int run_step(fsl_ctx *ctx, float dt, const float *dt_dev, const fsl_solve_params *params,
+             fsl_solve_stats *stats) {
+  int rc;
+  auto mark = [&](int ev) -> int {
+    if (ctx->timing) FSL_HIP(hipEventRecord(ctx->tev[ev], ctx->stream));
+    return FSL_OK;
+  };
+#define STAGE(s, body)                     \
+  if ((rc = mark(2 * (s)))) return rc;     \
+  body;                                    \
+  if ((rc = mark(2 * (s) + 1))) return rc;
+  // The host's counts go to the device; from here on they are bounds.
+  ctx->host_u32[8 + fsl::kCountParticles] = uint32_t(ctx->np);
+  ctx->host_u32[8 + fsl::kCountActive] = uint32_t(ctx->n_act);
+  ctx->host_u32[8 + fsl::kCountFluid] = uint32_t(ctx->n_fluid);
+  FSL_HIP(hipMemcpyAsync(ctx->dcount, ctx->host_u32 + 8, 3 * sizeof(uint32_t),
+                         hipMemcpyHostToDevice, ctx->stream));
+  ctx->counts_on_device = true;
+  // MoveParticles + DeleteParticlesInSolids. Within a step nothing reads the
+  // particles' grid velocities between the move and G2P, which rewrites
+  // them: the sort leaves them behind.
+  STAGE(FSL_T_MOVE, {
+    FSL_HIP(fsl::launch_advect(ctx, dt, dt_dev, ctx->outside_deleted, fsl::KillBoxes()));
+    FSL_HIP(fsl::exclusive_scan(ctx->vcount, ctx->vstart, ctx->nvox, ctx->scan_tmp,
+                                ctx->stream));
+    FSL_HIP(fsl::launch_sort_tail(ctx, 0, false));
+    ctx->outside_deleted = false;
+  })
+  // ClearGridData: the old list's leaves, then the new active list.
+  STAGE(FSL_T_CLEAR, {
+    FSL_HIP(fsl::launch_clear_shell(ctx));
+    if (ctx->n_act > 0) FSL_HIP(fsl::launch_clear_leaves(ctx, fsl::act_leaves(ctx)));
+    FSL_HIP(fsl::launch_leaf_flags(ctx));
+    ctx->n_act = ctx->nleaves;
+    ctx->n_fluid = ctx->nleaves;
+    ctx->grid_ready = true;
+    ctx->pcg_ready = false;
+  })
+  STAGE(FSL_T_P2G, FSL_HIP(fsl::launch_p2g(ctx)))
+  // ApplyBoundaryConditions + SaveGridVelocities in one pass.
+  STAGE(FSL_T_BC_SAVE, FSL_HIP(fsl::launch_bc_save(ctx)); ctx->bc_applied = true)
+  ctx->outside_deleted = true;  // DeleteOutsideParticles: applied by the next sort
+  STAGE(FSL_T_GRAVITY_BC, FSL_HIP(fsl::launch_gravity(ctx, dt, dt_dev));
+        FSL_HIP(fsl::launch_bc(ctx)))
+  STAGE(FSL_T_DIVERGENCE, {
+    FSL_HIP(fsl::launch_fill_u32(ctx, ctx->lfluid, ctx->nleaves, 0u));
+    FSL_HIP(fsl::launch_fill_u32(ctx, reinterpret_cast<uint32_t *>(ctx->flags),
+                                 int64_t(sizeof(fsl::DevFlags) / sizeof(uint32_t)), 0u));
+    FSL_HIP(fsl::launch_divergence(ctx));
+    FSL_HIP(fsl::launch_compact(ctx, ctx->lfluid, ctx->fluid_list,
+                                ctx->dcount + fsl::kCountFluid));
+    // The counts, the flags and the step state in one round trip.
+    FSL_HIP(hipMemcpyAsync(ctx->host_u32, ctx->dcount, 3 * sizeof(uint32_t),
+                           hipMemcpyDeviceToHost, ctx->stream));
+    FSL_HIP(hipMemcpyAsync(ctx->flags_host, ctx->flags, sizeof(fsl::DevFlags),
+                           hipMemcpyDeviceToHost, ctx->stream));
+    FSL_HIP(hipMemcpyAsync(ctx->sstate_host, ctx->sstate, sizeof(fsl::StepState),
+                           hipMemcpyDeviceToHost, ctx->stream));
+    if (!ctx->ev_counts) FSL_HIP(hipEventCreateWithFlags(&ctx->ev_counts, hipEventDisableTiming));
+    FSL_HIP(hipEventRecord(ctx->ev_counts, ctx->stream));
+    FSL_HIP(hipEventSynchronize(ctx->ev_counts));
+    ctx->np = ctx->host_u32[fsl::kCountParticles];
+    ctx->n_act = ctx->host_u32[fsl::kCountActive];
+    ctx->n_fluid = ctx->host_u32[fsl::kCountFluid];
+    ctx->counts_on_device = false;
+  })
+  if (ctx->flags_host->zero_diag) {
+    ctx->err = "FLUID cell with no non-SOLID neighbour (zero Jacobi diagonal)";
+    return FSL_ERR_ZERO_DIAGONAL;
+  }
+  fsl_solve_stats local{};
+  fsl_solve_stats *st = stats ? stats : &local;
+  *st = fsl_solve_stats{};
+  st->n_rows = int(ctx->flags_host->n_rows);
+  st->n_cols = st->n_rows;
+  STAGE(FSL_T_SOLVE, {
+    rc = fsl::pcg_solve_iterations(ctx, params, st);
+    if (rc) {
+      if (ctx->err.empty()) ctx->err = "solver failed";
+      st->iterations = ctx->pcg_host->it;
+      return rc;
+    }
+    // CopySolution, and the true residual for the statistics: read back
+    // while the rest of the step runs.
+    if ((rc = fsl::pcg_finish_enqueue(ctx))) return rc;
+  })
+  // The velocity update's consistency flag is read once, at the end.
+  STAGE(FSL_T_UPDATE, FSL_HIP(fsl::launch_pressure_update(ctx));
+        FSL_HIP(fsl::launch_fill_u32(ctx, &ctx->flags->vel_update, 1, 0u));
+        FSL_HIP(fsl::launch_velocity_update(ctx)))
+  STAGE(FSL_T_G2P, FSL_HIP(fsl::launch_extrapolate(ctx)); FSL_HIP(fsl::launch_g2p(ctx)))
+#undef STAGE
+  FSL_HIP(hipMemcpyAsync(ctx->host_u32 + 16, &ctx->flags->vel_update, sizeof(uint32_t),
+                         hipMemcpyDeviceToHost, ctx->stream));
+  if ((rc = fsl::pcg_finish_collect(ctx, st))) return rc;
+  if (ctx->timing) {
+    FSL_HIP(hipStreamSynchronize(ctx->stream));
+    for (int s = FSL_T_MOVE; s <= FSL_T_G2P; ++s) {
+      float ms = 0.0f;
+      FSL_HIP(hipEventElapsedTime(&ms, ctx->tev[2 * s], ctx->tev[2 * s + 1]));
+      ctx->tacc[s] += ms;
+    }
+    ctx->tacc[FSL_T_STEPS] += 1;
+  }
   return FSL_OK;
 }
 
@@ -194,6 +343,9 @@ int fsl_create(fsl_ctx **out, const int dims[3], const int tiles[3], int device,
   ctx->scan_tmp_len = fsl::scan_tmp_elems(std::max<int64_t>(ctx->nvox, ncell));
   if (!rc) rc = dev_alloc(ctx, &ctx->scan_tmp, size_t(ctx->scan_tmp_len));
   if (!rc) rc = dev_alloc(ctx, &ctx->flags, 1);
+  if (!rc) rc = dev_alloc(ctx, &ctx->dcount, 4);
+  if (!rc) rc = dev_alloc(ctx, &ctx->sstate, 1);
+  if (!rc && hipHostMalloc(reinterpret_cast<void **>(&ctx->sstate_host), sizeof(fsl::StepState)) != hipSuccess) rc = FSL_ERR_HIP;
   if (!rc && hipHostMalloc(reinterpret_cast<void **>(&ctx->flags_host), sizeof(fsl::DevFlags)) != hipSuccess) rc = FSL_ERR_HIP;
   if (!rc && hipHostMalloc(reinterpret_cast<void **>(&ctx->pcg_host), 3 * sizeof(fsl::PcgState)) != hipSuccess) rc = FSL_ERR_HIP;
   if (!rc && hipHostMalloc(reinterpret_cast<void **>(&ctx->host_u32), 64 * sizeof(uint32_t)) != hipSuccess) rc = FSL_ERR_HIP;
@@ -216,7 +368,8 @@ int fsl_destroy(fsl_ctx *ctx) {
                   ctx->lact, ctx->lfluid, ctx->lscan, ctx->act_list, ctx->fluid_list, ctx->morton,
                   ctx->lrank,
                   ctx->lflag_m, ctx->key, ctx->skey, ctx->p2g_w, ctx->p2g_ring,
-                  ctx->perm, ctx->vcount, ctx->vstart, ctx->scan_tmp, ctx->flags};
+                  ctx->perm, ctx->vcount, ctx->vstart, ctx->scan_tmp, ctx->flags, ctx->dcount,
+                  ctx->sstate};
   for (void *b : bufs) if (b) (void)hipFree(b);
   for (int a = 0; a < 9; ++a) {
     if (ctx->P.a[a]) (void)hipFree(ctx->P.a[a]);
@@ -224,8 +377,11 @@ int fsl_destroy(fsl_ctx *ctx) {
   }
   if (ctx->src_buf) (void)hipFree(ctx->src_buf);
   for (hipEvent_t e : ctx->tev) if (e) (void)hipEventDestroy(e);
+  if (ctx->ev_finish) (void)hipEventDestroy(ctx->ev_finish);
+  if (ctx->ev_counts) (void)hipEventDestroy(ctx->ev_counts);
   if (ctx->flags_host) (void)hipHostFree(ctx->flags_host);
   if (ctx->pcg_host) (void)hipHostFree(ctx->pcg_host);
+  if (ctx->sstate_host) (void)hipHostFree(ctx->sstate_host);
   if (ctx->host_u32) (void)hipHostFree(ctx->host_u32);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -470,7 +626,7 @@ int fsl_move_particles(fsl_ctx *ctx, float dt) {
 int fsl_clear(fsl_ctx *ctx) {
   CHECK_CTX();
   FSL_HIP(fsl::launch_clear_shell(ctx));
-  if (ctx->n_act > 0) FSL_HIP(fsl::launch_clear_leaves(ctx, ctx->act_list, ctx->n_act));
+  if (ctx->n_act > 0) FSL_HIP(fsl::launch_clear_leaves(ctx, fsl::act_leaves(ctx)));
   FSL_HIP(fsl::launch_leaf_flags(ctx));
   uint32_t n_act = 0;
   int rc = read_u32(ctx, ctx->lscan + ctx->nleaves, &n_act);
@@ -529,7 +685,7 @@ int fsl_divergence(fsl_ctx *ctx) {
   FSL_HIP(hipMemsetAsync(ctx->lfluid, 0, size_t(ctx->nleaves) * sizeof(uint32_t), ctx->stream));
   FSL_HIP(hipMemsetAsync(ctx->flags, 0, sizeof(fsl::DevFlags), ctx->stream));
   FSL_HIP(fsl::launch_divergence(ctx));
-  FSL_HIP(fsl::launch_compact(ctx, ctx->lfluid, ctx->fluid_list));
+  FSL_HIP(fsl::launch_compact(ctx, ctx->lfluid, ctx->fluid_list, ctx->dcount + fsl::kCountFluid));
   // The fluid-leaf count and the flags in one round trip.
   FSL_HIP(hipMemcpyAsync(ctx->host_u32, ctx->lscan + ctx->nleaves, sizeof(uint32_t),
                          hipMemcpyDeviceToHost, ctx->stream));
@@ -606,50 +762,46 @@ int fsl_g2p(fsl_ctx *ctx) {
 int fsl_step(fsl_ctx *ctx, float dt, const fsl_solve_params *params, fsl_solve_stats *stats) {
   CHECK_CTX();
   if (!params) return FSL_ERR_ARG;
+  int rc = run_step(ctx, dt, nullptr, params, stats);
+  if (rc) return rc;
+  return check_step_flags(ctx);
+}
+
+int fsl_advance_frame(fsl_ctx *ctx, int cfl, float frame_step, const fsl_solve_params *params,
+                      fsl_frame_stats *stats) {
+  CHECK_CTX();
+  if (!params || !(frame_step > 0.0f) || cfl <= 0) return FSL_ERR_ARG;
+  fsl_frame_stats local{};
+  fsl_frame_stats *fs = stats ? stats : &local;
+  *fs = fsl_frame_stats{};
+  // WhileAndComputeDt's constants, in the host's float/double arithmetic
+  // (fsl_compute_dt).
+  fsl::DtParams p;
+  p.v_gravity = std::sqrt(0.5 * double(ctx->gravity));
+  p.snap = 0.9 * double(frame_step);
+  p.cfl = float(cfl);
+  p.frame_step = frame_step;
+  p.dt_max = float(1.0 / double(frame_step));
+  p.dt_min = frame_step / 10;
   int rc;
-  auto mark = [&](int ev) -> int {
-    if (ctx->timing) FSL_HIP(hipEventRecord(ctx->tev[ev], ctx->stream));
-    return FSL_OK;
-  };
-#define STAGE(s, body)                     \
-  if ((rc = mark(2 * (s)))) return rc;     \
-  body;                                    \
-  if ((rc = mark(2 * (s) + 1))) return rc;
-  // Within a step nothing reads the particles' grid velocities between the
-  // move and G2P, which rewrites them: the sort leaves them behind.
-  STAGE(FSL_T_MOVE, if ((rc = sort_particles(ctx, dt, ctx->outside_deleted, false))) return rc)
-  STAGE(FSL_T_CLEAR, if ((rc = fsl_clear(ctx))) return rc)
-  STAGE(FSL_T_P2G, if ((rc = fsl_p2g(ctx))) return rc)
-  // ApplyBoundaryConditions + SaveGridVelocities in one pass.
-  STAGE(FSL_T_BC_SAVE, FSL_HIP(fsl::launch_bc_save(ctx)); ctx->bc_applied = true)
-  if ((rc = fsl_delete_outside(ctx))) return rc;
-  STAGE(FSL_T_GRAVITY_BC, if ((rc = fsl_add_gravity(ctx, dt))) return rc;
-        if ((rc = fsl_apply_bc(ctx))) return rc)
-  STAGE(FSL_T_DIVERGENCE, if ((rc = fsl_divergence(ctx))) return rc)
-  STAGE(FSL_T_SOLVE, if ((rc = fsl_pcg_solve(ctx, params, stats))) return rc)
-  // The velocity update's consistency flag is read once, after G2P, so the
-  // update, the extrapolation and G2P are enqueued without a host round trip.
-  STAGE(FSL_T_UPDATE, if ((rc = fsl_pressure_update(ctx))) return rc;
-        FSL_HIP(hipMemsetAsync(&ctx->flags->vel_update, 0, sizeof(unsigned), ctx->stream));
-        FSL_HIP(fsl::launch_velocity_update(ctx)))
-  STAGE(FSL_T_G2P, FSL_HIP(fsl::launch_extrapolate(ctx)); FSL_HIP(fsl::launch_g2p(ctx)))
-#undef STAGE
-  uint32_t bad = 0;
-  if ((rc = read_u32(ctx, &ctx->flags->vel_update, &bad))) return rc;
-  if (bad) {
-    ctx->err = "face active in grid_velocity_ but not in grid_velocity_update_";
-    return FSL_ERR_VELOCITY_UPDATE;
-  }
-  if (ctx->timing) {
-    FSL_HIP(hipStreamSynchronize(ctx->stream));
-    for (int s = FSL_T_MOVE; s <= FSL_T_G2P; ++s) {
-      float ms = 0.0f;
-      FSL_HIP(hipEventElapsedTime(&ms, ctx->tev[2 * s], ctx->tev[2 * s + 1]));
-      ctx->tacc[s] += ms;
+  for (int s = 0;; ++s) {
+    if (s >= 100000) {
+      ctx->err = "frame did not end";
+      return FSL_ERR_STATE;
     }
-    ctx->tacc[FSL_T_STEPS] += 1;
+    // CFL of the current grid velocity and dt on the device.
+    FSL_HIP(fsl::launch_fill_u32(ctx, &ctx->flags->cfl_bits, 1, 0u));
+    if (ctx->grid_ready && ctx->n_act > 0) FSL_HIP(fsl::launch_cfl(ctx));
+    FSL_HIP(fsl::launch_dt(ctx, p, s == 0 ? 1 : 0));
+    fsl_solve_stats st{};
+    if ((rc = run_step(ctx, 0.0f, &ctx->sstate->dt, params, &st))) return rc;
+    fs->steps += 1;
+    fs->pcg_iterations += st.iterations;
+    fs->dt_last = ctx->sstate_host->dt;
+    fs->last_solve = st;
+    if (!ctx->sstate_host->more) break;
   }
-  return FSL_OK;
+  return check_step_flags(ctx);
 }
 
 int fsl_set_timing(fsl_ctx *ctx, int enable) {

@@ -22,6 +22,31 @@ __device__ inline int64_t xcd_work(int64_t nwork) {
   return int64_t(blockIdx.x % kXcd) * chunk + blockIdx.x / kXcd;
 }
 
+// XCD-aware walk over a leaf list of n entries (n read from *count when
+// count is set): workgroup b runs on XCD x = b % 8 and takes entries
+// x * chunk + j, j = b / 8, b / 8 + G / 8, ... of the XCD's contiguous chunk
+// (chunk = ceil(n / 8), G the grid size). A grid of 8 * ceil(n / 8)
+// workgroups gives each one entry (as xcd_work); a smaller grid makes the
+// workgroups persistent. f(leaf) runs once per entry; every thread of the
+// workgroup takes the same entries, so f may synchronise the workgroup.
+template <typename F>
+__device__ inline void for_each_leaf(const int32_t *list, const uint32_t *count, int64_t n,
+                                     F &&f) {
+  if (count) n = int64_t(*count);
+  const int64_t chunk = (n + kXcd - 1) / kXcd;
+  const int64_t beg = int64_t(blockIdx.x % kXcd) * chunk;
+  const int64_t end = beg + chunk < n ? beg + chunk : n;
+  const int64_t per = gridDim.x / kXcd;
+  for (int64_t w = beg + blockIdx.x / kXcd; w < end; w += per) f(int64_t(list[w]));
+}
+// Workgroups for a walk over a leaf list: one per entry for a host-known
+// count, at most kLeafGridCap (persistent) for a device count.
+constexpr int kLeafGridCap = kXcd * 1024;
+inline int leaf_grid(int64_t n, bool device_count) {
+  const int g = xcd_grid(n);
+  return device_count && g > kLeafGridCap ? kLeafGridCap : g;
+}
+
 // Fused multiply-add in the operands' own precision (__builtin_fma is the
 // double builtin: float operands would be promoted, multiplied and added in
 // double, then rounded back).

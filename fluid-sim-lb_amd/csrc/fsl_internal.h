@@ -124,6 +124,22 @@ struct SourceBox {
   uint32_t k0, k1;
 };
 
+// Time step on the device (fsl_advance_frame): the frame's elapsed time and
+// the dt of the current step, from the CFL maximum of the grid velocity.
+struct StepState {
+  float dt;       // dt of the current step
+  float step;     // time elapsed in the frame after the current step
+  int more;       // another step follows in this frame
+  int pad;
+};
+// Constants of WhileAndComputeDt (flip_app_sim.cc:357-410) for k_dt, formed
+// on the host in the reference's float/double arithmetic.
+struct DtParams {
+  double v_gravity;  // sqrt(0.5 * gravity) (flip_simulation.cc:485)
+  double snap;       // 0.9 * frame_step
+  float cfl, frame_step, dt_max, dt_min;  // dt_max = 1 / frame_step, dt_min = frame_step / 10
+};
+
 struct DevFlags {
   unsigned zero_diag;
   unsigned vel_update;
@@ -136,6 +152,21 @@ struct DevFlags {
 struct Particles {
   float *a[9];  // px py pz vx vy vz gx gy gz
 };
+
+// A leaf list on the device, list[0, n). When `count` is set the length is
+// the value there (written on the device by the compaction that built the
+// list) and n is only an upper bound of it, used to size launches, so work
+// can be enqueued before the host knows the length.
+struct LeafList {
+  const int32_t *list;
+  const uint32_t *count;
+  int64_t n;
+};
+
+// Device-resident counts (fsl_ctx::dcount).
+constexpr int kCountParticles = 0;  // particles alive after the last sort
+constexpr int kCountActive = 1;     // entries of act_list
+constexpr int kCountFluid = 2;      // entries of fluid_list
 
 }  // namespace fsl
 
@@ -179,6 +210,14 @@ struct fsl_ctx {
   int32_t *lrank = nullptr;    // leaf -> rank in VDB traversal order (dom.lrank)
   uint32_t *lflag_m = nullptr; // flags gathered in Morton order (scan input)
   int64_t n_act = 0, n_fluid = 0;
+  // Device-resident step (fsl_advance_frame): the counts are kept on the
+  // device (dcount, kCount*), and n_act, n_fluid and np are then upper
+  // bounds until the host reads the counts back (sync_counts).
+  uint32_t *dcount = nullptr;
+  bool counts_on_device = false;
+  fsl::StepState *sstate = nullptr, *sstate_host = nullptr;  // device and pinned mirror
+  hipEvent_t ev_finish = nullptr;  // the solver state's copy after the solve's finish
+  hipEvent_t ev_counts = nullptr;  // the counts' copy within a device-resident step
   bool bc_applied = false, grid_ready = false, outside_deleted = false;
   // Particles.
   int64_t cap = 0, np = 0;
@@ -208,6 +247,18 @@ struct fsl_ctx {
 
 namespace fsl {
 
+// api.hip: reads the device-resident counts back (after a device-resident
+// step), so the host's n_act, n_fluid and np are exact again.
+int host_counts(fsl_ctx *ctx);
+
+// The active and FLUID leaf lists as the kernels take them.
+inline LeafList act_leaves(const fsl_ctx *c) {
+  return {c->act_list, c->counts_on_device ? c->dcount + kCountActive : nullptr, c->n_act};
+}
+inline LeafList fluid_leaves(const fsl_ctx *c) {
+  return {c->fluid_list, c->counts_on_device ? c->dcount + kCountFluid : nullptr, c->n_fluid};
+}
+
 // scan.hip
 int64_t scan_tmp_elems(int64_t n);
 hipError_t exclusive_scan(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *tmp,
@@ -216,14 +267,15 @@ hipError_t exclusive_scan(const uint32_t *in, uint32_t *out, int64_t n, uint32_t
 // particles.hip
 hipError_t launch_seed(fsl_ctx *c, const uint8_t *mask_dev, const uint32_t *offs, int ppc,
                        uint32_t seed, const float vel[3]);
-hipError_t launch_advect(fsl_ctx *c, float dt, bool drop_outside, const KillBoxes &kill);
+hipError_t launch_advect(fsl_ctx *c, float dt, const float *dt_dev, bool drop_outside,
+                         const KillBoxes &kill);
 hipError_t launch_source_count(fsl_ctx *c, const SourceBox &s, int64_t ncell, uint32_t *cnt);
 hipError_t launch_source_emit(fsl_ctx *c, const SourceBox &s, int64_t ncell, const uint32_t *off,
                               int64_t base);
 hipError_t launch_sort_tail(fsl_ctx *c, uint32_t n_alive, bool keep_gv = true);
 hipError_t launch_mask_count(fsl_ctx *c, const uint8_t *mask_dev, int64_t n, uint32_t *cnt);
 hipError_t launch_leaf_flags(fsl_ctx *c);
-hipError_t launch_compact(fsl_ctx *c, const uint32_t *flag, int32_t *list);
+hipError_t launch_compact(fsl_ctx *c, const uint32_t *flag, int32_t *list, uint32_t *count);
 size_t p2g_ring_bytes(int64_t nleaves);
 hipError_t launch_p2g(fsl_ctx *c, float *const wsum[3] = nullptr);
 hipError_t launch_g2p_transpose(fsl_ctx *c, float *const pval[3], float *const out[3]);
@@ -233,17 +285,20 @@ hipError_t launch_g2p(fsl_ctx *c);
 hipError_t launch_cell_counts_box(fsl_ctx *c, int32_t *out_dev);
 
 // grid.hip
-hipError_t launch_clear_leaves(fsl_ctx *c, const int32_t *list, int64_t n);
+hipError_t launch_clear_leaves(fsl_ctx *c, const LeafList &L);
 hipError_t launch_bc(fsl_ctx *c);
 hipError_t launch_save(fsl_ctx *c);
 hipError_t launch_bc_save(fsl_ctx *c);
-hipError_t launch_gravity(fsl_ctx *c, float dt);
+hipError_t launch_gravity(fsl_ctx *c, float dt, const float *dt_dev = nullptr);
 hipError_t launch_divergence(fsl_ctx *c);
 hipError_t launch_pressure_update(fsl_ctx *c);
 hipError_t launch_velocity_update(fsl_ctx *c);
 hipError_t launch_extrapolate(fsl_ctx *c);
 hipError_t launch_clear_shell(fsl_ctx *c);
 hipError_t launch_cfl(fsl_ctx *c);
+// dt of the next step on the device, from the CFL maximum in flags->cfl_bits.
+hipError_t launch_dt(fsl_ctx *c, const DtParams &p, int reset_frame);
+hipError_t launch_fill_u32(fsl_ctx *c, uint32_t *p, int64_t n, uint32_t value);
 hipError_t launch_box_vec(fsl_ctx *c, int which, float *out3, uint8_t *act, bool to_box);
 hipError_t launch_box_scalar(fsl_ctx *c, float *grid, float *box, bool to_box);
 hipError_t launch_box_u8(fsl_ctx *c, uint8_t *grid, int32_t *box, bool to_box);
@@ -258,6 +313,13 @@ int pcg_local1(fsl_ctx *c, double *pAp);
 int pcg_local2(fsl_ctx *c, double *alpha, double *rr, double *rz);
 int pcg_local3(fsl_ctx *c, double *beta);
 int pcg_finish(fsl_ctx *c, fsl_solve_stats *st);
+// pcg_finish in two parts: the solution copy and the true-residual kernels
+// with the state copied to the host (enqueued), then the wait for that copy
+// and the statistics; work enqueued in between overlaps the wait.
+int pcg_finish_enqueue(fsl_ctx *c);
+int pcg_finish_collect(fsl_ctx *c, fsl_solve_stats *st);
+// The solve without its finish (pcg_run = pcg_solve_iterations + pcg_finish).
+int pcg_solve_iterations(fsl_ctx *c, const fsl_solve_params *prm, fsl_solve_stats *st);
 int pcg_true_residual(fsl_ctx *c, double *rr_true);
 
 // ic.hip: block IC(0). ic_factor sorts the FLUID leaves by level and factors

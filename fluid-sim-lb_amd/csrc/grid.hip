@@ -12,29 +12,28 @@ namespace {
 constexpr int kThreads = 256;
 inline unsigned blocks_for(int64_t n, int t) { return unsigned((n + t - 1) / t); }
 
-// Voxel handled by this thread of a per-leaf kernel; false past the list.
-__device__ inline bool leaf_voxel(const Dom &d, const int32_t *list, int64_t nlist, int64_t &v,
-                                  int c[3]) {
-  const int64_t w = xcd_work(nlist);
-  if (w >= nlist) return false;
-  const int64_t leaf = list[w];
-  int o[3];
-  leaf_origin(d, leaf, o);
-  const int t = threadIdx.x;
-  c[0] = o[0] + (t >> 6); c[1] = o[1] + ((t >> 3) & 7); c[2] = o[2] + (t & 7);
-  v = (leaf << 9) | t;
-  return true;
+// Runs f(v, c) for each leaf of L with v this thread's voxel (one thread
+// per voxel, 512 per workgroup) and c its cell coordinates.
+template <typename F>
+__device__ inline void for_each_voxel(const Dom &d, const LeafList &L, F &&f) {
+  for_each_leaf(L.list, L.count, L.n, [&](int64_t leaf) {
+    int o[3];
+    leaf_origin(d, leaf, o);
+    const int t = threadIdx.x;
+    const int c[3] = {o[0] + (t >> 6), o[1] + ((t >> 3) & 7), o[2] + (t & 7)};
+    f((leaf << 9) | t, c);
+  });
 }
 
-__global__ __launch_bounds__(512) void k_clear(const int32_t *list, int64_t nlist, float *v0,
-                                               float *v1, float *v2, float *u0, float *u1,
-                                               float *u2, uint8_t *vflag, uint8_t *marker,
-                                               uint8_t *lab, float *div, float *pres) {
-  const int64_t w = xcd_work(nlist);
-  if (w >= nlist) return;
-  const int64_t v = (int64_t(list[w]) << 9) | threadIdx.x;
-  v0[v] = 0; v1[v] = 0; v2[v] = 0; u0[v] = 0; u1[v] = 0; u2[v] = 0;
-  vflag[v] = 0; marker[v] = 0; lab[v] = 0; div[v] = 0; pres[v] = 0;
+__global__ __launch_bounds__(512) void k_clear(LeafList L, float *v0, float *v1, float *v2,
+                                               float *u0, float *u1, float *u2, uint8_t *vflag,
+                                               uint8_t *marker, uint8_t *lab, float *div,
+                                               float *pres) {
+  for_each_leaf(L.list, L.count, L.n, [&](int64_t leaf) {
+    const int64_t v = (leaf << 9) | threadIdx.x;
+    v0[v] = 0; v1[v] = 0; v2[v] = 0; u0[v] = 0; u1[v] = 0; u2[v] = 0;
+    vflag[v] = 0; marker[v] = 0; lab[v] = 0; div[v] = 0; pres[v] = 0;
+  });
 }
 
 __device__ inline uint8_t solid_at(const Dom &d, const uint8_t *solid, int i, int j, int k) {
@@ -45,158 +44,156 @@ __device__ inline uint8_t solid_at(const Dom &d, const uint8_t *solid, int i, in
 // marked SOLID; the normal component on a face between a solid and a
 // non-solid cell is zeroed when the face is active. Face d of voxel c lies
 // between c - e_d and c, so it is zeroed iff solid(c) != solid(c - e_d).
-__global__ __launch_bounds__(512) void k_bc(Dom d, const int32_t *list, int64_t nlist,
+__global__ __launch_bounds__(512) void k_bc(Dom d, LeafList L,
                                             const uint8_t *solid, uint8_t *marker,
                                             const uint8_t *vflag, float *vx, float *vy,
                                             float *vz, int save, float *ux, float *uy, float *uz,
                                             uint8_t *vflag_out) {
-  int64_t v;
-  int c[3];
-  if (!leaf_voxel(d, list, nlist, v, c)) return;
-  const uint8_t s = solid[v];
-  if (s) marker[v] = kSolid;
-  const uint8_t f = vflag[v];
-  float a = vx[v], b = vy[v], e = vz[v];
-  if (f & 1) {
-    if (s != solid_at(d, solid, c[0] - 1, c[1], c[2])) a = 0.0f;
-    if (s != solid_at(d, solid, c[0], c[1] - 1, c[2])) b = 0.0f;
-    if (s != solid_at(d, solid, c[0], c[1], c[2] - 1)) e = 0.0f;
-    vx[v] = a; vy[v] = b; vz[v] = e;
-  }
-  if (save) {  // SaveGridVelocities (flip_simulation.cc:490-496)
-    ux[v] = a; uy[v] = b; uz[v] = e;
-    vflag_out[v] = (f & 1) ? uint8_t(3) : uint8_t(0);
-  }
+  for_each_voxel(d, L, [&](int64_t v, const int *c) {
+    const uint8_t s = solid[v];
+    if (s) marker[v] = kSolid;
+    const uint8_t f = vflag[v];
+    float a = vx[v], b = vy[v], e = vz[v];
+    if (f & 1) {
+      if (s != solid_at(d, solid, c[0] - 1, c[1], c[2])) a = 0.0f;
+      if (s != solid_at(d, solid, c[0], c[1] - 1, c[2])) b = 0.0f;
+      if (s != solid_at(d, solid, c[0], c[1], c[2] - 1)) e = 0.0f;
+      vx[v] = a; vy[v] = b; vz[v] = e;
+    }
+    if (save) {  // SaveGridVelocities (flip_simulation.cc:490-496)
+      ux[v] = a; uy[v] = b; uz[v] = e;
+      vflag_out[v] = (f & 1) ? uint8_t(3) : uint8_t(0);
+    }
+  });
 }
 
-__global__ __launch_bounds__(512) void k_save(Dom d, const int32_t *list, int64_t nlist,
+__global__ __launch_bounds__(512) void k_save(Dom d, LeafList L,
                                               const float *vx, const float *vy, const float *vz,
                                               float *ux, float *uy, float *uz, uint8_t *vflag) {
-  int64_t v;
-  int c[3];
-  if (!leaf_voxel(d, list, nlist, v, c)) return;
-  ux[v] = vx[v]; uy[v] = vy[v]; uz[v] = vz[v];
-  vflag[v] = (vflag[v] & 1) ? uint8_t(3) : uint8_t(0);
+  for_each_voxel(d, L, [&](int64_t v, const int *c) {
+    ux[v] = vx[v]; uy[v] = vy[v]; uz[v] = vz[v];
+    vflag[v] = (vflag[v] & 1) ? uint8_t(3) : uint8_t(0);
+  });
 }
 
 // AddGravity (flip_simulation.cc:505-515) on active faces.
-__global__ __launch_bounds__(512) void k_gravity(Dom d, const int32_t *list, int64_t nlist,
+// dt_dev, when set, holds dt on the device (dt is then unused).
+__global__ __launch_bounds__(512) void k_gravity(Dom d, LeafList L,
                                                  const uint8_t *vflag, float *vx, float *vy,
-                                                 float *vz, float ax, float ay, float az) {
+                                                 float *vz, float dt, const float *dt_dev,
+                                                 float fx, float fy, float fz) {
 #pragma clang fp contract(off)
-  int64_t v;
-  int c[3];
-  if (!leaf_voxel(d, list, nlist, v, c)) return;
-  if (!(vflag[v] & 1)) return;
-  vx[v] = vx[v] + ax;
-  vy[v] = vy[v] - ay;
-  vz[v] = vz[v] + az;
+  if (dt_dev) dt = *dt_dev;
+  const float ax = dt * fx, ay = dt * fy, az = dt * fz;
+  for_each_voxel(d, L, [&](int64_t v, const int *c) {
+#pragma clang fp contract(off)
+    if (!(vflag[v] & 1)) return;
+    vx[v] = vx[v] + ax;
+    vy[v] = vy[v] - ay;
+    vz[v] = vz[v] + az;
+  });
 }
 
 // ComputeDivergence (flip_simulation.cc:715-736) fused with the solver
 // labels: lab = marker | diag << 2, diag = count of FLUID or AIR neighbours
 // (CreateLaplacianMatrix, solver.cc:214-231). Also flags leaves holding FLUID
 // cells and counts rows per tile (solver.cc:80, metagrid.cc:84-87).
-__global__ __launch_bounds__(512) void k_divergence(Dom d, const int32_t *list, int64_t nlist,
+__global__ __launch_bounds__(512) void k_divergence(Dom d, LeafList L,
                                                     const uint8_t *marker, const float *vx,
                                                     const float *vy, const float *vz,
                                                     float *div, uint8_t *lab, uint32_t *lfluid,
                                                     DevFlags *flags) {
 #pragma clang fp contract(off)
   __shared__ unsigned s_cnt[16];
-  int64_t v;
-  int c[3];
-  if (!leaf_voxel(d, list, nlist, v, c)) return;
-  // Every load is issued whatever the marker turns out to be, so none waits
-  // for it: cells outside the box read their own voxel in place of the
-  // neighbours (in the box, the neighbours lie in [-1, n], inside storage).
-  const bool inb = in_bbox(d, c[0], c[1], c[2]);
-  const int off[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
-  int64_t nv[6];
+  for_each_voxel(d, L, [&](int64_t v, const int *c) {
+#pragma clang fp contract(off)
+    // Every load is issued whatever the marker turns out to be, so none waits
+    // for it: cells outside the box read their own voxel in place of the
+    // neighbours (in the box, the neighbours lie in [-1, n], inside storage).
+    const bool inb = in_bbox(d, c[0], c[1], c[2]);
+    const int off[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+    int64_t nv[6];
 #pragma unroll
-  for (int q = 0; q < 6; ++q)
-    nv[q] = inb ? vox_of(d, c[0] + off[q][0], c[1] + off[q][1], c[2] + off[q][2]) : v;
-  const uint8_t m = marker[v];
-  uint8_t nm[6];
+    for (int q = 0; q < 6; ++q)
+      nv[q] = inb ? vox_of(d, c[0] + off[q][0], c[1] + off[q][1], c[2] + off[q][2]) : v;
+    const uint8_t m = marker[v];
+    uint8_t nm[6];
 #pragma unroll
-  for (int q = 0; q < 6; ++q) nm[q] = marker[nv[q]];
-  const float ux = vx[v], uxp = vx[nv[1]], uy = vy[v], uyp = vy[nv[3]], uz = vz[v],
-              uzp = vz[nv[5]];
-  const bool fluid = m == kFluid && inb;
-  float dv = 0.0f;
-  uint8_t l = m;
-  if (fluid) {
-    dv = uxp - ux;
-    dv = dv + (uyp - uy);
-    dv = dv + (uzp - uz);
-    int diag = 0;
+    for (int q = 0; q < 6; ++q) nm[q] = marker[nv[q]];
+    const float ux = vx[v], uxp = vx[nv[1]], uy = vy[v], uyp = vy[nv[3]], uz = vz[v],
+                uzp = vz[nv[5]];
+    const bool fluid = m == kFluid && inb;
+    float dv = 0.0f;
+    uint8_t l = m;
+    if (fluid) {
+      dv = uxp - ux;
+      dv = dv + (uyp - uy);
+      dv = dv + (uzp - uz);
+      int diag = 0;
 #pragma unroll
-    for (int q = 0; q < 6; ++q) diag += (nm[q] == kFluid || nm[q] == kAir) ? 1 : 0;
-    if (diag == 0) atomicOr(&flags->zero_diag, 1u);
-    l = uint8_t(kFluid | (diag << 2));
-  }
-  // A FLUID marker outside the partition box is not a row of this system
-  // (in the reference it is a ghost column); label 3 keeps it out of the
-  // solve while still counting as FLUID in its neighbours' diagonals.
-  if (m == kFluid && !fluid) l = 3;
-  div[v] = dv;
-  lab[v] = l;
-  const unsigned cnt = unsigned(block_sum<unsigned>(fluid ? 1u : 0u, s_cnt));
-  if (threadIdx.x == 0) {
-    lfluid[v >> 9] = cnt ? 1u : 0u;
-    if (cnt) {
-      atomicAdd(&flags->n_rows, (unsigned long long)cnt);
-      atomicAdd(&flags->tile_rows[tile_of(d, c[0], c[1], c[2]) & 63], (unsigned long long)cnt);
+      for (int q = 0; q < 6; ++q) diag += (nm[q] == kFluid || nm[q] == kAir) ? 1 : 0;
+      if (diag == 0) atomicOr(&flags->zero_diag, 1u);
+      l = uint8_t(kFluid | (diag << 2));
     }
-  }
+    // A FLUID marker outside the partition box is not a row of this system
+    // (in the reference it is a ghost column); label 3 keeps it out of the
+    // solve while still counting as FLUID in its neighbours' diagonals.
+    if (m == kFluid && !fluid) l = 3;
+    div[v] = dv;
+    lab[v] = l;
+    const unsigned cnt = unsigned(block_sum<unsigned>(fluid ? 1u : 0u, s_cnt));
+    if (threadIdx.x == 0) {
+      lfluid[v >> 9] = cnt ? 1u : 0u;
+      if (cnt) {
+        atomicAdd(&flags->n_rows, (unsigned long long)cnt);
+        atomicAdd(&flags->tile_rows[tile_of(d, c[0], c[1], c[2]) & 63], (unsigned long long)cnt);
+      }
+    }
+  });
 }
 
 // UpdateVelocityFromPressure (flip_simulation.cc:738-778) over faces
 // [0, n] of active leaves; AIR pressure is 0 (the cleared grid).
-__global__ __launch_bounds__(512) void k_pressure_update(Dom d, const int32_t *list,
-                                                         int64_t nlist, const uint8_t *marker,
+__global__ __launch_bounds__(512) void k_pressure_update(Dom d, LeafList L, const uint8_t *marker,
                                                          const float *pres, float *vx, float *vy,
                                                          float *vz, uint8_t *vflag) {
-  int64_t v;
-  int c[3];
-  if (!leaf_voxel(d, list, nlist, v, c)) return;
-  if (c[0] < 0 || c[1] < 0 || c[2] < 0 || c[0] > d.n[0] || c[1] > d.n[1] || c[2] > d.n[2]) return;
-  const uint8_t m = marker[v];
-  if (m == kSolid) return;
-  const float pc = pres[v];
-  bool upd = false;
-  float *g[3] = {vx, vy, vz};
-  const int off[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  for_each_voxel(d, L, [&](int64_t v, const int *c) {
+    if (c[0] < 0 || c[1] < 0 || c[2] < 0 || c[0] > d.n[0] || c[1] > d.n[1] || c[2] > d.n[2]) return;
+    const uint8_t m = marker[v];
+    if (m == kSolid) return;
+    const float pc = pres[v];
+    bool upd = false;
+    float *g[3] = {vx, vy, vz};
+    const int off[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
 #pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const int64_t w = vox_of(d, c[0] - off[a][0], c[1] - off[a][1], c[2] - off[a][2]);
-    const uint8_t mw = marker[w];
-    if ((mw == kFluid || m == kFluid) && mw != kSolid) {
-      g[a][v] = g[a][v] + (pc - pres[w]);
-      upd = true;
+    for (int a = 0; a < 3; ++a) {
+      const int64_t w = vox_of(d, c[0] - off[a][0], c[1] - off[a][1], c[2] - off[a][2]);
+      const uint8_t mw = marker[w];
+      if ((mw == kFluid || m == kFluid) && mw != kSolid) {
+        g[a][v] = g[a][v] + (pc - pres[w]);
+        upd = true;
+      }
     }
-  }
-  if (upd) vflag[v] = vflag[v] | 1;
+    if (upd) vflag[v] = vflag[v] | 1;
+  });
 }
 
 // GetVelocityUpdate (flip_simulation.cc:780-801): dv = v - saved on faces
 // [0, n] active in both; elsewhere the saved value stays.
-__global__ __launch_bounds__(512) void k_velocity_update(Dom d, const int32_t *list,
-                                                         int64_t nlist, const uint8_t *vflag,
+__global__ __launch_bounds__(512) void k_velocity_update(Dom d, LeafList L, const uint8_t *vflag,
                                                          const float *vx, const float *vy,
                                                          const float *vz, float *ux, float *uy,
                                                          float *uz, DevFlags *flags) {
-  int64_t v;
-  int c[3];
-  if (!leaf_voxel(d, list, nlist, v, c)) return;
-  if (c[0] < 0 || c[1] < 0 || c[2] < 0 || c[0] > d.n[0] || c[1] > d.n[1] || c[2] > d.n[2]) return;
-  const uint8_t f = vflag[v];
-  if ((f & 1) && !(f & 2)) atomicOr(&flags->vel_update, 1u);
-  if ((f & 3) == 3) {
-    ux[v] = vx[v] - ux[v];
-    uy[v] = vy[v] - uy[v];
-    uz[v] = vz[v] - uz[v];
-  }
+  for_each_voxel(d, L, [&](int64_t v, const int *c) {
+    if (c[0] < 0 || c[1] < 0 || c[2] < 0 || c[0] > d.n[0] || c[1] > d.n[1] || c[2] > d.n[2]) return;
+    const uint8_t f = vflag[v];
+    if ((f & 1) && !(f & 2)) atomicOr(&flags->vel_update, 1u);
+    if ((f & 3) == 3) {
+      ux[v] = vx[v] - ux[v];
+      uy[v] = vy[v] - uy[v];
+      uz[v] = vz[v] - uz[v];
+    }
+  });
 }
 
 // VectorGrid::ExtrapolateToBoundary (vector_grid.cc:302-381) for one axis
@@ -246,15 +243,16 @@ __global__ void k_clear_shell(Dom d, uint8_t *vflag) {
 // active leaves (at most kCflBlocks workgroups, so one atomic each); the
 // activity byte and the three components are loaded together.
 constexpr int kCflBlocks = 1024;
-__global__ __launch_bounds__(512) void k_cfl(Dom d, const int32_t *list, int64_t nlist,
-                                             const uint8_t *vflag, const float *vx,
-                                             const float *vy, const float *vz, unsigned *bits) {
+__global__ __launch_bounds__(512) void k_cfl(Dom d, LeafList L, const uint8_t *vflag,
+                                             const float *vx, const float *vy, const float *vz,
+                                             unsigned *bits) {
   __shared__ float s_m[8];
   const int t = threadIdx.x;
   const int e0 = d.n[0] - 1, e1 = d.n[1] - 1, e2 = d.n[2] - 1;
+  const int64_t nlist = L.count ? int64_t(*L.count) : L.n;
   float m = 0.0f;
   for (int64_t w = blockIdx.x; w < nlist; w += gridDim.x) {
-    const int64_t leaf = list[w];
+    const int64_t leaf = L.list[w];
     int o[3];
     leaf_origin(d, leaf, o);
     const int c0 = o[0] + (t >> 6), c1 = o[1] + ((t >> 3) & 7), c2 = o[2] + (t & 7);
@@ -275,6 +273,38 @@ __global__ __launch_bounds__(512) void k_cfl(Dom d, const int32_t *list, int64_t
     for (int q = 0; q < 8; ++q) x = fmaxf(x, s_m[q]);
     atomicMax(bits, __float_as_uint(x));  // non-negative floats order as uints
   }
+}
+
+// WhileAndComputeDt's dt (flip_app_sim.cc:357-410) on the device: the CFL
+// speed max(sqrt(g / 2), v_inf, 1e-2) (flip_simulation.cc:485-487, v_inf
+// from k_cfl), dt = min(1 / frame_step, cfl / speed), floored at frame_step
+// / 10, cut at the frame's end and snapped to it past 90% of the frame; the
+// elapsed time advances by dt. The float/double mix is the host's
+// (fsl_compute_dt). reset: the step starts a frame.
+__global__ void k_dt(const unsigned *cfl_bits, DtParams p, int reset, StepState *ss) {
+#pragma clang fp contract(off)
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const float v_inf = __uint_as_float(*cfl_bits);
+  float v = float(fmax(p.v_gravity, double(v_inf)));
+  if (v < 1e-2f) v = 1e-2f;
+  const float inv = float(1.0 / double(v));
+  const float local_dt = p.cfl * inv;
+  const float step = reset ? 0.0f : ss->step;
+  float g = p.dt_max < local_dt ? p.dt_max : local_dt;
+  if (g < p.dt_min) g = p.dt_min;
+  if (step + g > p.frame_step) g = p.frame_step - step;
+  if (double(step + g) > p.snap) g = p.frame_step - step;
+  const float after = step + g;
+  ss->dt = g;
+  ss->step = after;
+  // The While test, step + THRESH < frame_step (flip_app_sim.cc:363, THRESH
+  // 1e-5 in double).
+  ss->more = double(after) + 1e-5 < double(p.frame_step) ? 1 : 0;
+}
+
+__global__ void k_fill_u32(uint32_t *p, int64_t n, uint32_t value) {
+  const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q < n) p[q] = value;
 }
 
 // Box format <-> leaf storage.
@@ -327,59 +357,61 @@ __global__ void k_all_leaves(int64_t n, const int32_t *morton, uint32_t *lact, u
 
 }  // namespace
 
-hipError_t launch_clear_leaves(fsl_ctx *c, const int32_t *list, int64_t n) {
-  if (n == 0) return hipSuccess;
-  k_clear<<<xcd_grid(n), 512, 0, c->stream>>>(list, n, c->v[0], c->v[1], c->v[2], c->u[0],
-                                               c->u[1], c->u[2], c->vflag, c->marker, c->lab,
-                                               c->div, c->pres);
+hipError_t launch_clear_leaves(fsl_ctx *c, const LeafList &L) {
+  if (L.n == 0) return hipSuccess;
+  k_clear<<<leaf_grid(L.n, L.count), 512, 0, c->stream>>>(L, c->v[0], c->v[1], c->v[2], c->u[0],
+                                                           c->u[1], c->u[2], c->vflag, c->marker,
+                                                           c->lab, c->div, c->pres);
   return hipGetLastError();
 }
+// Workgroups of a per-leaf kernel over the active leaves.
+inline int act_grid(const fsl_ctx *c) { return leaf_grid(c->n_act, c->counts_on_device); }
 hipError_t launch_bc(fsl_ctx *c) {
   if (c->n_act == 0) return hipSuccess;
-  k_bc<<<xcd_grid(c->n_act), 512, 0, c->stream>>>(c->dom, c->act_list, c->n_act, c->solid,
-                                                   c->marker, c->vflag, c->v[0], c->v[1], c->v[2],
-                                                   0, nullptr, nullptr, nullptr, nullptr);
+  k_bc<<<act_grid(c), 512, 0, c->stream>>>(c->dom, act_leaves(c), c->solid, c->marker, c->vflag,
+                                            c->v[0], c->v[1], c->v[2], 0, nullptr, nullptr,
+                                            nullptr, nullptr);
   return hipGetLastError();
 }
 hipError_t launch_bc_save(fsl_ctx *c) {
   if (c->n_act == 0) return hipSuccess;
-  k_bc<<<xcd_grid(c->n_act), 512, 0, c->stream>>>(c->dom, c->act_list, c->n_act, c->solid,
-                                                   c->marker, c->vflag, c->v[0], c->v[1], c->v[2],
-                                                   1, c->u[0], c->u[1], c->u[2], c->vflag);
+  k_bc<<<act_grid(c), 512, 0, c->stream>>>(c->dom, act_leaves(c), c->solid, c->marker, c->vflag,
+                                            c->v[0], c->v[1], c->v[2], 1, c->u[0], c->u[1],
+                                            c->u[2], c->vflag);
   return hipGetLastError();
 }
 hipError_t launch_save(fsl_ctx *c) {
   if (c->n_act == 0) return hipSuccess;
-  k_save<<<xcd_grid(c->n_act), 512, 0, c->stream>>>(c->dom, c->act_list, c->n_act, c->v[0],
-                                                     c->v[1], c->v[2], c->u[0], c->u[1], c->u[2],
-                                                     c->vflag);
+  k_save<<<act_grid(c), 512, 0, c->stream>>>(c->dom, act_leaves(c), c->v[0], c->v[1], c->v[2],
+                                              c->u[0], c->u[1], c->u[2], c->vflag);
   return hipGetLastError();
 }
-hipError_t launch_gravity(fsl_ctx *c, float dt) {
+hipError_t launch_gravity(fsl_ctx *c, float dt, const float *dt_dev) {
   if (c->n_act == 0) return hipSuccess;
-  const float ax = dt * c->force[0], ay = dt * c->force[1], az = dt * c->force[2];
-  k_gravity<<<xcd_grid(c->n_act), 512, 0, c->stream>>>(c->dom, c->act_list, c->n_act, c->vflag,
-                                                        c->v[0], c->v[1], c->v[2], ax, ay, az);
+  k_gravity<<<act_grid(c), 512, 0, c->stream>>>(c->dom, act_leaves(c), c->vflag, c->v[0],
+                                                 c->v[1], c->v[2], dt, dt_dev, c->force[0],
+                                                 c->force[1], c->force[2]);
   return hipGetLastError();
 }
 hipError_t launch_divergence(fsl_ctx *c) {
   if (c->n_act == 0) return hipSuccess;
-  k_divergence<<<xcd_grid(c->n_act), 512, 0, c->stream>>>(c->dom, c->act_list, c->n_act,
-                                                           c->marker, c->v[0], c->v[1], c->v[2],
-                                                           c->div, c->lab, c->lfluid, c->flags);
+  k_divergence<<<act_grid(c), 512, 0, c->stream>>>(c->dom, act_leaves(c), c->marker, c->v[0],
+                                                    c->v[1], c->v[2], c->div, c->lab, c->lfluid,
+                                                    c->flags);
   return hipGetLastError();
 }
 hipError_t launch_pressure_update(fsl_ctx *c) {
   if (c->n_act == 0) return hipSuccess;
-  k_pressure_update<<<xcd_grid(c->n_act), 512, 0, c->stream>>>(
-      c->dom, c->act_list, c->n_act, c->marker, c->pres, c->v[0], c->v[1], c->v[2], c->vflag);
+  k_pressure_update<<<act_grid(c), 512, 0, c->stream>>>(c->dom, act_leaves(c), c->marker,
+                                                         c->pres, c->v[0], c->v[1], c->v[2],
+                                                         c->vflag);
   return hipGetLastError();
 }
 hipError_t launch_velocity_update(fsl_ctx *c) {
   if (c->n_act == 0) return hipSuccess;
-  k_velocity_update<<<xcd_grid(c->n_act), 512, 0, c->stream>>>(
-      c->dom, c->act_list, c->n_act, c->vflag, c->v[0], c->v[1], c->v[2], c->u[0], c->u[1],
-      c->u[2], c->flags);
+  k_velocity_update<<<act_grid(c), 512, 0, c->stream>>>(c->dom, act_leaves(c), c->vflag,
+                                                         c->v[0], c->v[1], c->v[2], c->u[0],
+                                                         c->u[1], c->u[2], c->flags);
   return hipGetLastError();
 }
 hipError_t launch_extrapolate(fsl_ctx *c) {
@@ -399,8 +431,18 @@ hipError_t launch_clear_shell(fsl_ctx *c) {
 hipError_t launch_cfl(fsl_ctx *c) {
   if (c->n_act == 0) return hipSuccess;
   const int g = int(std::min<int64_t>(c->n_act, kCflBlocks));
-  k_cfl<<<g, 512, 0, c->stream>>>(c->dom, c->act_list, c->n_act, c->vflag, c->v[0], c->v[1],
-                                  c->v[2], &c->flags->cfl_bits);
+  k_cfl<<<g, 512, 0, c->stream>>>(c->dom, act_leaves(c), c->vflag, c->v[0], c->v[1], c->v[2],
+                                  &c->flags->cfl_bits);
+  return hipGetLastError();
+}
+hipError_t launch_dt(fsl_ctx *c, const DtParams &p, int reset_frame) {
+  k_dt<<<1, 64, 0, c->stream>>>(&c->flags->cfl_bits, p, reset_frame, c->sstate);
+  return hipGetLastError();
+}
+// Fills n words (in place of a memset, which must not be captured in a graph).
+hipError_t launch_fill_u32(fsl_ctx *c, uint32_t *p, int64_t n, uint32_t value) {
+  if (n <= 0) return hipSuccess;
+  k_fill_u32<<<blocks_for(n, kThreads), kThreads, 0, c->stream>>>(p, n, value);
   return hipGetLastError();
 }
 hipError_t launch_box_vec(fsl_ctx *c, int which, float *out3, uint8_t *act, bool to_box) {

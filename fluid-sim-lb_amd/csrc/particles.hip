@@ -147,9 +147,14 @@ __device__ inline KeyRun key_run(uint32_t key) {
 // Particles::StepInGrid (particles.cc:504-543) + DeleteParticlesInSolids
 // (particles.cc:202-219). Particles outside the partition box were deleted by
 // DeleteOutsideParticles after the previous P2G; they are dropped here.
-__global__ void k_advect(Dom d, int64_t np, Particles P, float dt, const uint8_t *solid,
-                         uint32_t *key, uint32_t *vcount, int drop_outside, KillBoxes kill) {
+// np_dev and dt_dev, when set, hold the particle count and dt on the device
+// (np is then an upper bound that sizes the launch).
+__global__ void k_advect(Dom d, int64_t np, const uint32_t *np_dev, Particles P, float dt,
+                         const float *dt_dev, const uint8_t *solid, uint32_t *key,
+                         uint32_t *vcount, int drop_outside, KillBoxes kill) {
 #pragma clang fp contract(off)
+  if (np_dev) np = *np_dev;
+  if (dt_dev) dt = *dt_dev;
   const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   uint32_t k = kDead;
   if (q < np) {
@@ -194,8 +199,10 @@ __global__ void k_advect(Dom d, int64_t np, Particles P, float dt, const uint8_t
 // order, ascending from the cell's next free slot, so a cell whose particles
 // arrive in one run is already in its previous order. skey[slot] records the
 // slot's cell.
-__global__ void k_scatter(int64_t np, const uint32_t *key, const uint32_t *vstart,
-                          uint32_t *vcount, uint32_t *perm, uint32_t *skey) {
+__global__ void k_scatter(int64_t np, const uint32_t *np_dev, const uint32_t *key,
+                          const uint32_t *vstart, uint32_t *vcount, uint32_t *perm,
+                          uint32_t *skey) {
+  if (np_dev) np = *np_dev;
   const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const uint32_t k = q < np ? key[q] : kDead;
   const KeyRun r = key_run(k);
@@ -217,8 +224,10 @@ __global__ void k_scatter(int64_t np, const uint32_t *key, const uint32_t *vstar
 // last) and written back only when it was out of order; larger cells take
 // the insertion sort in memory.
 constexpr uint32_t kCellRegs = 16;
-__global__ void k_cell_sort(uint32_t n, const uint32_t *skey, const uint32_t *vstart,
-                            uint32_t *perm) {
+// n_dev, when set, holds the count of sorted particles (vstart's last entry).
+__global__ void k_cell_sort(uint32_t n, const uint32_t *n_dev, const uint32_t *skey,
+                            const uint32_t *vstart, uint32_t *perm) {
+  if (n_dev) n = *n_dev;
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const uint32_t k = skey[s];
@@ -265,8 +274,15 @@ __global__ void k_cell_sort(uint32_t n, const uint32_t *skey, const uint32_t *vs
 
 // Moves arrays 0..NA-1 into sorted order (NA = 6 leaves out the grid
 // velocities gx, gy, gz, which only advection reads and G2P rewrites).
+// With n_dev (the sorted count on the device) the new count is also stored
+// in *count_out.
 template <int NA>
-__global__ void k_gather(uint32_t n, const uint32_t *perm, Particles in, Particles out) {
+__global__ void k_gather(uint32_t n, const uint32_t *n_dev, uint32_t *count_out,
+                         const uint32_t *perm, Particles in, Particles out) {
+  if (n_dev) {
+    n = *n_dev;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *count_out = n;
+  }
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const uint32_t q = perm[s];
@@ -309,9 +325,10 @@ __global__ void k_gather_flags(int64_t n, const int32_t *order, const uint32_t *
   if (m < n) fm[m] = flag[order[m]];
 }
 __global__ void k_compact(int64_t n, const int32_t *order, const uint32_t *fm,
-                          const uint32_t *pos, int32_t *list) {
+                          const uint32_t *pos, int32_t *list, uint32_t *count_out) {
   const int64_t m = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (m < n && fm[m]) list[pos[m]] = order[m];
+  if (count_out && m == n - 1) *count_out = pos[m] + fm[m];
 }
 
 // ------------------------------------------------------------------ P2G
@@ -489,7 +506,7 @@ __device__ inline void p2g_own(const Particles &P, const int c[3], int r, uint32
 // sums sum_p w * value_p with the particle values taken from P.a[3..5]; no
 // weights, labels or activity are written.
 template <bool ADJ>
-__global__ __launch_bounds__(512) void k_p2g_own(Dom d, const int32_t *list, int64_t nlist,
+__global__ __launch_bounds__(512) void k_p2g_own(Dom d, LeafList L,
                                                  const uint32_t *lhas, const uint32_t *vstart,
                                                  Particles P, float *vx, float *vy, float *vz,
                                                  float *wx, float *wy, float *wz, uint8_t *vflag,
@@ -497,44 +514,45 @@ __global__ __launch_bounds__(512) void k_p2g_own(Dom d, const int32_t *list, int
   __shared__ float s_vw[3][1000];
   __shared__ float s_w[3][1000];
   __shared__ uint8_t s_touch[1000];
-  const int64_t wi = xcd_work(nlist);
-  if (wi >= nlist) return;
-  const int64_t leaf = list[wi];
-  if (!lhas[leaf]) return;
-  int o[3];
-  leaf_origin(d, leaf, o);
-  const int t = threadIdx.x;
-  for (int i = t; i < 1000; i += 512) {
+  for_each_leaf(L.list, L.count, L.n, [&](int64_t leaf) {
+    // The previous leaf's LDS reads are done before the tile is refilled.
+    __syncthreads();
+    if (!lhas[leaf]) return;
+    int o[3];
+    leaf_origin(d, leaf, o);
+    const int t = threadIdx.x;
+    for (int i = t; i < 1000; i += 512) {
 #pragma unroll
-    for (int a = 0; a < 3; ++a) { s_vw[a][i] = 0.0f; s_w[a][i] = 0.0f; }
-    s_touch[i] = 0;
-  }
-  const int lx = t >> 6, ly = (t >> 3) & 7, lz = t & 7;
-  const int c[3] = {o[0] + lx, o[1] + ly, o[2] + lz};
-  const int r = (lx + 1) * 100 + (ly + 1) * 10 + (lz + 1);
-  const int64_t v = (leaf << 9) | t;
-  const int64_t sv = sort_begin(d, leaf) | t;
-  const uint32_t beg = vstart[sv], end = vstart[sv + 1];
-  __syncthreads();
-  p2g_own(P, c, r, beg, end, s_vw, s_w, s_touch);
-  vx[v] = s_vw[0][r]; vy[v] = s_vw[1][r]; vz[v] = s_vw[2][r];
-  if (!ADJ) {
-    wx[v] = s_w[0][r]; wy[v] = s_w[1][r]; wz[v] = s_w[2][r];
-    vflag[v] = s_touch[r];
-  }
-  for (int i = t; i < kRing; i += 512) {
-    int x, y, z;
-    ring_cell(i, x, y, z);
-    const int g = x * 100 + y * 10 + z;
-    const int64_t e = leaf * kRing + i;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) ring[a * rstride + e] = s_vw[a][g];
-    if (!ADJ) {
-#pragma unroll
-      for (int a = 0; a < 3; ++a) ring[(3 + a) * rstride + e] = s_w[a][g];
-      ring_t[e] = s_touch[g];
+      for (int a = 0; a < 3; ++a) { s_vw[a][i] = 0.0f; s_w[a][i] = 0.0f; }
+      s_touch[i] = 0;
     }
-  }
+    const int lx = t >> 6, ly = (t >> 3) & 7, lz = t & 7;
+    const int c[3] = {o[0] + lx, o[1] + ly, o[2] + lz};
+    const int r = (lx + 1) * 100 + (ly + 1) * 10 + (lz + 1);
+    const int64_t v = (leaf << 9) | t;
+    const int64_t sv = sort_begin(d, leaf) | t;
+    const uint32_t beg = vstart[sv], end = vstart[sv + 1];
+    __syncthreads();
+    p2g_own(P, c, r, beg, end, s_vw, s_w, s_touch);
+    vx[v] = s_vw[0][r]; vy[v] = s_vw[1][r]; vz[v] = s_vw[2][r];
+    if (!ADJ) {
+      wx[v] = s_w[0][r]; wy[v] = s_w[1][r]; wz[v] = s_w[2][r];
+      vflag[v] = s_touch[r];
+    }
+    for (int i = t; i < kRing; i += 512) {
+      int x, y, z;
+      ring_cell(i, x, y, z);
+      const int g = x * 100 + y * 10 + z;
+      const int64_t e = leaf * kRing + i;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) ring[a * rstride + e] = s_vw[a][g];
+      if (!ADJ) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) ring[(3 + a) * rstride + e] = s_w[a][g];
+        ring_t[e] = s_touch[g];
+      }
+    }
+  });
 }
 
 // Completes the samples of every active leaf: its own partial sums (zero
@@ -543,7 +561,7 @@ __global__ __launch_bounds__(512) void k_p2g_own(Dom d, const int32_t *list, int
 // dz) order; then normalises (flip_simulation.cc:900-922) and marks FLUID
 // (flip_simulation.cc:852-857).
 template <bool ADJ>
-__global__ __launch_bounds__(512) void k_p2g_combine(Dom d, const int32_t *list, int64_t nlist,
+__global__ __launch_bounds__(512) void k_p2g_combine(Dom d, LeafList L,
                                                      const uint32_t *lhas,
                                                      const uint32_t *vstart, float *vx,
                                                      float *vy, float *vz, float *wx, float *wy,
@@ -551,61 +569,60 @@ __global__ __launch_bounds__(512) void k_p2g_combine(Dom d, const int32_t *list,
                                                      const float *ring, const uint8_t *ring_t,
                                                      int64_t rstride, float *wsx, float *wsy,
                                                      float *wsz) {
-  const int64_t wi = xcd_work(nlist);
-  if (wi >= nlist) return;
-  const int64_t leaf = list[wi];
-  const int t = threadIdx.x;
-  const int lx = t >> 6, ly = (t >> 3) & 7, lz = t & 7;
-  const int64_t v = (leaf << 9) | t;
-  float vw[3] = {0.0f, 0.0f, 0.0f}, w[3] = {0.0f, 0.0f, 0.0f};
-  uint8_t touch = 0;
-  if (lhas[leaf]) {
-    vw[0] = vx[v]; vw[1] = vy[v]; vw[2] = vz[v];
-    if (!ADJ) {
-      w[0] = wx[v]; w[1] = wy[v]; w[2] = wz[v];
-      touch = vflag[v];
-    }
-  }
-  const int64_t l2 = d.L[2], l12 = int64_t(d.L[1]) * d.L[2];
-  const int L[3] = {int(leaf / l12), int((leaf / l2) % d.L[1]), int(leaf % l2)};
-  const int lc[3] = {lx, ly, lz};
-  int lo[3], hi[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    lo[a] = lc[a] == 0 && L[a] > 0 ? -1 : 0;
-    hi[a] = lc[a] == 7 && L[a] + 1 < d.L[a] ? 1 : 0;
-  }
-  for (int dx = lo[0]; dx <= hi[0]; ++dx)
-    for (int dy = lo[1]; dy <= hi[1]; ++dy)
-      for (int dz = lo[2]; dz <= hi[2]; ++dz) {
-        if (dx == 0 && dy == 0 && dz == 0) continue;
-        const int64_t nb = leaf + dx * l12 + dy * l2 + dz;
-        if (!lhas[nb]) continue;
-        const int64_t e = nb * kRing + ring_index(lx + 1 - 8 * dx, ly + 1 - 8 * dy, lz + 1 - 8 * dz);
-#pragma unroll
-        for (int a = 0; a < 3; ++a) vw[a] += ring[a * rstride + e];
-        if (!ADJ) {
-#pragma unroll
-          for (int a = 0; a < 3; ++a) w[a] += ring[(3 + a) * rstride + e];
-          touch |= ring_t[e];
-        }
+  for_each_leaf(L.list, L.count, L.n, [&](int64_t leaf) {
+    const int t = threadIdx.x;
+    const int lx = t >> 6, ly = (t >> 3) & 7, lz = t & 7;
+    const int64_t v = (leaf << 9) | t;
+    float vw[3] = {0.0f, 0.0f, 0.0f}, w[3] = {0.0f, 0.0f, 0.0f};
+    uint8_t touch = 0;
+    if (lhas[leaf]) {
+      vw[0] = vx[v]; vw[1] = vy[v]; vw[2] = vz[v];
+      if (!ADJ) {
+        w[0] = wx[v]; w[1] = wy[v]; w[2] = wz[v];
+        touch = vflag[v];
       }
-  if (ADJ) {
-    vx[v] = vw[0]; vy[v] = vw[1]; vz[v] = vw[2];
-    return;
-  }
-  if (wsx) { wsx[v] = w[0]; wsy[v] = w[1]; wsz[v] = w[2]; }
-  float out[3];
+    }
+    const int64_t l2 = d.L[2], l12 = int64_t(d.L[1]) * d.L[2];
+    const int L[3] = {int(leaf / l12), int((leaf / l2) % d.L[1]), int(leaf % l2)};
+    const int lc[3] = {lx, ly, lz};
+    int lo[3], hi[3];
 #pragma unroll
-  for (int a = 0; a < 3; ++a) out[a] = double(w[a]) < 1e-5 ? 0.0f : vw[a] / w[a];
-  vx[v] = out[0]; vy[v] = out[1]; vz[v] = out[2];
-  vflag[v] = touch ? 1 : 0;
-  int o[3];
-  leaf_origin(d, leaf, o);
-  uint8_t m = kAir;
-  const int64_t sv = sort_index(d, v);
-  if (in_bbox(d, o[0] + lx, o[1] + ly, o[2] + lz) && vstart[sv + 1] > vstart[sv]) m = kFluid;
-  marker[v] = m;
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = lc[a] == 0 && L[a] > 0 ? -1 : 0;
+      hi[a] = lc[a] == 7 && L[a] + 1 < d.L[a] ? 1 : 0;
+    }
+    for (int dx = lo[0]; dx <= hi[0]; ++dx)
+      for (int dy = lo[1]; dy <= hi[1]; ++dy)
+        for (int dz = lo[2]; dz <= hi[2]; ++dz) {
+          if (dx == 0 && dy == 0 && dz == 0) continue;
+          const int64_t nb = leaf + dx * l12 + dy * l2 + dz;
+          if (!lhas[nb]) continue;
+          const int64_t e = nb * kRing + ring_index(lx + 1 - 8 * dx, ly + 1 - 8 * dy, lz + 1 - 8 * dz);
+#pragma unroll
+          for (int a = 0; a < 3; ++a) vw[a] += ring[a * rstride + e];
+          if (!ADJ) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) w[a] += ring[(3 + a) * rstride + e];
+            touch |= ring_t[e];
+          }
+        }
+    if (ADJ) {
+      vx[v] = vw[0]; vy[v] = vw[1]; vz[v] = vw[2];
+      return;
+    }
+    if (wsx) { wsx[v] = w[0]; wsy[v] = w[1]; wsz[v] = w[2]; }
+    float out[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) out[a] = double(w[a]) < 1e-5 ? 0.0f : vw[a] / w[a];
+    vx[v] = out[0]; vy[v] = out[1]; vz[v] = out[2];
+    vflag[v] = touch ? 1 : 0;
+    int o[3];
+    leaf_origin(d, leaf, o);
+    uint8_t m = kAir;
+    const int64_t sv = sort_index(d, v);
+    if (in_bbox(d, o[0] + lx, o[1] + ly, o[2] + lz) && vstart[sv + 1] > vstart[sv]) m = kFluid;
+    marker[v] = m;
+  });
 }
 
 // ------------------------------------------------------------------ G2P
@@ -641,7 +658,7 @@ __device__ inline float interp(const float pos[3], const int c[3], const int lc[
   return v0 + tt;
 }
 
-__global__ __launch_bounds__(256) void k_g2p(Dom d, const int32_t *list, int64_t nlist,
+__global__ __launch_bounds__(256) void k_g2p(Dom d, LeafList L,
                                              const uint32_t *lhas, const uint32_t *vstart,
                                              Particles P, const float *vx, const float *vy,
                                              const float *vz, const float *ux, const float *uy,
@@ -649,61 +666,63 @@ __global__ __launch_bounds__(256) void k_g2p(Dom d, const int32_t *list, int64_t
 #pragma clang fp contract(off)
   __shared__ float s_v[3][1000];
   __shared__ float s_u[3][1000];
-  const int64_t w = xcd_work(nlist);
-  if (w >= nlist) return;
-  const int64_t leaf = list[w];
-  if (!lhas[leaf]) return;
-  int o[3];
-  leaf_origin(d, leaf, o);
-  for (int idx = threadIdx.x; idx < 1000; idx += blockDim.x) {
-    const int i = o[0] - 1 + idx / 100, j = o[1] - 1 + (idx / 10) % 10, k = o[2] - 1 + idx % 10;
-    float a[6] = {0, 0, 0, 0, 0, 0};
-    if (in_store(d, i, j, k)) {
-      const int64_t v = vox_of(d, i, j, k);
-      a[0] = vx[v]; a[1] = vy[v]; a[2] = vz[v]; a[3] = ux[v]; a[4] = uy[v]; a[5] = uz[v];
+  for_each_leaf(L.list, L.count, L.n, [&](int64_t leaf) {
+#pragma clang fp contract(off)
+    // The previous leaf's LDS reads are done before the tile is refilled.
+    __syncthreads();
+    if (!lhas[leaf]) return;
+    int o[3];
+    leaf_origin(d, leaf, o);
+    for (int idx = threadIdx.x; idx < 1000; idx += blockDim.x) {
+      const int i = o[0] - 1 + idx / 100, j = o[1] - 1 + (idx / 10) % 10, k = o[2] - 1 + idx % 10;
+      float a[6] = {0, 0, 0, 0, 0, 0};
+      if (in_store(d, i, j, k)) {
+        const int64_t v = vox_of(d, i, j, k);
+        a[0] = vx[v]; a[1] = vy[v]; a[2] = vz[v]; a[3] = ux[v]; a[4] = uy[v]; a[5] = uz[v];
+      }
+      s_v[0][idx] = a[0]; s_v[1][idx] = a[1]; s_v[2][idx] = a[2];
+      s_u[0][idx] = a[3]; s_u[1][idx] = a[4]; s_u[2][idx] = a[5];
     }
-    s_v[0][idx] = a[0]; s_v[1][idx] = a[1]; s_v[2][idx] = a[2];
-    s_u[0][idx] = a[3]; s_u[1][idx] = a[4]; s_u[2][idx] = a[5];
-  }
-  __syncthreads();
-  const uint32_t b = vstart[sort_begin(d, leaf)], e = vstart[sort_begin(d, leaf) + kLeaf];
-  // Four particles per trip (q, q + 256, ...), their loads issued together
-  // (indices clamped to the leaf's last particle; clamped copies are not
-  // processed).
-  constexpr uint32_t kTrip = 4;
-  for (uint32_t q0 = b + threadIdx.x; q0 < e; q0 += kTrip * blockDim.x) {
-    float pp[kTrip][3], pv[kTrip][3];
+    __syncthreads();
+    const uint32_t b = vstart[sort_begin(d, leaf)], e = vstart[sort_begin(d, leaf) + kLeaf];
+    // Four particles per trip (q, q + 256, ...), their loads issued together
+    // (indices clamped to the leaf's last particle; clamped copies are not
+    // processed).
+    constexpr uint32_t kTrip = 4;
+    for (uint32_t q0 = b + threadIdx.x; q0 < e; q0 += kTrip * blockDim.x) {
+      float pp[kTrip][3], pv[kTrip][3];
 #pragma unroll
-    for (uint32_t u = 0; u < kTrip; ++u) {
-      const uint32_t q = q0 + u * blockDim.x, qq = q < e ? q : e - 1;
+      for (uint32_t u = 0; u < kTrip; ++u) {
+        const uint32_t q = q0 + u * blockDim.x, qq = q < e ? q : e - 1;
 #pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        pp[u][a] = P.a[a][qq];
-        pv[u][a] = P.a[3 + a][qq];
+        for (int a = 0; a < 3; ++a) {
+          pp[u][a] = P.a[a][qq];
+          pv[u][a] = P.a[3 + a][qq];
+        }
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < kTrip; ++u) {
+        const uint32_t q = q0 + u * blockDim.x;
+        if (q >= e) break;
+        const float *pos = pp[u];
+        const int c[3] = {int(floorf(pos[0])), int(floorf(pos[1])), int(floorf(pos[2]))};
+        if (!in_bbox(d, c[0], c[1], c[2])) continue;  // removed by DeleteOutsideParticles
+        const int lc[3] = {c[0] - o[0] + 1, c[1] - o[1] + 1, c[2] - o[2] + 1};
+        const float gv[3] = {interp<0>(pos, c, lc, s_v[0]), interp<1>(pos, c, lc, s_v[1]),
+                             interp<2>(pos, c, lc, s_v[2])};
+        const float dv[3] = {interp<0>(pos, c, lc, s_u[0]), interp<1>(pos, c, lc, s_u[1]),
+                             interp<2>(pos, c, lc, s_u[2])};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          const float s1 = dv[a] + pv[u][a];
+          const float s2 = s1 - gv[a];
+          const float s3 = flip * s2;
+          P.a[3 + a][q] = gv[a] + s3;
+          P.a[6 + a][q] = gv[a];
+        }
       }
     }
-#pragma unroll
-    for (uint32_t u = 0; u < kTrip; ++u) {
-      const uint32_t q = q0 + u * blockDim.x;
-      if (q >= e) break;
-      const float *pos = pp[u];
-      const int c[3] = {int(floorf(pos[0])), int(floorf(pos[1])), int(floorf(pos[2]))};
-      if (!in_bbox(d, c[0], c[1], c[2])) continue;  // removed by DeleteOutsideParticles
-      const int lc[3] = {c[0] - o[0] + 1, c[1] - o[1] + 1, c[2] - o[2] + 1};
-      const float gv[3] = {interp<0>(pos, c, lc, s_v[0]), interp<1>(pos, c, lc, s_v[1]),
-                           interp<2>(pos, c, lc, s_v[2])};
-      const float dv[3] = {interp<0>(pos, c, lc, s_u[0]), interp<1>(pos, c, lc, s_u[1]),
-                           interp<2>(pos, c, lc, s_u[2])};
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        const float s1 = dv[a] + pv[u][a];
-        const float s2 = s1 - gv[a];
-        const float s3 = flip * s2;
-        P.a[3 + a][q] = gv[a] + s3;
-        P.a[6 + a][q] = gv[a];
-      }
-    }
-  }
+  });
 }
 
 // Transpose of P2G's normalised transfer: for every particle (outside the
@@ -713,42 +732,44 @@ __global__ __launch_bounds__(256) void k_g2p(Dom d, const int32_t *list, int64_t
 // weights are the trilinear ones of the forward transfer, evaluated with the
 // G2P interpolation.
 __global__ __launch_bounds__(256) void k_p2g_transpose(
-    Dom d, const int32_t *list, int64_t nlist, const uint32_t *lhas, const uint32_t *vstart,
+    Dom d, LeafList L, const uint32_t *lhas, const uint32_t *vstart,
     Particles P, const float *gx, const float *gy, const float *gz, const float *wx,
     const float *wy, const float *wz, float keep, Particles io) {
 #pragma clang fp contract(off)
   __shared__ float s_g[3][1000];
-  const int64_t w = xcd_work(nlist);
-  if (w >= nlist) return;
-  const int64_t leaf = list[w];
-  if (!lhas[leaf]) return;
-  int o[3];
-  leaf_origin(d, leaf, o);
-  for (int idx = threadIdx.x; idx < 1000; idx += blockDim.x) {
-    const int i = o[0] - 1 + idx / 100, j = o[1] - 1 + (idx / 10) % 10, k = o[2] - 1 + idx % 10;
-    float a[3] = {0, 0, 0};
-    if (in_store(d, i, j, k)) {
-      const int64_t v = vox_of(d, i, j, k);
-      const float g[3] = {gx[v], gy[v], gz[v]}, ws[3] = {wx[v], wy[v], wz[v]};
+  for_each_leaf(L.list, L.count, L.n, [&](int64_t leaf) {
+#pragma clang fp contract(off)
+    // The previous leaf's LDS reads are done before the tile is refilled.
+    __syncthreads();
+    if (!lhas[leaf]) return;
+    int o[3];
+    leaf_origin(d, leaf, o);
+    for (int idx = threadIdx.x; idx < 1000; idx += blockDim.x) {
+      const int i = o[0] - 1 + idx / 100, j = o[1] - 1 + (idx / 10) % 10, k = o[2] - 1 + idx % 10;
+      float a[3] = {0, 0, 0};
+      if (in_store(d, i, j, k)) {
+        const int64_t v = vox_of(d, i, j, k);
+        const float g[3] = {gx[v], gy[v], gz[v]}, ws[3] = {wx[v], wy[v], wz[v]};
 #pragma unroll
-      for (int c = 0; c < 3; ++c) a[c] = double(ws[c]) < 1e-5 ? 0.0f : g[c] / ws[c];
+        for (int c = 0; c < 3; ++c) a[c] = double(ws[c]) < 1e-5 ? 0.0f : g[c] / ws[c];
+      }
+      s_g[0][idx] = a[0]; s_g[1][idx] = a[1]; s_g[2][idx] = a[2];
     }
-    s_g[0][idx] = a[0]; s_g[1][idx] = a[1]; s_g[2][idx] = a[2];
-  }
-  __syncthreads();
-  const uint32_t b = vstart[sort_begin(d, leaf)], e = vstart[sort_begin(d, leaf) + kLeaf];
-  for (uint32_t q = b + threadIdx.x; q < e; q += blockDim.x) {
-    const float pos[3] = {P.a[0][q], P.a[1][q], P.a[2][q]};
-    const int c[3] = {int(floorf(pos[0])), int(floorf(pos[1])), int(floorf(pos[2]))};
-    const int lc[3] = {c[0] - o[0] + 1, c[1] - o[1] + 1, c[2] - o[2] + 1};
-    const float t[3] = {interp<0>(pos, c, lc, s_g[0]), interp<1>(pos, c, lc, s_g[1]),
-                        interp<2>(pos, c, lc, s_g[2])};
+    __syncthreads();
+    const uint32_t b = vstart[sort_begin(d, leaf)], e = vstart[sort_begin(d, leaf) + kLeaf];
+    for (uint32_t q = b + threadIdx.x; q < e; q += blockDim.x) {
+      const float pos[3] = {P.a[0][q], P.a[1][q], P.a[2][q]};
+      const int c[3] = {int(floorf(pos[0])), int(floorf(pos[1])), int(floorf(pos[2]))};
+      const int lc[3] = {c[0] - o[0] + 1, c[1] - o[1] + 1, c[2] - o[2] + 1};
+      const float t[3] = {interp<0>(pos, c, lc, s_g[0]), interp<1>(pos, c, lc, s_g[1]),
+                          interp<2>(pos, c, lc, s_g[2])};
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      const float m = keep * io.a[a][q];
-      io.a[a][q] = m + t[a];
+      for (int a = 0; a < 3; ++a) {
+        const float m = keep * io.a[a][q];
+        io.a[a][q] = m + t[a];
+      }
     }
-  }
+  });
 }
 
 __global__ void k_cell_counts_box(Dom d, const uint32_t *vstart, int32_t *out) {
@@ -777,10 +798,13 @@ hipError_t launch_mask_count(fsl_ctx *c, const uint8_t *mask_dev, int64_t n, uin
   return hipGetLastError();
 }
 
-hipError_t launch_advect(fsl_ctx *c, float dt, bool drop_outside, const KillBoxes &kill) {
+hipError_t launch_advect(fsl_ctx *c, float dt, const float *dt_dev, bool drop_outside,
+                         const KillBoxes &kill) {
   if (c->np == 0) return hipSuccess;
+  const uint32_t *np_dev = c->counts_on_device ? c->dcount + kCountParticles : nullptr;
   k_advect<<<blocks_for(c->np, kThreads), kThreads, 0, c->stream>>>(
-      c->dom, c->np, c->P, dt, c->solid, c->key, c->vcount, drop_outside ? 1 : 0, kill);
+      c->dom, c->np, np_dev, c->P, dt, dt_dev, c->solid, c->key, c->vcount,
+      drop_outside ? 1 : 0, kill);
   return hipGetLastError();
 }
 
@@ -799,25 +823,34 @@ hipError_t launch_source_emit(fsl_ctx *c, const SourceBox &s, int64_t ncell, con
 // After the advection kernel and the scan of vcount into vstart: scatter,
 // restore in-cell order, gather into the other buffer and swap. Without
 // keep_gv the grid velocities are left behind (stale until G2P writes them).
+// With the counts on the device, n_alive is not known to the host: the
+// sorted count is read from vstart's last entry by the kernels, the launches
+// are sized by the count before the sort (an upper bound, kept in c->np),
+// and the new count is stored in dcount.
 hipError_t launch_sort_tail(fsl_ctx *c, uint32_t n_alive, bool keep_gv) {
+  const bool dev = c->counts_on_device;
+  const uint32_t *np_dev = dev ? c->dcount + kCountParticles : nullptr;
+  const uint32_t *na_dev = dev ? c->vstart + c->nvox : nullptr;
+  uint32_t *out = dev ? c->dcount + kCountParticles : nullptr;
+  const int64_t nb = dev ? c->np : int64_t(n_alive);
   if (c->np > 0) {
     k_scatter<<<blocks_for(c->np, kThreads), kThreads, 0, c->stream>>>(
-        c->np, c->key, c->vstart, c->vcount, c->perm, c->skey);
+        c->np, np_dev, c->key, c->vstart, c->vcount, c->perm, c->skey);
   }
-  if (n_alive > 0) {
-    k_cell_sort<<<blocks_for(n_alive, kThreads), kThreads, 0, c->stream>>>(n_alive, c->skey,
-                                                                            c->vstart, c->perm);
+  if (nb > 0) {
+    k_cell_sort<<<blocks_for(nb, kThreads), kThreads, 0, c->stream>>>(n_alive, na_dev, c->skey,
+                                                                       c->vstart, c->perm);
     if (keep_gv)
-      k_gather<9><<<blocks_for(n_alive, kThreads), kThreads, 0, c->stream>>>(n_alive, c->perm,
-                                                                              c->P, c->Q);
+      k_gather<9><<<blocks_for(nb, kThreads), kThreads, 0, c->stream>>>(n_alive, na_dev, out,
+                                                                         c->perm, c->P, c->Q);
     else
-      k_gather<6><<<blocks_for(n_alive, kThreads), kThreads, 0, c->stream>>>(n_alive, c->perm,
-                                                                              c->P, c->Q);
+      k_gather<6><<<blocks_for(nb, kThreads), kThreads, 0, c->stream>>>(n_alive, na_dev, out,
+                                                                         c->perm, c->P, c->Q);
   }
   Particles t = c->P;
   c->P = c->Q;
   c->Q = t;
-  c->np = n_alive;
+  if (!dev) c->np = n_alive;
   return hipGetLastError();
 }
 
@@ -827,19 +860,19 @@ hipError_t launch_leaf_flags(fsl_ctx *c) {
   k_leaf_has<<<blocks_for(nl, kThreads), kThreads, 0, c->stream>>>(nl, c->lrank, c->vstart,
                                                                    c->lhas);
   k_leaf_dilate<<<blocks_for(nl, kThreads), kThreads, 0, c->stream>>>(c->dom, c->lhas, c->lact);
-  return launch_compact(c, c->lact, c->act_list);
+  return launch_compact(c, c->lact, c->act_list, c->dcount + kCountActive);
 }
 
 // Compacts the flagged leaves into `list` in Morton order; the count is
-// left in lscan[nleaves].
-hipError_t launch_compact(fsl_ctx *c, const uint32_t *flag, int32_t *list) {
+// left in lscan[nleaves] and stored in *count.
+hipError_t launch_compact(fsl_ctx *c, const uint32_t *flag, int32_t *list, uint32_t *count) {
   const int64_t nl = c->nleaves;
   k_gather_flags<<<blocks_for(nl, kThreads), kThreads, 0, c->stream>>>(nl, c->morton, flag,
                                                                         c->lflag_m);
   hipError_t e = exclusive_scan(c->lflag_m, c->lscan, nl, c->scan_tmp, c->stream);
   if (e != hipSuccess) return e;
   k_compact<<<blocks_for(nl, kThreads), kThreads, 0, c->stream>>>(nl, c->morton, c->lflag_m,
-                                                                   c->lscan, list);
+                                                                   c->lscan, list, count);
   return hipGetLastError();
 }
 
@@ -855,11 +888,11 @@ hipError_t launch_p2g(fsl_ctx *c, float *const wsum[3]) {
   float *ring = static_cast<float *>(c->p2g_ring);
   uint8_t *ring_t = reinterpret_cast<uint8_t *>(ring + 6 * rs);
   float *w[3] = {c->p2g_w, c->p2g_w + c->nvox, c->p2g_w + 2 * c->nvox};
-  k_p2g_own<false><<<xcd_grid(c->n_act), 512, 0, c->stream>>>(
-      c->dom, c->act_list, c->n_act, c->lhas, c->vstart, c->P, c->v[0], c->v[1], c->v[2], w[0],
+  k_p2g_own<false><<<leaf_grid(c->n_act, c->counts_on_device), 512, 0, c->stream>>>(
+      c->dom, act_leaves(c), c->lhas, c->vstart, c->P, c->v[0], c->v[1], c->v[2], w[0],
       w[1], w[2], c->vflag, ring, ring_t, rs);
-  k_p2g_combine<false><<<xcd_grid(c->n_act), 512, 0, c->stream>>>(
-      c->dom, c->act_list, c->n_act, c->lhas, c->vstart, c->v[0], c->v[1], c->v[2], w[0], w[1],
+  k_p2g_combine<false><<<leaf_grid(c->n_act, c->counts_on_device), 512, 0, c->stream>>>(
+      c->dom, act_leaves(c), c->lhas, c->vstart, c->v[0], c->v[1], c->v[2], w[0], w[1],
       w[2], c->vflag, c->marker, ring, ring_t, rs, wsum ? wsum[0] : nullptr,
       wsum ? wsum[1] : nullptr, wsum ? wsum[2] : nullptr);
   return hipGetLastError();
@@ -871,20 +904,20 @@ hipError_t launch_g2p_transpose(fsl_ctx *c, float *const pval[3], float *const o
   for (int a = 0; a < 3; ++a) A.a[3 + a] = pval[a];
   const int64_t rs = c->nleaves * kRing;
   float *ring = static_cast<float *>(c->p2g_ring);
-  k_p2g_own<true><<<xcd_grid(c->n_act), 512, 0, c->stream>>>(
-      c->dom, c->act_list, c->n_act, c->lhas, c->vstart, A, out[0], out[1], out[2], nullptr,
+  k_p2g_own<true><<<leaf_grid(c->n_act, c->counts_on_device), 512, 0, c->stream>>>(
+      c->dom, act_leaves(c), c->lhas, c->vstart, A, out[0], out[1], out[2], nullptr,
       nullptr, nullptr, nullptr, ring, nullptr, rs);
-  k_p2g_combine<true><<<xcd_grid(c->n_act), 512, 0, c->stream>>>(
-      c->dom, c->act_list, c->n_act, c->lhas, c->vstart, out[0], out[1], out[2], nullptr,
+  k_p2g_combine<true><<<leaf_grid(c->n_act, c->counts_on_device), 512, 0, c->stream>>>(
+      c->dom, act_leaves(c), c->lhas, c->vstart, out[0], out[1], out[2], nullptr,
       nullptr, nullptr, nullptr, nullptr, ring, nullptr, rs, nullptr, nullptr, nullptr);
   return hipGetLastError();
 }
 
 hipError_t launch_g2p(fsl_ctx *c) {
   if (c->n_act == 0) return hipSuccess;
-  k_g2p<<<xcd_grid(c->n_act), 256, 0, c->stream>>>(c->dom, c->act_list, c->n_act, c->lhas,
-                                                     c->vstart, c->P, c->v[0], c->v[1], c->v[2],
-                                                     c->u[0], c->u[1], c->u[2], c->flip);
+  k_g2p<<<leaf_grid(c->n_act, c->counts_on_device), 256, 0, c->stream>>>(
+      c->dom, act_leaves(c), c->lhas, c->vstart, c->P, c->v[0], c->v[1], c->v[2], c->u[0],
+      c->u[1], c->u[2], c->flip);
   return hipGetLastError();
 }
 
@@ -893,8 +926,8 @@ hipError_t launch_p2g_transpose(fsl_ctx *c, float *const g[3], float *const wsum
   if (c->n_act == 0) return hipSuccess;
   Particles A{};
   for (int a = 0; a < 3; ++a) A.a[a] = io[a];
-  k_p2g_transpose<<<xcd_grid(c->n_act), 256, 0, c->stream>>>(
-      c->dom, c->act_list, c->n_act, c->lhas, c->vstart, c->P, g[0], g[1], g[2], wsum[0],
+  k_p2g_transpose<<<leaf_grid(c->n_act, c->counts_on_device), 256, 0, c->stream>>>(
+      c->dom, act_leaves(c), c->lhas, c->vstart, c->P, g[0], g[1], g[2], wsum[0],
       wsum[1], wsum[2], keep, A);
   return hipGetLastError();
 }

@@ -27,8 +27,10 @@
 //           the stencil summed in the reference's column order without FMA.
 //   FP64  - double everywhere.
 //   MIXED - float vectors and stencil, double dot products (exact products
-//           accumulated by FMA in each thread; TwoSum/Neumaier-compensated
-//           through the wave, workgroup and cross-workgroup sums), and every `refresh`
+//           accumulated by FMA in each thread, then DPP trees and fixed-order
+//           workgroup and cross-workgroup sums in double: for these sums of
+//           positive terms the rounding stays below about 1e-14 relative; the
+//           two-pass path's totals are Neumaier-compensated), and every `refresh`
 //           iterations the float iterate is folded into a double solution and
 //           the true residual b - A x is recomputed in double (a reliable
 //           update; CG continues with the same direction).
@@ -176,39 +178,25 @@ __device__ inline Walk walk(int64_t nf) {
 __device__ inline int gtid() { return threadIdx.x % kG; }
 
 // Block-reduce K values and publish this workgroup's partial sums: wave
-// sums, one barrier, then thread 0 adds the waves' sums in order. Double
-// values are summed with TwoSum at every step (the error terms are carried
-// and folded in at the end).
+// sums, one barrier, then thread 0 adds the waves' sums in order.
 template <typename A, int K>
 __device__ inline void publish(A vals[K], A *partials) {
   __shared__ A s_red[K][16];
-  __shared__ A s_cor[K][16];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nw = (blockDim.x + 63) >> 6;
-  A cor[K];
 #pragma unroll
-  for (int k = 0; k < K; ++k) {
-    cor[k] = A(0);
-    if constexpr (sizeof(A) == 8) wave_sum_c(vals[k], cor[k]);
-    else vals[k] = wave_sum(vals[k]);
-  }
+  for (int k = 0; k < K; ++k) vals[k] = wave_sum(vals[k]);
   if (lane == 0) {
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      s_red[k][wave] = vals[k];
-      s_cor[k][wave] = cor[k];
-    }
+    for (int k = 0; k < K; ++k) s_red[k][wave] = vals[k];
   }
   __syncthreads();
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      A t = A(0), c = A(0);
-      for (int w = 0; w < nw; ++w) {
-        if constexpr (sizeof(A) == 8) comp_add(t, c, s_red[k][w], s_cor[k][w]);
-        else t += s_red[k][w];
-      }
-      vals[k] = t + c;
+      A t = A(0);
+      for (int w = 0; w < nw; ++w) t += s_red[k][w];
+      vals[k] = t;
     }
 #pragma unroll
     for (int k = 0; k < K; ++k) partials[int64_t(blockIdx.x) * K + k] = vals[k];
@@ -233,46 +221,34 @@ __device__ inline double total(const A *partials, int n, int k) {
 }
 
 // Totals of all K values of the n published (stride K) partials in one pass:
-// one load round and one combined workgroup reduction, the same in every
-// workgroup. Double partials are summed with compensation: Neumaier within a
-// thread's strided partials, TwoSum through the wave and across the waves.
+// one load round and one combined workgroup reduction, in A's precision, the
+// same in every workgroup. A thread adds at most a few partials (n is at most
+// a few thousand workgroups), so plain sums in a fixed order suffice: the
+// rounding error is a few units of 1e-16 relative to the sum of magnitudes.
 template <typename A, int K>
 __device__ inline void total_all(const A *partials, int n, double out[K]) {
   __shared__ A s_t[K][16];
-  __shared__ A s_c[K][16];
-  A v[K], cv[K];
+  A v[K];
 #pragma unroll
-  for (int j = 0; j < K; ++j) v[j] = cv[j] = A(0);
+  for (int j = 0; j < K; ++j) v[j] = A(0);
   for (int q = threadIdx.x; q < n; q += blockDim.x) {
 #pragma unroll
-    for (int j = 0; j < K; ++j) {
-      if constexpr (sizeof(A) == 8) neumaier_add(v[j], cv[j], partials[K * q + j]);
-      else v[j] += partials[K * q + j];
-    }
+    for (int j = 0; j < K; ++j) v[j] += partials[K * q + j];
   }
 #pragma unroll
-  for (int j = 0; j < K; ++j) {
-    if constexpr (sizeof(A) == 8) wave_sum_c(v[j], cv[j]);
-    else v[j] = wave_sum(v[j]);
-  }
+  for (int j = 0; j < K; ++j) v[j] = wave_sum(v[j]);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nw = (blockDim.x + 63) >> 6;
   if (lane == 0) {
 #pragma unroll
-    for (int j = 0; j < K; ++j) {
-      s_t[j][wave] = v[j];
-      s_c[j][wave] = cv[j];
-    }
+    for (int j = 0; j < K; ++j) s_t[j][wave] = v[j];
   }
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < K; ++j) {
-    A t = A(0), c = A(0);
-    for (int w = 0; w < nw; ++w) {
-      if constexpr (sizeof(A) == 8) comp_add(t, c, s_t[j][w], s_c[j][w]);
-      else t += s_t[j][w];
-    }
-    out[j] = double(t + c);
+    A t = A(0);
+    for (int w = 0; w < nw; ++w) t += s_t[j][w];
+    out[j] = double(t);
   }
 }
 
@@ -1440,50 +1416,65 @@ __global__ __launch_bounds__(1024) void k_stage_pap(const A *part_a, int g, PcgS
 // Dot products summed in the oracle's row order (FSL_DOT_ROW_ORDER): over
 // the FLUID rows in lexicographic (i, j, k) order, the row index order of
 // CreateIndexAndMaskTree (solver.cc:114-151), one rounding per product and
-// per sum, s = s + a_i b_i, as a sequential loop. K = 1: a0.b0; K = 2: also
-// a1.b1. The totals replace a pass's published partials: entry 0 of `part`
-// holds them and every other workgroup slot holds 0, so the pass's own
-// total over g partials returns them unchanged. One workgroup: chunks of
-// 256 cells are loaded and multiplied in parallel, and thread 0 adds them in
-// order (double-buffered, one barrier per chunk). For checks of the stage
-// arithmetic on small systems, not for speed.
+// per sum, s = s + a_i b_i, as a sequential loop. The row list (voxel of
+// each row, in row order) is built once per solve: k_row_flag marks the
+// rows over the cells in lexicographic order, a scan numbers them and
+// k_row_list writes each row's voxel at its number.
+__global__ void k_row_flag(Dom d, const uint8_t *lab, uint32_t *flag) {
+  const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t n12 = int64_t(d.n[1]) * d.n[2];
+  if (q >= int64_t(d.n[0]) * n12) return;
+  const int i = int(q / n12), j = int((q / d.n[2]) % d.n[1]), k = int(q % d.n[2]);
+  flag[q] = is_fluid(lab[vox_of(d, i, j, k)]) ? 1u : 0u;
+}
+__global__ void k_row_list(Dom d, const uint8_t *lab, const uint32_t *offs, uint32_t *rows) {
+  const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t n12 = int64_t(d.n[1]) * d.n[2];
+  if (q >= int64_t(d.n[0]) * n12) return;
+  const int i = int(q / n12), j = int((q / d.n[2]) % d.n[1]), k = int(q % d.n[2]);
+  const int64_t v = vox_of(d, i, j, k);
+  if (is_fluid(lab[v])) rows[offs[q]] = uint32_t(v);
+}
+// The sequential sums of a0.b0 (K = 1) and also a1.b1 (K = 2) over the row
+// list. One workgroup: chunks of 1024 rows are loaded and multiplied by all
+// threads into a double-buffered LDS chunk, and thread 0 adds each chunk in
+// order while the next one loads (one barrier per chunk). The totals replace
+// a pass's published partials: entry 0 of `part` holds them and every other
+// workgroup slot holds 0, so the pass's own total over g partials returns
+// them unchanged. For checks of the stage arithmetic against the oracle, not
+// for speed.
+constexpr int kRowChunk = 1024;
 template <typename V, typename A, int K>
-__global__ __launch_bounds__(256) void k_rowdot(Dom d, const uint8_t *lab, const V *a0,
-                                                const V *b0, const V *a1, const V *b1, A *part,
-                                                int g, const PcgState *st, int force) {
+__global__ __launch_bounds__(kRowChunk) void k_rowdot(const uint32_t *rows,
+                                                      const unsigned long long *nrows,
+                                                      const V *a0, const V *b0, const V *a1,
+                                                      const V *b1, A *part, int g,
+                                                      const PcgState *st, int force) {
 #pragma clang fp contract(off)
-  __shared__ A s_m[2][K][256];
-  __shared__ uint8_t s_f[2][256];
+  __shared__ A s_m[2][K][kRowChunk];
   if ((!force && st->done) || st->status) return;
   const int t = threadIdx.x;
-  const int64_t n12 = int64_t(d.n[1]) * d.n[2];
-  const int64_t ncell = int64_t(d.n[0]) * n12;
+  const int64_t n = int64_t(*nrows);
   A sum[K];
 #pragma unroll
   for (int j = 0; j < K; ++j) sum[j] = A(0);
   int buf = 0;
-  for (int64_t q0 = 0; q0 < ncell; q0 += 256, buf ^= 1) {
-    const int64_t q = q0 + t;
-    uint8_t f = 0;
+  for (int64_t c0 = 0; c0 < n; c0 += kRowChunk, buf ^= 1) {
+    const int64_t q = c0 + t;
     A m[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) m[j] = A(0);
-    if (q < ncell) {
-      const int i = int(q / n12), jj = int((q / d.n[2]) % d.n[1]), k = int(q % d.n[2]);
-      const int64_t v = vox_of(d, i, jj, k);
-      f = is_fluid(lab[v]) ? 1 : 0;
-      if (f) {
-        m[0] = A(a0[v]) * A(b0[v]);
-        if (K == 2) m[K - 1] = A(a1[v]) * A(b1[v]);
-      }
+    if (q < n) {
+      const uint32_t v = rows[q];
+      m[0] = A(a0[v]) * A(b0[v]);
+      if (K == 2) m[K - 1] = A(a1[v]) * A(b1[v]);
     }
 #pragma unroll
     for (int j = 0; j < K; ++j) s_m[buf][j][t] = m[j];
-    s_f[buf][t] = f;
     __syncthreads();
     if (t == 0) {
-      for (int u = 0; u < 256; ++u) {
-        if (!s_f[buf][u]) continue;
+      const int cnt = n - c0 < kRowChunk ? int(n - c0) : kRowChunk;
+      for (int u = 0; u < cnt; ++u) {
 #pragma unroll
         for (int j = 0; j < K; ++j) sum[j] = sum[j] + s_m[buf][j][u];
       }
@@ -1494,7 +1485,7 @@ __global__ __launch_bounds__(256) void k_rowdot(Dom d, const uint8_t *lab, const
 #pragma unroll
     for (int j = 0; j < K; ++j) part[j] = sum[j];
   }
-  for (int e = K + t; e < K * g; e += 256) part[e] = A(0);
+  for (int e = K + t; e < K * g; e += kRowChunk) part[e] = A(0);
 }
 
 // After a forced refresh: the true r.r decides whether the solve is done.
@@ -1632,7 +1623,10 @@ struct Kernels {
         c->dom, c->fluid_list, c->n_fluid, c->lab, c->div, c->pres, warm, (V *)c->x, (V *)c->r,
         (V *)c->z, c->x64, (A *)P.b[1], (A *)P.bb);
     if (c->pcg_ic) ic_apply(c, P.b[1], 0);
-    if (c->pcg_roworder) rowdot_b(c, P.b[1], 1);
+    if (c->pcg_roworder) {
+      build_row_list(c);
+      rowdot_b(c, P.b[1], 1);
+    }
     k_prepare_scalars<A><<<1, 1024, 0, c->stream>>>((const A *)P.b[1], (const A *)P.bb, g, c->pcg);
   }
   // Block IC(0) in place of B's Jacobi z: z = M^-1 r, then (r.r, r.z) into
@@ -1663,16 +1657,29 @@ struct Kernels {
                           (const V *)c->p[(k + 1) & 1], (V *)c->p[k & 1], (V *)c->Ap,
                           (const A *)P.b[(k + 1) & 1], (A *)P.a, g, k, c->pcg);
     if (c->pcg_roworder)
-      k_rowdot<V, A, 1><<<1, 256, 0, c->stream>>>(c->dom, c->lab, (const V *)c->p[k & 1],
-                                                  (const V *)c->Ap, nullptr, nullptr, (A *)P.a,
-                                                  g, c->pcg, 0);
+      k_rowdot<V, A, 1><<<1, kRowChunk, 0, c->stream>>>(
+          row_list(c), &c->flags->n_rows, (const V *)c->p[k & 1], (const V *)c->Ap, nullptr,
+          nullptr, (A *)P.a, g, c->pcg, 0);
+  }
+  // The row list of the row-order dot products lives in the x64 buffer
+  // (used by the mixed solve only, which row order excludes).
+  static const uint32_t *row_list(fsl_ctx *c) { return reinterpret_cast<uint32_t *>(c->x64); }
+  static void build_row_list(fsl_ctx *c) {
+    const Dom &d = c->dom;
+    const int64_t ncell = int64_t(d.n[0]) * d.n[1] * d.n[2];
+    // Row numbers in the Ap buffer (free until the first pass A).
+    uint32_t *offs = reinterpret_cast<uint32_t *>(c->Ap);
+    const unsigned blocks = unsigned((ncell + 255) / 256);
+    k_row_flag<<<blocks, 256, 0, c->stream>>>(d, c->lab, offs);
+    (void)exclusive_scan(offs, offs, ncell, c->scan_tmp, c->stream);
+    k_row_list<<<blocks, 256, 0, c->stream>>>(d, c->lab, offs, reinterpret_cast<uint32_t *>(c->x64));
   }
   // (r.r, r.z) of B(k) (or of the preconditioner apply completing it) in the
   // oracle's row order, in place of the pass's partials.
   static void rowdot_b(fsl_ctx *c, void *part, int force) {
-    k_rowdot<V, A, 2><<<1, 256, 0, c->stream>>>(c->dom, c->lab, (const V *)c->r, (const V *)c->r,
-                                                (const V *)c->r, (const V *)c->z, (A *)part,
-                                                grid_for(c->n_fluid), c->pcg, force);
+    k_rowdot<V, A, 2><<<1, kRowChunk, 0, c->stream>>>(
+        row_list(c), &c->flags->n_rows, (const V *)c->r, (const V *)c->r, (const V *)c->r,
+        (const V *)c->z, (A *)part, grid_for(c->n_fluid), c->pcg, force);
   }
   static void pass_b(fsl_ctx *c, int k, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
     const int g = grid_for(c->n_fluid);
@@ -2016,7 +2023,7 @@ static int run_pipelined(fsl_ctx *ctx, int &k, int batch, bool adaptive, Issue &
   return sync_state(ctx);
 }
 
-static int pcg_run_sr(fsl_ctx *ctx, const fsl_solve_params *prm, fsl_solve_stats *st) {
+static int pcg_iterations_sr(fsl_ctx *ctx, const fsl_solve_params *prm, fsl_solve_stats *st) {
   int rc = pcg_setup(ctx, prm);
   if (rc) return rc;
   ctx->pcg_sr = true;
@@ -2133,12 +2140,11 @@ static int pcg_run_sr(fsl_ctx *ctx, const fsl_solve_params *prm, fsl_solve_stats
   FSL_HIP(hipEventDestroy(e0));
   FSL_HIP(hipEventDestroy(e1));
   if (st) st->seconds = double(ms) * 1e-3;
-  if (ctx->pcg_host->status) return ctx->pcg_host->status;
-  return pcg_finish(ctx, st);
+  return ctx->pcg_host->status;
 }
 
-int pcg_run(fsl_ctx *ctx, const fsl_solve_params *prm, fsl_solve_stats *st) {
-  if (pcg_single_reduction(prm)) return pcg_run_sr(ctx, prm, st);
+int pcg_solve_iterations(fsl_ctx *ctx, const fsl_solve_params *prm, fsl_solve_stats *st) {
+  if (pcg_single_reduction(prm)) return pcg_iterations_sr(ctx, prm, st);
   int rc = pcg_prepare(ctx, prm);
   if (rc) return rc;
   const int batch = prm->batch > 0 ? prm->batch : 16;
@@ -2252,7 +2258,12 @@ int pcg_run(fsl_ctx *ctx, const fsl_solve_params *prm, fsl_solve_stats *st) {
   FSL_HIP(hipEventDestroy(e0));
   FSL_HIP(hipEventDestroy(e1));
   if (st) st->seconds = double(ms) * 1e-3;
-  if (ctx->pcg_host->status) return ctx->pcg_host->status;
+  return ctx->pcg_host->status;
+}
+
+int pcg_run(fsl_ctx *ctx, const fsl_solve_params *prm, fsl_solve_stats *st) {
+  const int rc = pcg_solve_iterations(ctx, prm, st);
+  if (rc) return rc;
   return pcg_finish(ctx, st);
 }
 
@@ -2267,15 +2278,26 @@ int pcg_true_residual(fsl_ctx *ctx, double *rr_true) {
   return FSL_OK;
 }
 
-int pcg_finish(fsl_ctx *ctx, fsl_solve_stats *st) {
+int pcg_finish_enqueue(fsl_ctx *ctx) {
   if (!ctx->pcg_ready) { ctx->err = "pcg not prepared"; return FSL_ERR_STATE; }
   if (ctx->n_act > 0) {
     dispatch(ctx->pcg_precision, [&](auto k) { decltype(k)::finish(ctx); });
     FSL_HIP(hipGetLastError());
   }
-  double rr_true = 0.0;
-  int rc = pcg_true_residual(ctx, &rr_true);
-  if (rc) return rc;
+  if (ctx->n_fluid > 0) {
+    dispatch(ctx->pcg_precision, [&](auto k) { decltype(k)::true_residual(ctx); });
+    FSL_HIP(hipGetLastError());
+  }
+  if (!ctx->ev_finish) FSL_HIP(hipEventCreateWithFlags(&ctx->ev_finish, hipEventDisableTiming));
+  FSL_HIP(hipMemcpyAsync(ctx->pcg_host, ctx->pcg, sizeof(PcgState), hipMemcpyDeviceToHost,
+                         ctx->stream));
+  FSL_HIP(hipEventRecord(ctx->ev_finish, ctx->stream));
+  return FSL_OK;
+}
+
+int pcg_finish_collect(fsl_ctx *ctx, fsl_solve_stats *st) {
+  FSL_HIP(hipEventSynchronize(ctx->ev_finish));
+  const double rr_true = ctx->n_fluid > 0 ? ctx->pcg_host->rr_true : 0.0;
   if (st) {
     st->iterations = ctx->pcg_host->it;
     st->rr = ctx->pcg_host->rr;
@@ -2284,6 +2306,12 @@ int pcg_finish(fsl_ctx *ctx, fsl_solve_stats *st) {
     st->n_fluid_leaves = int(ctx->n_fluid);
   }
   return ctx->pcg_host->status;
+}
+
+int pcg_finish(fsl_ctx *ctx, fsl_solve_stats *st) {
+  const int rc = pcg_finish_enqueue(ctx);
+  if (rc) return rc;
+  return pcg_finish_collect(ctx, st);
 }
 
 }  // namespace fsl

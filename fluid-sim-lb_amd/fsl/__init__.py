@@ -52,6 +52,16 @@ class SolveStats(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class FrameStats(C.Structure):
+    _fields_ = [("steps", C.c_int), ("pcg_iterations", C.c_int), ("dt_last", C.c_float),
+                ("last_solve", SolveStats)]
+
+    def as_dict(self):
+        d = {f: getattr(self, f) for f, _ in self._fields_[:3]}
+        d["last_solve"] = self.last_solve.as_dict()
+        return d
+
+
 # Exported symbols and their argument types (the declarations of include/fsl.h).
 _P = C.c_void_p
 _I = C.c_int
@@ -89,6 +99,7 @@ SIGNATURES = {
     "fsl_extrapolate": [_P],
     "fsl_g2p": [_P],
     "fsl_step": [_P, _F, C.POINTER(SolveParams), C.POINTER(SolveStats)],
+    "fsl_advance_frame": [_P, _I, _F, C.POINTER(SolveParams), C.POINTER(FrameStats)],
     "fsl_compute_dt": [_P, _I, _F, _FP, _FP],
     "fsl_get_vec": [_P, _I, _P, _P],
     "fsl_set_vec": [_P, _I, _P, _P],
@@ -307,6 +318,12 @@ class Context:
         st = SolveStats()
         self.call("fsl_step", float(dt), C.byref(params), C.byref(st))
         return st.as_dict()
+
+    def advance_frame(self, cfl, frame_step, params: SolveParams) -> dict:
+        """One frame of FlipDriver's loop with dt computed on the device."""
+        fs = FrameStats()
+        self.call("fsl_advance_frame", int(cfl), float(frame_step), C.byref(params), C.byref(fs))
+        return fs.as_dict()
 
     def project_forward(self, params: SolveParams) -> dict:
         st = SolveStats()

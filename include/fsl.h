@@ -192,8 +192,24 @@ int fsl_extrapolate(fsl_ctx *ctx);
 /* UpdateParticlesFromGrid (flip_simulation.cc:1024-1189), incl. extrapolation. */
 int fsl_g2p(fsl_ctx *ctx);
 
-/* FlipDriver::RunOneStep (flip_app_sim.cc:211-254) with a given dt. */
+/* FlipDriver::RunOneStep (flip_app_sim.cc:211-254) with a given dt. The
+ * particle count and the leaf lists stay on the device from the advection
+ * to the divergence; the host reads them once per step, to size the solve. */
 int fsl_step(fsl_ctx *ctx, float dt, const fsl_solve_params *params, fsl_solve_stats *stats);
+
+typedef struct fsl_frame_stats {
+  int steps;              /* RunOneStep calls in the frame */
+  int pcg_iterations;     /* summed over the frame's solves */
+  float dt_last;          /* dt of the frame's last step */
+  fsl_solve_stats last_solve;
+} fsl_frame_stats;
+/* One frame of FlipDriver's loop (flip_app_sim.cc:357-462): While
+ * (step + 1e-5 < frame_step) { WhileAndComputeDt; RunOneStep }, with dt
+ * computed on the device from the CFL maximum (cfl * CFL(), min with
+ * 1/frame_step, floor at frame_step/10, cut and snapped at the frame's end)
+ * and read by the advection and gravity there; the elapsed time starts at 0. */
+int fsl_advance_frame(fsl_ctx *ctx, int cfl, float frame_step, const fsl_solve_params *params,
+                      fsl_frame_stats *stats);
 /* WhileAndComputeDt's dt (flip_app_sim.cc:371-410): cfl * CFL(), min with
  * 1/frame_step, floor at frame_step/10, snap to the frame end; advances *step. */
 int fsl_compute_dt(fsl_ctx *ctx, int cfl, float frame_step, float *step, float *dt);

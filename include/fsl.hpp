@@ -674,7 +674,7 @@ class FlipDriver {
   /* One frame: steps until the frame's time is used up (the While loop of
    * flip_app_sim.cc:357-369), then ResetDt and AdvanceTime. */
   void NextFrame() {
-    while (step_ + 1e-5f < frame_step_) RunOneStep(ComputeDt());
+    while (double(step_) + 1e-5 < double(frame_step_)) RunOneStep(ComputeDt());
     step_ = 0;
     time_ += frame_step_;
     ++frame_;

@@ -58,7 +58,8 @@ def test_struct_layouts_match_the_header(tmp_path):
 
     import fsl
 
-    structs = {"fsl_solve_params": fsl.SolveParams, "fsl_solve_stats": fsl.SolveStats}
+    structs = {"fsl_solve_params": fsl.SolveParams, "fsl_solve_stats": fsl.SolveStats,
+               "fsl_frame_stats": fsl.FrameStats}
     lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "fsl.h"', "int main(void) {"]
     for cname, py in structs.items():
         lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
