@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "fsl_internal.h"
@@ -109,11 +110,16 @@ int check_step_flags(fsl_ctx *ctx) {
 // device: particle count, active leaves and FLUID leaves stay on the device
 // from the advection to the divergence, and the host reads them (with the
 // solver flags and the device dt) once, after the divergence, to size the
-// solve. dt_dev, when set, holds dt on the device (fsl_advance_frame). The
+// solve. The step is two halves: step_front enqueues everything up to that
+// read (it can be captured in a HIP graph: with graph set, every launch is
+// sized by an upper bound and reads its count on the device, so the
+// captured launches are valid for any later step), step_back waits for the
+// counts, solves and enqueues the rest. dt_dev, when set, holds dt on the
+// device; with device_dt the front also computes it (CFL and k_dt). The
 // velocity update's flag is copied to host_u32[16] and checked by the
 // caller (check_step_flags) once the stream has drained.
-int run_step(fsl_ctx *ctx, float dt, const float *dt_dev, const fsl_solve_params *params,
-             fsl_solve_stats *stats) {
+int step_front(fsl_ctx *ctx, float dt, const float *dt_dev, bool device_dt, bool graph,
+               const fsl::DtParams *dtp) {
   int rc;
   auto mark = [&](int ev) -> int {
     if (ctx->timing) FSL_HIP(hipEventRecord(ctx->tev[ev], ctx->stream));
@@ -123,13 +129,22 @@ int run_step(fsl_ctx *ctx, float dt, const float *dt_dev, const fsl_solve_params
   if ((rc = mark(2 * (s)))) return rc;     \
   body;                                    \
   if ((rc = mark(2 * (s) + 1))) return rc;
-  // The host's counts go to the device; from here on they are bounds.
+  // The host's counts go to the device (staged in host_u32[8..10], which the
+  // caller fills before a graph replay); from here on they are bounds.
   ctx->host_u32[8 + fsl::kCountParticles] = uint32_t(ctx->np);
   ctx->host_u32[8 + fsl::kCountActive] = uint32_t(ctx->n_act);
   ctx->host_u32[8 + fsl::kCountFluid] = uint32_t(ctx->n_fluid);
   FSL_HIP(hipMemcpyAsync(ctx->dcount, ctx->host_u32 + 8, 3 * sizeof(uint32_t),
                          hipMemcpyHostToDevice, ctx->stream));
   ctx->counts_on_device = true;
+  if (graph) ctx->n_act = ctx->nleaves;
+  if (device_dt) {
+    // WhileAndComputeDt: the CFL maximum of the current grid velocity, then
+    // dt on the device.
+    FSL_HIP(fsl::launch_fill_u32(ctx, &ctx->flags->cfl_bits, 1, 0u));
+    if (ctx->grid_ready && ctx->n_act > 0) FSL_HIP(fsl::launch_cfl(ctx));
+    FSL_HIP(fsl::launch_dt(ctx, *dtp));
+  }
   // MoveParticles + DeleteParticlesInSolids. Within a step nothing reads the
   // particles' grid velocities between the move and G2P, which rewrites
   // them: the sort leaves them behind.
@@ -163,21 +178,31 @@ int run_step(fsl_ctx *ctx, float dt, const float *dt_dev, const fsl_solve_params
     FSL_HIP(fsl::launch_divergence(ctx));
     FSL_HIP(fsl::launch_compact(ctx, ctx->lfluid, ctx->fluid_list,
                                 ctx->dcount + fsl::kCountFluid));
-    // The counts, the flags and the step state in one round trip.
+    // The counts, the flags and the step state, read by step_back in one
+    // round trip.
     FSL_HIP(hipMemcpyAsync(ctx->host_u32, ctx->dcount, 3 * sizeof(uint32_t),
                            hipMemcpyDeviceToHost, ctx->stream));
     FSL_HIP(hipMemcpyAsync(ctx->flags_host, ctx->flags, sizeof(fsl::DevFlags),
                            hipMemcpyDeviceToHost, ctx->stream));
     FSL_HIP(hipMemcpyAsync(ctx->sstate_host, ctx->sstate, sizeof(fsl::StepState),
                            hipMemcpyDeviceToHost, ctx->stream));
-    if (!ctx->ev_counts) FSL_HIP(hipEventCreateWithFlags(&ctx->ev_counts, hipEventDisableTiming));
-    FSL_HIP(hipEventRecord(ctx->ev_counts, ctx->stream));
-    FSL_HIP(hipEventSynchronize(ctx->ev_counts));
-    ctx->np = ctx->host_u32[fsl::kCountParticles];
-    ctx->n_act = ctx->host_u32[fsl::kCountActive];
-    ctx->n_fluid = ctx->host_u32[fsl::kCountFluid];
-    ctx->counts_on_device = false;
   })
+  return FSL_OK;
+}
+
+int step_back(fsl_ctx *ctx, const fsl_solve_params *params, fsl_solve_stats *stats) {
+  int rc;
+  auto mark = [&](int ev) -> int {
+    if (ctx->timing) FSL_HIP(hipEventRecord(ctx->tev[ev], ctx->stream));
+    return FSL_OK;
+  };
+  if (!ctx->ev_counts) FSL_HIP(hipEventCreateWithFlags(&ctx->ev_counts, hipEventDisableTiming));
+  FSL_HIP(hipEventRecord(ctx->ev_counts, ctx->stream));
+  FSL_HIP(hipEventSynchronize(ctx->ev_counts));
+  ctx->np = ctx->host_u32[fsl::kCountParticles];
+  ctx->n_act = ctx->host_u32[fsl::kCountActive];
+  ctx->n_fluid = ctx->host_u32[fsl::kCountFluid];
+  ctx->counts_on_device = false;
   if (ctx->flags_host->zero_diag) {
     ctx->err = "FLUID cell with no non-SOLID neighbour (zero Jacobi diagonal)";
     return FSL_ERR_ZERO_DIAGONAL;
@@ -216,6 +241,75 @@ int run_step(fsl_ctx *ctx, float dt, const float *dt_dev, const fsl_solve_params
     }
     ctx->tacc[FSL_T_STEPS] += 1;
   }
+  return FSL_OK;
+}
+
+int run_step(fsl_ctx *ctx, float dt, const float *dt_dev, const fsl_solve_params *params,
+             fsl_solve_stats *stats) {
+  int rc = step_front(ctx, dt, dt_dev, false, false, nullptr);
+  if (rc) return rc;
+  return step_back(ctx, params, stats);
+}
+
+// The front half of a device-dt step as a HIP graph, one per particle
+// buffer (the sort gathers into the other buffer and the buffers swap each
+// step), captured on first use and again when the particle count bound
+// grows (it sizes the particle launches).
+int step_front_graph(fsl_ctx *ctx, const fsl::DtParams &p) {
+  const int parity = ctx->P.a[0] == ctx->pbuf0 ? 0 : 1;
+  fsl::StepGraph &g = ctx->graphs[parity];
+  const bool fresh = !g.exec || ctx->np > g.np || std::memcmp(&g.dtp, &p, sizeof p) != 0 ||
+                     ctx->outside_deleted != g.outside_deleted;
+  if (fresh) {
+    if (g.exec) {
+      (void)hipGraphExecDestroy(g.exec);
+      g.exec = nullptr;
+    }
+    g.np = ctx->np;
+    g.dtp = p;
+    g.outside_deleted = ctx->outside_deleted;
+    // Capturing only records the launches: the host state the front
+    // changes is restored afterwards (the replay below runs them).
+    const fsl::Particles P = ctx->P, Q = ctx->Q;
+    const int64_t np = ctx->np, n_act = ctx->n_act, n_fluid = ctx->n_fluid;
+    const bool od = ctx->outside_deleted, gr = ctx->grid_ready, bc = ctx->bc_applied;
+    FSL_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = step_front(ctx, 0.0f, &ctx->sstate->dt, true, true, &p);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    ctx->P = P;
+    ctx->Q = Q;
+    ctx->np = np;
+    ctx->n_act = n_act;
+    ctx->n_fluid = n_fluid;
+    ctx->outside_deleted = od;
+    ctx->grid_ready = gr;
+    ctx->bc_applied = bc;
+    ctx->counts_on_device = false;
+    if (rc) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    FSL_HIP(e);
+    const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    FSL_HIP(ei);
+  }
+  ctx->host_u32[8 + fsl::kCountParticles] = uint32_t(ctx->np);
+  ctx->host_u32[8 + fsl::kCountActive] = uint32_t(ctx->n_act);
+  ctx->host_u32[8 + fsl::kCountFluid] = uint32_t(ctx->n_fluid);
+  FSL_HIP(hipGraphLaunch(g.exec, ctx->stream));
+  // The host-side effects of the front (as step_front leaves them).
+  const fsl::Particles t = ctx->P;
+  ctx->P = ctx->Q;
+  ctx->Q = t;
+  ctx->counts_on_device = true;
+  ctx->n_act = ctx->nleaves;
+  ctx->n_fluid = ctx->nleaves;
+  ctx->grid_ready = true;
+  ctx->pcg_ready = false;
+  ctx->bc_applied = true;
+  ctx->outside_deleted = true;
   return FSL_OK;
 }
 
@@ -328,6 +422,7 @@ int fsl_create(fsl_ctx **out, const int dims[3], const int tiles[3], int device,
   if (!rc) rc = dev_alloc(ctx, &ctx->fluid_list, nl);
   for (int a = 0; a < 9 && !rc; ++a) rc = dev_alloc(ctx, &ctx->P.a[a], size_t(ctx->cap));
   for (int a = 0; a < 9 && !rc; ++a) rc = dev_alloc(ctx, &ctx->Q.a[a], size_t(ctx->cap));
+  ctx->pbuf0 = ctx->P.a[0];
   if (!rc) rc = dev_alloc(ctx, &ctx->key, size_t(ctx->cap));
   if (!rc) rc = dev_alloc(ctx, &ctx->perm, size_t(ctx->cap));
   if (!rc) rc = dev_alloc(ctx, &ctx->skey, size_t(ctx->cap));
@@ -378,6 +473,8 @@ int fsl_destroy(fsl_ctx *ctx) {
   if (ctx->src_buf) (void)hipFree(ctx->src_buf);
   for (hipEvent_t e : ctx->tev) if (e) (void)hipEventDestroy(e);
   if (ctx->ev_finish) (void)hipEventDestroy(ctx->ev_finish);
+  for (fsl::StepGraph &g : ctx->graphs)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
   if (ctx->ev_counts) (void)hipEventDestroy(ctx->ev_counts);
   if (ctx->flags_host) (void)hipHostFree(ctx->flags_host);
   if (ctx->pcg_host) (void)hipHostFree(ctx->pcg_host);
@@ -784,17 +881,23 @@ int fsl_advance_frame(fsl_ctx *ctx, int cfl, float frame_step, const fsl_solve_p
   p.dt_max = float(1.0 / double(frame_step));
   p.dt_min = frame_step / 10;
   int rc;
+  // The frame's elapsed time starts at 0.
+  FSL_HIP(fsl::launch_fill_u32(ctx, reinterpret_cast<uint32_t *>(&ctx->sstate->step), 1, 0u));
   for (int s = 0;; ++s) {
     if (s >= 100000) {
       ctx->err = "frame did not end";
       return FSL_ERR_STATE;
     }
-    // CFL of the current grid velocity and dt on the device.
-    FSL_HIP(fsl::launch_fill_u32(ctx, &ctx->flags->cfl_bits, 1, 0u));
-    if (ctx->grid_ready && ctx->n_act > 0) FSL_HIP(fsl::launch_cfl(ctx));
-    FSL_HIP(fsl::launch_dt(ctx, p, s == 0 ? 1 : 0));
+    // The front half (CFL and dt on the device, then the stages up to the
+    // divergence), replayed from a graph once the grids exist and the
+    // timing hook is off.
+    if (ctx->use_graph && ctx->grid_ready && !ctx->timing && ctx->outside_deleted)
+      rc = step_front_graph(ctx, p);
+    else
+      rc = step_front(ctx, 0.0f, &ctx->sstate->dt, true, false, &p);
+    if (rc) return rc;
     fsl_solve_stats st{};
-    if ((rc = run_step(ctx, 0.0f, &ctx->sstate->dt, params, &st))) return rc;
+    if ((rc = step_back(ctx, params, &st))) return rc;
     fs->steps += 1;
     fs->pcg_iterations += st.iterations;
     fs->dt_last = ctx->sstate_host->dt;
@@ -802,6 +905,12 @@ int fsl_advance_frame(fsl_ctx *ctx, int cfl, float frame_step, const fsl_solve_p
     if (!ctx->sstate_host->more) break;
   }
   return check_step_flags(ctx);
+}
+
+int fsl_set_step_graph(fsl_ctx *ctx, int enable) {
+  CHECK_CTX();
+  ctx->use_graph = enable != 0;
+  return FSL_OK;
 }
 
 int fsl_set_timing(fsl_ctx *ctx, int enable) {

@@ -140,6 +140,15 @@ struct DtParams {
   float cfl, frame_step, dt_max, dt_min;  // dt_max = 1 / frame_step, dt_min = frame_step / 10
 };
 
+// A captured front half of a device-resident step (api.hip) and what it
+// was captured with.
+struct StepGraph {
+  hipGraphExec_t exec = nullptr;
+  int64_t np = 0;
+  DtParams dtp{};
+  bool outside_deleted = false;
+};
+
 struct DevFlags {
   unsigned zero_diag;
   unsigned vel_update;
@@ -216,6 +225,9 @@ struct fsl_ctx {
   uint32_t *dcount = nullptr;
   bool counts_on_device = false;
   fsl::StepState *sstate = nullptr, *sstate_host = nullptr;  // device and pinned mirror
+  bool use_graph = false;          // fsl_advance_frame replays its steps' front halves
+  fsl::StepGraph graphs[2];        // by particle buffer (which one P is)
+  float *pbuf0 = nullptr;          // the first particle buffer's px array
   hipEvent_t ev_finish = nullptr;  // the solver state's copy after the solve's finish
   hipEvent_t ev_counts = nullptr;  // the counts' copy within a device-resident step
   bool bc_applied = false, grid_ready = false, outside_deleted = false;
@@ -297,7 +309,7 @@ hipError_t launch_extrapolate(fsl_ctx *c);
 hipError_t launch_clear_shell(fsl_ctx *c);
 hipError_t launch_cfl(fsl_ctx *c);
 // dt of the next step on the device, from the CFL maximum in flags->cfl_bits.
-hipError_t launch_dt(fsl_ctx *c, const DtParams &p, int reset_frame);
+hipError_t launch_dt(fsl_ctx *c, const DtParams &p);
 hipError_t launch_fill_u32(fsl_ctx *c, uint32_t *p, int64_t n, uint32_t value);
 hipError_t launch_box_vec(fsl_ctx *c, int which, float *out3, uint8_t *act, bool to_box);
 hipError_t launch_box_scalar(fsl_ctx *c, float *grid, float *box, bool to_box);

@@ -280,8 +280,9 @@ __global__ __launch_bounds__(512) void k_cfl(Dom d, LeafList L, const uint8_t *v
 // from k_cfl), dt = min(1 / frame_step, cfl / speed), floored at frame_step
 // / 10, cut at the frame's end and snapped to it past 90% of the frame; the
 // elapsed time advances by dt. The float/double mix is the host's
-// (fsl_compute_dt). reset: the step starts a frame.
-__global__ void k_dt(const unsigned *cfl_bits, DtParams p, int reset, StepState *ss) {
+// (fsl_compute_dt). The frame's elapsed time is reset to 0 before its first
+// step.
+__global__ void k_dt(const unsigned *cfl_bits, DtParams p, StepState *ss) {
 #pragma clang fp contract(off)
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const float v_inf = __uint_as_float(*cfl_bits);
@@ -289,7 +290,7 @@ __global__ void k_dt(const unsigned *cfl_bits, DtParams p, int reset, StepState 
   if (v < 1e-2f) v = 1e-2f;
   const float inv = float(1.0 / double(v));
   const float local_dt = p.cfl * inv;
-  const float step = reset ? 0.0f : ss->step;
+  const float step = ss->step;
   float g = p.dt_max < local_dt ? p.dt_max : local_dt;
   if (g < p.dt_min) g = p.dt_min;
   if (step + g > p.frame_step) g = p.frame_step - step;
@@ -435,8 +436,8 @@ hipError_t launch_cfl(fsl_ctx *c) {
                                   &c->flags->cfl_bits);
   return hipGetLastError();
 }
-hipError_t launch_dt(fsl_ctx *c, const DtParams &p, int reset_frame) {
-  k_dt<<<1, 64, 0, c->stream>>>(&c->flags->cfl_bits, p, reset_frame, c->sstate);
+hipError_t launch_dt(fsl_ctx *c, const DtParams &p) {
+  k_dt<<<1, 64, 0, c->stream>>>(&c->flags->cfl_bits, p, c->sstate);
   return hipGetLastError();
 }
 // Fills n words (in place of a memset, which must not be captured in a graph).

@@ -100,6 +100,7 @@ SIGNATURES = {
     "fsl_g2p": [_P],
     "fsl_step": [_P, _F, C.POINTER(SolveParams), C.POINTER(SolveStats)],
     "fsl_advance_frame": [_P, _I, _F, C.POINTER(SolveParams), C.POINTER(FrameStats)],
+    "fsl_set_step_graph": [_P, _I],
     "fsl_compute_dt": [_P, _I, _F, _FP, _FP],
     "fsl_get_vec": [_P, _I, _P, _P],
     "fsl_set_vec": [_P, _I, _P, _P],
@@ -324,6 +325,10 @@ class Context:
         fs = FrameStats()
         self.call("fsl_advance_frame", int(cfl), float(frame_step), C.byref(params), C.byref(fs))
         return fs.as_dict()
+
+    def set_step_graph(self, enable=True):
+        """advance_frame replays each step's front half from a HIP graph."""
+        self.call("fsl_set_step_graph", int(enable))
 
     def project_forward(self, params: SolveParams) -> dict:
         st = SolveStats()

@@ -3,8 +3,8 @@
 Particles are placed with Philox4x32-10 (Salmon, Moraes, Dror, Shaw,
 "Parallel random numbers: as easy as 1, 2, 3", SC'11), keyed by
 (seed, 0) with counter (cell linear index, slot, stream, 0). The device
-seeding kernel in csrc/fsl_kernels.hip computes the same integers, so a
-scene seeded on the GPU equals the one built here bit for bit.
+seeding kernel (k_seed in csrc/particles.hip) computes the same integers, so
+a scene seeded on the GPU equals the one built here bit for bit.
 
 Jitter: u = (x >> 8) * 2^-24 in [0, 1); pos = f32(i) + f32(u) rounded to
 nearest, then clamped to the largest float below i + 1 so floor(pos) == i.

@@ -210,6 +210,12 @@ typedef struct fsl_frame_stats {
  * and read by the advection and gravity there; the elapsed time starts at 0. */
 int fsl_advance_frame(fsl_ctx *ctx, int cfl, float frame_step, const fsl_solve_params *params,
                       fsl_frame_stats *stats);
+/* With enable set, fsl_advance_frame replays the front half of each step
+ * (CFL and dt, the move and sort, clear, P2G, boundary conditions, gravity,
+ * divergence: about 25 launches and copies) from a captured HIP graph
+ * instead of launching it; the results are the same. No reference
+ * counterpart (the reference's transforms are scheduled by Canary). */
+int fsl_set_step_graph(fsl_ctx *ctx, int enable);
 /* WhileAndComputeDt's dt (flip_app_sim.cc:371-410): cfl * CFL(), min with
  * 1/frame_step, floor at frame_step/10, snap to the frame end; advances *step. */
 int fsl_compute_dt(fsl_ctx *ctx, int cfl, float frame_step, float *step, float *dt);

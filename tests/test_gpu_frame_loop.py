@@ -4,7 +4,10 @@ same loop driven from the host (fsl_compute_dt, a host read of the CFL
 maximum, then fsl_step): the same dt sequence, step counts and bit-identical
 particles and pressure. A fast scene (initial velocity 40 voxel/s, so
 v_inf > 30 and a frame takes two steps or more) exercises the dt clamps:
-cfl / v_inf, the cut at the frame's end and the snap past 90% of it.
+cfl / v_inf, the cut at the frame's end and the snap past 90% of it. With
+the step graph on, the front half of every step after the first is replayed
+from a captured HIP graph (both particle buffers, so both graphs), with the
+same results.
 """
 import numpy as np
 import pytest
@@ -29,11 +32,13 @@ def make(vel0):
     return ctx
 
 
+@pytest.mark.parametrize("graph", [False, True])
 @pytest.mark.parametrize("vel0", [(0.0, 0.0, 0.0), (40.0, -25.0, 10.0)])
-def test_device_frame_loop_matches_host_loop(vel0):
+def test_device_frame_loop_matches_host_loop(vel0, graph):
     prm = fsl.solve_params(precision=fsl.PCG_MIXED, stop_mode=fsl.STOP_REL, threshold=1e-6,
                            max_iterations=5000)
     a, b = make(vel0), make(vel0)
+    a.set_step_graph(graph)
     steps_a = steps_b = 0
     for f in range(3):
         fs = a.advance_frame(1, FRAME, prm)
