@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -30,6 +31,34 @@ int host_counts(fsl_ctx *ctx) {
 }  // namespace fsl
 
 namespace {
+
+// Staging array of fsl_get_scalar_noalloc: a file-scope device array (one
+// copy per GPU, held until the process ends), so the call allocates
+// nothing. 8 Mi floats (32 MiB): boxes of up to 8,388,608 voxels, i.e.
+// domains up to 200^3 (the box is (n + 2)^3). Calls that overlap on one GPU
+// share the array and race; the kernel writes every element a call reads,
+// so nothing needs clearing.
+constexpr int64_t kScratchFloats = int64_t(8) << 20;
+__device__ __align__(256) float scratch[kScratchFloats];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float *g_scratch[kMaxDevices] = {};
+
+// The device address of `scratch` on `dev` (the current device), looked up
+// once per device: host code uses this address, never the array's name,
+// which is a host-side stand-in.
+int scratch_address(int dev, float **out) {
+  *out = nullptr;
+  if (dev < 0 || dev >= kMaxDevices) return FSL_ERR_ARG;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return FSL_ERR_HIP;
+    g_scratch[dev] = static_cast<float *>(p);
+  }
+  *out = g_scratch[dev];
+  return FSL_OK;
+}
 
 // Every entry point except the device-resident step works with the counts
 // on the host: after a device-resident step they are read back first.
@@ -349,6 +378,7 @@ int fsl_create(fsl_ctx **out, const int dims[3], const int tiles[3], int device,
   ctx->cap = particle_capacity;
   auto fail = [&](int rc) { fsl_destroy(ctx); return rc; };
   if (hipSetDevice(device) != hipSuccess) { delete ctx; return FSL_ERR_HIP; }
+  if (scratch_address(device, &ctx->scratch) != FSL_OK) { delete ctx; return FSL_ERR_HIP; }
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
     delete ctx;
     return FSL_ERR_HIP;
@@ -993,6 +1023,28 @@ int fsl_get_scalar(fsl_ctx *ctx, int which, float *out) {
   FSL_HIP(hipStreamSynchronize(ctx->stream));
   FSL_HIP(hipMemcpy(out, dbox, size_t(nb) * sizeof(float), hipMemcpyDeviceToHost));
   (void)hipFree(dbox);
+  return FSL_OK;
+}
+
+int fsl_get_scalar_noalloc(fsl_ctx *ctx, int which, float *out) {
+  CHECK_CTX();
+  if (!out || (which != FSL_GRID_DIVERGENCE && which != FSL_GRID_PRESSURE)) return FSL_ERR_ARG;
+  const int64_t nb = box_count(ctx);
+  if (nb > kScratchFloats) {
+    ctx->err = "fsl_get_scalar_noalloc: the box has " + std::to_string(nb) +
+               " voxels, more than its staging array's " + std::to_string(kScratchFloats) +
+               " (domains up to 200^3); use fsl_get_scalar";
+    return FSL_ERR_ARG;
+  }
+  if (!ctx->scratch) {
+    ctx->err = "fsl_get_scalar_noalloc: no staging array on this device";
+    return FSL_ERR_STATE;
+  }
+  FSL_HIP(fsl::launch_box_scalar(ctx, which == FSL_GRID_DIVERGENCE ? ctx->div : ctx->pres,
+                                 ctx->scratch, true));
+  FSL_HIP(hipMemcpyAsync(out, ctx->scratch, size_t(nb) * sizeof(float), hipMemcpyDeviceToHost,
+                         ctx->stream));
+  FSL_HIP(hipStreamSynchronize(ctx->stream));
   return FSL_OK;
 }
 

@@ -228,6 +228,7 @@ struct fsl_ctx {
   bool use_graph = false;          // fsl_advance_frame replays its steps' front halves
   fsl::StepGraph graphs[2];        // by particle buffer (which one P is)
   float *pbuf0 = nullptr;          // the first particle buffer's px array
+  float *scratch = nullptr;        // device address of the staging array (api.hip)
   hipEvent_t ev_finish = nullptr;  // the solver state's copy after the solve's finish
   hipEvent_t ev_counts = nullptr;  // the counts' copy within a device-resident step
   bool bc_applied = false, grid_ready = false, outside_deleted = false;

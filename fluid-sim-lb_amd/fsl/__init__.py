@@ -105,6 +105,7 @@ SIGNATURES = {
     "fsl_get_vec": [_P, _I, _P, _P],
     "fsl_set_vec": [_P, _I, _P, _P],
     "fsl_get_scalar": [_P, _I, _P],
+    "fsl_get_scalar_noalloc": [_P, _I, _P],
     "fsl_set_scalar": [_P, _I, _P],
     "fsl_get_int": [_P, _I, _P],
     "fsl_set_marker": [_P, _P],
@@ -451,6 +452,13 @@ class Context:
     def get_scalar(self, which):
         out = np.empty(self.box, np.float32)
         self.call("fsl_get_scalar", int(which), _ptr(out))
+        return out
+
+    def get_scalar_noalloc(self, which):
+        """get_scalar staged through the library's fixed device array (no
+        allocation; boxes of at most 8,388,608 voxels)."""
+        out = np.empty(self.box, np.float32)
+        self.call("fsl_get_scalar_noalloc", int(which), _ptr(out))
         return out
 
     def set_scalar(self, which, values):

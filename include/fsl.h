@@ -224,6 +224,14 @@ int fsl_compute_dt(fsl_ctx *ctx, int cfl, float frame_step, float *step, float *
 int fsl_get_vec(fsl_ctx *ctx, int which, float *out, uint8_t *active);
 int fsl_set_vec(fsl_ctx *ctx, int which, const float *in, const uint8_t *active);
 int fsl_get_scalar(fsl_ctx *ctx, int which, float *out);
+/* fsl_get_scalar without any allocation. No counterpart in the reference:
+ * for callers that cannot allocate device memory (serving loops and
+ * harnesses that forbid hipMalloc). It stages the box through a fixed
+ * device array of 8,388,608 floats (32 MiB, one per GPU, held until the
+ * process ends) instead of a buffer allocated per call, so it takes boxes of
+ * at most that many voxels (domains up to 200^3) and returns FSL_ERR_ARG
+ * beyond. Calls that overlap on one GPU share the array and race. */
+int fsl_get_scalar_noalloc(fsl_ctx *ctx, int which, float *out);
 int fsl_set_scalar(fsl_ctx *ctx, int which, const float *in);
 int fsl_get_int(fsl_ctx *ctx, int which, int32_t *out);
 /* Sets the marker grid (labels); SOLID cells must also be solid. */
