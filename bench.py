@@ -13,7 +13,14 @@ A step is one FlipDriver::RunOneStep (advection + sort, clear, P2G, boundary
 conditions, gravity, divergence, PCG solve, pressure update, velocity update,
 extrapolation, G2P) with all data resident in HBM.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+The timed loop is FlipDriver's frame loop on the device (fsl_advance_frame:
+dt computed on the device, counts kept there, each step's front half
+replayed from a HIP graph), until K steps ran. The same steps are then run
+again from the same particles without the graph and as the host-driven
+loop (fsl_compute_dt + fsl_step), then K more with the per-kernel timing
+hook for the stage and kernel durations.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-compare]
 """
 from __future__ import annotations
 
@@ -447,19 +454,35 @@ def main():
     # device synchronisations.
     ctx.set_step_graph(True)
     frames(a.warmup)
+    snapshot = ctx.get_particles()
     (n_g, stats, iters), elapsed = timed(lambda: frames(a.steps))
     steps_per_s = n_g / elapsed
+
+    def restart():
+        """Back to the particles the timed loop started from: each loop
+        below runs the same n_g steps (the same dt, hence the same PCG
+        systems) as the headline loop."""
+        nonlocal step_t
+        ctx.set_particles(*snapshot)
+        ctx.clear()
+        step_t = 0.0
+
     # The same loop without the graph, and the host-driven loop.
     ctx.set_step_graph(False)
-    (n_ng, _, iters_ng), el_ng = timed(lambda: frames(a.steps))
-    host_stats, el_host = timed(lambda: [one_step() for _ in range(a.steps)])
+    restart()
+    (n_ng, _, iters_ng), el_ng = timed(lambda: frames(n_g))
+    restart()
+    host_stats, el_host = timed(lambda: [one_step() for _ in range(n_g)])
+    del snapshot
     variants = {
-        "device_loop_graph": {"steps_per_s": steps_per_s, "ms_per_step": 1e3 * elapsed / n_g,
-                              "pcg_iterations_mean": float(np.mean(iters)) * len(iters) / n_g},
-        "device_loop_no_graph": {"steps_per_s": n_ng / el_ng, "ms_per_step": 1e3 * el_ng / n_ng,
-                                 "pcg_iterations_mean": float(np.sum(iters_ng)) / n_ng},
-        "host_loop": {"steps_per_s": a.steps / el_host, "ms_per_step": 1e3 * el_host / a.steps,
-                      "pcg_iterations_mean": float(np.mean([s["iterations"] for s in host_stats]))},
+        "device_loop_graph": {"steps": n_g, "steps_per_s": steps_per_s,
+                              "ms_per_step": 1e3 * elapsed / n_g,
+                              "pcg_iterations_total": int(np.sum(iters))},
+        "device_loop_no_graph": {"steps": n_ng, "steps_per_s": n_ng / el_ng,
+                                 "ms_per_step": 1e3 * el_ng / n_ng,
+                                 "pcg_iterations_total": int(np.sum(iters_ng))},
+        "host_loop": {"steps": n_g, "steps_per_s": n_g / el_host, "ms_per_step": 1e3 * el_host / n_g,
+                      "pcg_iterations_total": int(sum(s["iterations"] for s in host_stats))},
     }
     # Kernel durations: the next K steps with the hook on, which records each
     # PCG pass's own start and end on the library's stream.
@@ -471,8 +494,7 @@ def main():
     hooked_ms = 1e3 * (time.perf_counter() - t1) / a.steps
     tm = ctx.timing()
     ctx.set_timing(False)
-    iters = [int(i) for i in iters]
-    iters = [s["iterations"] for s in stats]
+    iters_per_step = float(np.sum(iters)) / n_g
     iters_t = [s["iterations"] for s in stats_t]
     rows_t = [s["n_rows"] for s in stats_t]
     rows_w = float(np.average(rows_t, weights=np.maximum(iters_t, 1)))
@@ -498,7 +520,7 @@ def main():
 
     cpu = None
     if not a.no_cpu:
-        cpu = cpu_baseline(int(round(float(np.mean(iters)))))
+        cpu = cpu_baseline(int(round(iters_per_step)))
 
     line = {
         "metric": ("FLIP steps/s at 256^3 (mixed-precision PCG, rel. residual 1e-6)"
@@ -524,7 +546,8 @@ def main():
             "particles": n_particles,
             "fluid_rows_mean": rows_w,
             "fluid_leaf_occupancy": occupancy,
-            "pcg_iterations_per_step": iters,
+            "pcg_iterations_per_frame": [int(i) for i in iters],
+            "pcg_iterations_per_step_mean": iters_per_step,
             "parallelism": "1 GPU, tiles as work units",
         },
         "pcg_GBps": pcg_gbps,
