@@ -103,6 +103,9 @@ struct PcgState {
   // start after the stopping decision was stored.
   int stop_k;
   unsigned cnt_a, cnt_b, cnt_r, cnt_x;
+  // The iteration a convergence check enqueued without a host read
+  // refreshed (the stopping iteration, or the next one when still running).
+  int refresh_kn, pad;
 };
 
 // Device-side error flags of the grid stages.
@@ -229,6 +232,9 @@ struct fsl_ctx {
   fsl::StepGraph graphs[2];        // by particle buffer (which one P is)
   float *pbuf0 = nullptr;          // the first particle buffer's px array
   float *scratch = nullptr;        // device address of the staging array (api.hip)
+  // The previous single-reduction solve's iteration count: with use_graph,
+  // the next solve enqueues that many launches at once (pcg.hip).
+  int pcg_last_iters = 0;
   hipEvent_t ev_finish = nullptr;  // the solver state's copy after the solve's finish
   hipEvent_t ev_counts = nullptr;  // the counts' copy within a device-resident step
   bool bc_applied = false, grid_ready = false, outside_deleted = false;

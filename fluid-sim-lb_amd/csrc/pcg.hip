@@ -38,6 +38,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <type_traits>
 
 #include "device_util.h"
@@ -250,6 +251,13 @@ __device__ inline void total_all(const A *partials, int n, double out[K]) {
     for (int w = 0; w < nw; ++w) t += s_t[j][w];
     out[j] = double(t);
   }
+}
+
+// Mode bits of single-reduction iteration k: bit 0, x is updated (odd k);
+// bit 1, x lacks alpha_{k-1} p_{k-1} (odd k, except the iteration a solve
+// resumed at); bit 2, x is folded into x64 before a scheduled refresh.
+inline int sr_mode_of(int k, int resume_k, int refresh) {
+  return ((k & 1) ? (k == resume_k ? 1 : 3) : 0) | (refresh > 0 && (k + 1) % refresh == 0 ? 4 : 0);
 }
 
 __device__ inline bool converged(const PcgState *st, double rr) {
@@ -613,10 +621,12 @@ __global__ __launch_bounds__(kT) void k_ic_dots(const int32_t *flist, int64_t nf
 }
 
 // MIXED refresh, part 1: fold the float iterate into the double solution.
+// force = 2: the fold of a convergence check enqueued without a host read,
+// done only when the solve has stopped.
 __global__ __launch_bounds__(kT) void k_refresh_x(const int32_t *flist, int64_t nf,
                                                   const uint8_t *lab, float *x, double *x64,
                                                   const PcgState *st, int force) {
-  if ((!force && st->done) || st->status) return;
+  if ((!force && st->done) || st->status || (force == 2 && !st->done)) return;
   for (Walk q = walk(nf); q.more(); q.next()) {
     if (!q.valid()) continue;
     const int64_t base = int64_t(flist[q.mine()]) << 9;
@@ -1192,11 +1202,16 @@ __global__ __launch_bounds__(kG * GR) void k_cg(Dom d, const int32_t *flist, int
 // folded into x64 before (by the k_cg launch before the refresh, or by
 // k_refresh_x).
 template <bool OFF32>
+// force = 2: a convergence check enqueued without knowing whether or where
+// the solve stopped. It records kn = it (stopped) or it + 1 (still running)
+// as refresh_kn and, when stopped, refreshes iteration kn, whose residual
+// buffer and partials are rout/part_out for even kn and rout_odd/part_odd
+// for odd kn; a running solve is left as it is.
 __global__ __launch_bounds__(kG) void k_refresh_cg(Dom d, const int32_t *flist, int64_t nf,
                                                    const uint8_t *lab, const float *div,
                                                    const double *x64, float *rout,
-                                                   double *part_out, const PcgState *st,
-                                                   int force) {
+                                                   double *part_out, float *rout_odd,
+                                                   double *part_odd, PcgState *st, int force) {
 #pragma clang fp contract(off)
   using Off = typename std::conditional<OFF32, uint32_t, int64_t>::type;
   __shared__ double s_p[kPSize];
@@ -1205,6 +1220,18 @@ __global__ __launch_bounds__(kG) void k_refresh_cg(Dom d, const int32_t *flist, 
   __shared__ uint8_t s_l1[192];
   __shared__ float s_dinv[32];
   if ((!force && st->done) || st->status) return;
+  if (force == 2) {
+    const bool stopped = st->done != 0;
+    const int kn = stopped ? st->it : st->it + 1;
+    // Every workgroup reads done and it before workgroup 0 stores
+    // refresh_kn; nothing in this launch writes them.
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->refresh_kn = kn;
+    if (!stopped) return;
+    if (kn & 1) {
+      rout = rout_odd;
+      part_out = part_odd;
+    }
+  }
   const int t = threadIdx.x;
   const int w = __builtin_amdgcn_readfirstlane(t >> 6), l = t & 63;
   if (t < 32) s_dinv[t] = is_fluid(uint8_t(t)) ? dinv<float>(t >> 2) : 0.0f;
@@ -1373,11 +1400,20 @@ __global__ __launch_bounds__(1024) void k_prepare_scalars_sr(const A *part_f, in
 }
 
 // After a forced refresh (single-reduction path): the true r.r decides.
+// part_odd set: the check of k_refresh_cg's force 2 (iteration refresh_kn,
+// stored there; its partials are part_odd when odd), which also becomes the
+// solve's iteration count.
 __global__ __launch_bounds__(1024) void k_refresh_scalars_sr(const double *part_f, int g,
-                                                             PcgState *st) {
+                                                             PcgState *st,
+                                                             const double *part_odd) {
+  // A check of a running solve refreshed nothing.
+  if (part_odd && !st->done) return;
+  const int kn = st->refresh_kn;
+  if (part_odd && (kn & 1)) part_f = part_odd;
   double tot[3];
   total_all<double, 3>(part_f, g, tot);
   if (threadIdx.x != 0) return;
+  if (part_odd) st->it = kn;
   st->rr_true = tot[2];
   st->rr = tot[2];
   if (isnan(tot[2])) st->status = FSL_ERR_NAN;
@@ -1763,20 +1799,36 @@ void launch_refresh(fsl_ctx *c, int k, int force) {
 
 // Single-reduction path, MIXED: replace r_kn (input of iteration kn) by the
 // FP64 true residual, and its totals by those of the true residual.
+// kn < 0: a convergence check whose iteration is taken from the state on
+// the device (k_refresh_cg, force 2).
 void launch_refresh_sr(fsl_ctx *c, int kn, int force) {
   const int g = Kernels<float, double, false, true>::sr_grid(c);
   Parts P = parts(c);
+  const bool from_state = kn < 0;
   float *rbuf = (float *)(kn & 1 ? c->z : c->r);
+  double *part = (double *)P.f[kn & 1];
   // Before a scheduled refresh the k_cg launch has folded x into x64; a
   // forced refresh (convergence check) folds here.
   if (force)
     k_refresh_x<<<grid_for(c->n_fluid), kT, 0, c->stream>>>(c->fluid_list, c->n_fluid, c->lab,
-                                                          (float *)c->x, c->x64, c->pcg, force);
+                                                          (float *)c->x, c->x64, c->pcg,
+                                                          from_state ? 2 : force);
   auto kern = uint64_t(c->nvox) * 8 < (uint64_t(1) << 32) ? k_refresh_cg<true> : k_refresh_cg<false>;
+  if (from_state) {
+    kern<<<g, kG, 0, c->stream>>>(c->dom, c->fluid_list, c->n_fluid, c->lab, c->div, c->x64,
+                                  (float *)c->r, (double *)P.f[0], (float *)c->z,
+                                  (double *)P.f[1], c->pcg, 2);
+    k_refresh_scalars_sr<<<1, 1024, 0, c->stream>>>((const double *)P.f[0], g, c->pcg,
+                                                    (const double *)P.f[1]);
+    return;
+  }
   kern<<<g, kG, 0, c->stream>>>(c->dom, c->fluid_list, c->n_fluid, c->lab, c->div, c->x64, rbuf,
-                                (double *)P.f[kn & 1], c->pcg, force);
-  if (force) k_refresh_scalars_sr<<<1, 1024, 0, c->stream>>>((const double *)P.f[kn & 1], g, c->pcg);
+                                part, rbuf, part, c->pcg, force);
+  if (force)
+    k_refresh_scalars_sr<<<1, 1024, 0, c->stream>>>(part, g, c->pcg, nullptr);
 }
+
+
 
 template <typename F>
 void dispatch(int precision, F &&fn) {
@@ -1784,6 +1836,7 @@ void dispatch(int precision, F &&fn) {
   else if (precision == FSL_PCG_MIXED) fn(Kernels<float, double, false, true>());
   else fn(Kernels<float, float, true, false>());
 }
+
 
 int sync_state(fsl_ctx *ctx) {
   FSL_HIP(hipMemcpyAsync(ctx->pcg_host, ctx->pcg, sizeof(PcgState), hipMemcpyDeviceToHost,
@@ -2053,17 +2106,41 @@ static int pcg_iterations_sr(fsl_ctx *ctx, const fsl_solve_params *prm, fsl_solv
   // alpha_{k-1} p_{k-1} + alpha_k p_k; a kernel that stops at an odd k applies
   // the pending term. After a forced refresh x is complete (resume_k).
   int resume_k = -1;
-  auto mode_of = [&](int kk) {
-    return ((kk & 1) ? (kk == resume_k ? 1 : 3) : 0) |
-           (mixed && (kk + 1) % ctx->pcg_refresh == 0 ? 4 : 0);
+  auto mode_of = [&](int kk) { return sr_mode_of(kk, resume_k, mixed ? ctx->pcg_refresh : 0); };
+  // Predicted solves (with the step graph on): the previous solve's
+  // iteration count predicts this one's. That many launches (rounded up to
+  // an even count) are enqueued at once, then the convergence check (mixed;
+  // it acts only if the solve stopped), and the state is read once. The
+  // host enqueues far faster than the launches run, so the launches past
+  // convergence, which exit on the stop, run back to back. A solve that has
+  // not stopped by then continues 32 launches (and a check) at a time, with
+  // the same iterates as without the prediction; a stop the mixed check
+  // rejects (the true residual does not confirm it) resumes with the
+  // host-checked batches below.
+  auto issue_sr = [&](int kk) {
+    dispatch(ctx->pcg_precision,
+             [&](auto K) { decltype(K)::sr_step(ctx, kk, mode_of(kk), nullptr, nullptr); });
+    if (mixed && (kk + 1) % ctx->pcg_refresh == 0) launch_refresh_sr(ctx, kk + 1, 0);
   };
+  if (!ctx->timing && adaptive && ctx->use_graph && ctx->pcg_last_iters > 0 && ctx->n_fluid > 0 &&
+      !ctx->pcg_host->done) {
+    int kend = std::max(2, ctx->pcg_last_iters + (ctx->pcg_last_iters & 1));
+    for (;;) {
+      for (; k < kend; ++k) issue_sr(k);
+      if (mixed) launch_refresh_sr(ctx, -1, 1);
+      FSL_HIP(hipGetLastError());
+      rc = sync_state(ctx);
+      if (rc) return rc;
+      const PcgState *h = ctx->pcg_host;
+      // The iteration a continuation starts from.
+      k = mixed ? h->refresh_kn : (h->done ? h->it : h->it + 1);
+      resume_k = k;
+      if (h->done || h->status || k != kend || k >= h->max_iter) break;
+      kend += 32;
+    }
+  }
   while (!ctx->timing && !ctx->pcg_host->done) {
-    rc = run_pipelined(ctx, k, nb, adaptive, [&](int kk) {
-      dispatch(ctx->pcg_precision, [&](auto K) {
-        decltype(K)::sr_step(ctx, kk, mode_of(kk), nullptr, nullptr);
-      });
-      if (mixed && (kk + 1) % ctx->pcg_refresh == 0) launch_refresh_sr(ctx, kk + 1, 0);
-    });
+    rc = run_pipelined(ctx, k, nb, adaptive, issue_sr);
     if (rc) return rc;
     const PcgState *h = ctx->pcg_host;
     if (h->status || !(mixed && h->it > 0 && h->it < h->max_iter)) break;
@@ -2140,6 +2217,7 @@ static int pcg_iterations_sr(fsl_ctx *ctx, const fsl_solve_params *prm, fsl_solv
   FSL_HIP(hipEventDestroy(e0));
   FSL_HIP(hipEventDestroy(e1));
   if (st) st->seconds = double(ms) * 1e-3;
+  if (!ctx->pcg_host->status) ctx->pcg_last_iters = ctx->pcg_host->it;
   return ctx->pcg_host->status;
 }
 

@@ -213,8 +213,12 @@ int fsl_advance_frame(fsl_ctx *ctx, int cfl, float frame_step, const fsl_solve_p
 /* With enable set, fsl_advance_frame replays the front half of each step
  * (CFL and dt, the move and sort, clear, P2G, boundary conditions, gravity,
  * divergence: about 25 launches and copies) from a captured HIP graph
- * instead of launching it; the results are the same. No reference
- * counterpart (the reference's transforms are scheduled by Canary). */
+ * instead of launching it, and single-reduction solves after the first one
+ * enqueue as many iterations as the previous solve took, then the
+ * convergence check, and read the solver state once (continuing 32
+ * iterations at a time when that was too few), instead of reading it after
+ * every batch; the results are the same. No reference counterpart (the
+ * reference's transforms are scheduled by Canary). */
 int fsl_set_step_graph(fsl_ctx *ctx, int enable);
 /* WhileAndComputeDt's dt (flip_app_sim.cc:371-410): cfl * CFL(), min with
  * 1/frame_step, floor at frame_step/10, snap to the frame end; advances *step. */
