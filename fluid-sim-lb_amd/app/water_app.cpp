@@ -8,6 +8,8 @@
 //   gravity=9.8 flip=0.95 frame_rate=30 cfl=1
 //   solver_max_iterations=1500 solver_threshold=5e-9 (times n^3)
 //   preconditioner=jacobi|ic   (ic: block IC(0) per partition)
+//   device_loop=1     each frame on the device (FlipDriver::NextFrameOnDevice:
+//                     dt there, step graph on; scenes without sources)
 //   output=<prefix>   writes <prefix>NNNNNNNN binary particle frames
 //   output_dir=<dir>  SaveDataToFile as the reference: <dir>/particles_NNNNNNNN,
 //                     the initial state as frame 0, then every frame
@@ -264,6 +266,8 @@ int main(int argc, char **argv) {
     prm.frame_rate = std::atoi(get(kv, "frame_rate", "30").c_str());
     prm.cfl = std::atoi(get(kv, "cfl", "1").c_str());
     fsl::FlipDriver driver(&sim, &solver, prm);
+    // device_loop=1: each frame on the device (dt there, step graph on).
+    const bool device_loop = get(kv, "device_loop", "0") == "1";
     const std::string out = get(kv, "output", "");
     // FlipDriver::SaveData (flip_app_sim.cc:204, 259-274): the state after
     // initialisation is frame 0, then one frame after every frame.
@@ -278,7 +282,8 @@ int main(int argc, char **argv) {
     for (int f = 0; f < prm.frames; ++f) {
       const int s0 = driver.steps();
       const auto t0 = std::chrono::steady_clock::now();
-      driver.NextFrame();
+      if (device_loop) driver.NextFrameOnDevice(true);
+      else driver.NextFrame();
       const double sec =
           std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
       if (!out.empty()) {

@@ -546,10 +546,19 @@ class PCGDriver {
    * r.r < threshold (FP32); fixed_iterations runs exactly max_iterations. */
   void Solve() {
     if (!sim_) throw Error(FSL_ERR_STATE, name_ + ": SetVariables first");
-    params_.threshold = fixed_iterations_ ? 0.0 : double(threshold_);
-    params_.max_iterations = max_iterations_;
-    sim_->check(fsl_pcg_solve(sim_->context(), &params_, &stats_));
+    const fsl_solve_params p = step_params();
+    sim_->check(fsl_pcg_solve(sim_->context(), &p, &stats_));
   }
+  /* The parameters Solve runs with (the stopping rule and iteration cap of
+   * the constructor, or of set_solve_params). */
+  fsl_solve_params step_params() const {
+    fsl_solve_params p = params_;
+    p.threshold = fixed_iterations_ ? 0.0 : double(threshold_);
+    p.max_iterations = max_iterations_;
+    return p;
+  }
+  /* The statistics of the last solve a device-resident frame ran. */
+  void set_stats(const fsl_solve_stats &s) { stats_ = s; }
 
   /* The preconditioner the solve builds: IC gives block IC(0), every other
    * type Jacobi (BuildPreconditioner's branches, solver.cc:103-111). */
@@ -675,6 +684,29 @@ class FlipDriver {
    * flip_app_sim.cc:357-369), then ResetDt and AdvanceTime. */
   void NextFrame() {
     while (double(step_) + 1e-5 < double(frame_step_)) RunOneStep(ComputeDt());
+    step_ = 0;
+    time_ += frame_step_;
+    ++frame_;
+  }
+  /* NextFrame with the frame on the device (fsl_advance_frame): dt from the
+   * CFL maximum there, the counts kept there, and with graph set the front
+   * half of each step replayed from a HIP graph and each solve enqueued at
+   * the previous solve's iteration count. Same results as NextFrame. Water
+   * sources are evaluated on the host at each step's start, so a scene with
+   * sources runs NextFrame. */
+  void NextFrameOnDevice(bool graph = true) {
+    if (sim_->HasSources()) {
+      NextFrame();
+      return;
+    }
+    sim_->check(fsl_set_step_graph(sim_->context(), graph ? 1 : 0));
+    const fsl_solve_params p = solver_->step_params();
+    fsl_frame_stats fs{};
+    sim_->check(fsl_advance_frame(sim_->context(), params_.cfl, frame_step_, &p, &fs));
+    solver_->set_stats(fs.last_solve);
+    steps_ += fs.steps;
+    // The frame's steps end exactly at its end (the last dt is cut there).
+    advanced_time_ += frame_step_;
     step_ = 0;
     time_ += frame_step_;
     ++frame_;
