@@ -330,7 +330,7 @@ int ensure_state(fsl_ctx *ctx) {
     if ((rc = alloc(ctx, &A.s[a], ctx->nvox))) return rc;
     if ((rc = alloc(ctx, &A.wsum[a], ctx->nvox))) return rc;
     for (int b = 0; b < 2; ++b)
-      if ((rc = alloc(ctx, &A.pa[b][a], ctx->cap))) return rc;
+      if ((rc = alloc(ctx, &A.pa[b][a], ctx->cap + fsl::kParticlePad))) return rc;
   }
   if ((rc = alloc(ctx, &A.rhs, ctx->nvox))) return rc;
   if ((rc = alloc(ctx, &A.y, ctx->nvox))) return rc;

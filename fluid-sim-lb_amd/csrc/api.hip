@@ -450,8 +450,10 @@ int fsl_create(fsl_ctx **out, const int dims[3], const int tiles[3], int device,
       rc = FSL_ERR_HIP;
   }
   if (!rc) rc = dev_alloc(ctx, &ctx->fluid_list, nl);
-  for (int a = 0; a < 9 && !rc; ++a) rc = dev_alloc(ctx, &ctx->P.a[a], size_t(ctx->cap));
-  for (int a = 0; a < 9 && !rc; ++a) rc = dev_alloc(ctx, &ctx->Q.a[a], size_t(ctx->cap));
+  for (int a = 0; a < 9 && !rc; ++a)
+    rc = dev_alloc(ctx, &ctx->P.a[a], size_t(ctx->cap + fsl::kParticlePad));
+  for (int a = 0; a < 9 && !rc; ++a)
+    rc = dev_alloc(ctx, &ctx->Q.a[a], size_t(ctx->cap + fsl::kParticlePad));
   ctx->pbuf0 = ctx->P.a[0];
   if (!rc) rc = dev_alloc(ctx, &ctx->key, size_t(ctx->cap));
   if (!rc) rc = dev_alloc(ctx, &ctx->perm, size_t(ctx->cap));

@@ -164,6 +164,10 @@ struct DevFlags {
 struct Particles {
   float *a[9];  // px py pz vx vy vz gx gy gz
 };
+// Particle-sized arrays that P2G reads (the particle arrays and the adjoint's
+// particle values) hold this many elements beyond the capacity: P2G loads
+// four consecutive particles at a time, up to three past a leaf's last one.
+constexpr int64_t kParticlePad = 4;
 
 // A leaf list on the device, list[0, n). When `count` is set the length is
 // the value there (written on the device by the compaction that built the

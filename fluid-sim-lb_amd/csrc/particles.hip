@@ -351,7 +351,9 @@ template <> struct Axes<2> { static constexpr int a = 0, b = 1; };
 // particle lies below the cell's centre line (the lower sample is in cell
 // c - 1) or on the slots (1, 2) otherwise. The reference's factored form
 // (flip_simulation.cc:977-990) is the same polynomial; the two roundings
-// differ in the last bits of a weight.
+// differ in the last bits of a weight. Each slot takes two fused
+// multiply-adds, w * vel as (fd * vel) * (fa * fb) and w as fd * (fa * fb),
+// so the weight itself is never rounded on its own.
 template <int D>
 __device__ inline void p2g_particle(const float pos[3], const int c[3], float vel, float avw[18],
                                     float aw[18], uint32_t &cls) {
@@ -383,16 +385,16 @@ __device__ inline void p2g_particle(const float pos[3], const int c[3], float ve
   for (int sa = 0; sa < 3; ++sa)
 #pragma unroll
     for (int sb = 0; sb < 3; ++sb) fab[sa][sb] = fa[sa] * fb[sb];
+  const float vd[2] = {fd[0] * vel, fd[1] * vel};
 #pragma unroll
   for (int sd = 0; sd < 2; ++sd)
 #pragma unroll
     for (int sa = 0; sa < 3; ++sa)
 #pragma unroll
       for (int sb = 0; sb < 3; ++sb) {
-        const float w = fd[sd] * fab[sa][sb];
         const int s = sd * 9 + sa * 3 + sb;
-        avw[s] = fmaf(w, vel, avw[s]);
-        aw[s] = aw[s] + w;
+        avw[s] = fmaf(vd[sd], fab[sa][sb], avw[s]);
+        aw[s] = fmaf(fd[sd], fab[sa][sb], aw[s]);
       }
 }
 
@@ -432,15 +434,36 @@ __device__ inline int ring_index(int x, int y, int z) {
   return 360 + (z ? 64 : 0) + (x - 1) * 8 + (y - 1);
 }
 
-// A leaf's own cells, all three components in one pass over the particles
-// (each particle's position and velocity are loaded once): thread t sums
-// its cell's particles into the 18 slots of each component in registers,
-// then the slots are committed to the region's LDS samples one phase at a
-// time, the three components in the same phase (within a phase every cell
-// targets a different sample of each component, so the order is fixed and
-// no atomics are needed; the touch flags are only ever set to 1). r is the
-// cell's region index (x * 100 + y * 10 + z).
-__device__ inline void p2g_own(const Particles &P, const int c[3], int r, uint32_t beg,
+// P2G reads the particle arrays through buffer descriptors based at a leaf's
+// first particle, so offsets stay 32-bit at any particle count; the record
+// count covers the leaf's run plus the three particles a four-wide load can
+// reach past it (the arrays carry kParticlePad spare elements).
+struct LeafRuns {
+  __amdgpu_buffer_rsrc_t a[6];  // px py pz and the three particle values
+  uint32_t first;               // the leaf's first particle
+};
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+// Descriptor word 3 for raw (untyped, stride 0) buffer loads on gfx9-family
+// chips: data format 32 bits, no swizzle.
+constexpr int kRawBufferWord3 = 0x00020000;
+
+// One cell's particles, all three components in one pass (each particle's
+// position and velocity are loaded once), summed in particle order into the
+// 18 slots of each component in registers; then committed to the region's
+// LDS samples. Loads take four consecutive particles per array (one 16-byte
+// load per array and trip: the load count, not the bytes, limits this loop);
+// particles past the cell's end are not accumulated.
+//
+// Commit: the x offset of slot (sd, sa, sb) is sd for component x and sa - 1
+// for y and z. Grouped by that offset (-1, 0, +1), the cells of one x plane
+// (one wavefront) write only to one plane of samples, a different plane for
+// each wavefront, so a group needs no workgroup barrier; inside a
+// wavefront, LDS operations are performed in issue order and within one
+// slot every lane targets a different sample, so the slots of a group are
+// applied one after another in a fixed order (the empty asm keeps the
+// compiler from reordering LDS accesses across slots). The touch flags are
+// only ever set to 1. r is the cell's region index (x * 100 + y * 10 + z).
+__device__ inline void p2g_own(const LeafRuns &R, const int c[3], int r, uint32_t beg,
                                uint32_t end, float (*s_vw)[1000], float (*s_w)[1000],
                                uint8_t *s_touch) {
   float avw[3][18], aw[3][18];
@@ -449,22 +472,20 @@ __device__ inline void p2g_own(const Particles &P, const int c[3], int r, uint32
   for (int a = 0; a < 3; ++a)
 #pragma unroll
     for (int q = 0; q < 18; ++q) { avw[a][q] = 0.0f; aw[a][q] = 0.0f; }
-  // Two particles per trip, their loads issued together (indices clamped to
-  // the cell's last particle; the clamped copy is not accumulated); the sums
-  // run in particle order.
-  for (uint32_t q = beg; q < end; q += 2) {
-    float pp[2][3], pv[2][3];
+  for (uint32_t q = beg; q < end; q += 4) {
+    float pp[4][3], pv[4][3];
+    const int off = int(q - R.first) * 4;
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      const uint32_t qq = q + u < end ? q + u : end - 1;
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        pp[u][a] = P.a[a][qq];
-        pv[u][a] = P.a[3 + a][qq];
-      }
+    for (int a = 0; a < 3; ++a) {
+      const u32x4 x = __builtin_amdgcn_raw_buffer_load_b128(R.a[a], off, 0, 0);
+      const u32x4 y = __builtin_amdgcn_raw_buffer_load_b128(R.a[3 + a], off, 0, 0);
+      pp[0][a] = __uint_as_float(x.x); pp[1][a] = __uint_as_float(x.y);
+      pp[2][a] = __uint_as_float(x.z); pp[3][a] = __uint_as_float(x.w);
+      pv[0][a] = __uint_as_float(y.x); pv[1][a] = __uint_as_float(y.y);
+      pv[2][a] = __uint_as_float(y.z); pv[3][a] = __uint_as_float(y.w);
     }
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
+    for (int u = 0; u < 4; ++u) {
       if (q + u >= end) break;
       p2g_particle<0>(pp[u], c, pv[u][0], avw[0], aw[0], cls[0]);
       p2g_particle<1>(pp[u], c, pv[u][1], avw[1], aw[1], cls[1]);
@@ -474,43 +495,54 @@ __device__ inline void p2g_own(const Particles &P, const int c[3], int r, uint32
   constexpr int st[3] = {100, 10, 1};
   const bool any = end > beg;
 #pragma unroll
-  for (int sd = 0; sd < 2; ++sd)
+  for (int dx = -1; dx <= 1; ++dx) {
 #pragma unroll
-    for (int sa = 0; sa < 3; ++sa)
+    for (int k = 0; k < 9; ++k) {
+      if (dx < 0 && k >= 6) continue;  // x has no slot at offset -1: six rounds
+      if (any) {
 #pragma unroll
-      for (int sb = 0; sb < 3; ++sb) {
-        const int slot = sd * 9 + sa * 3 + sb;
-        if (any) {
-#pragma unroll
-          for (int D = 0; D < 3; ++D) {
-            const int A = D == 0 ? 1 : 0, B = D == 2 ? 1 : 2;
-            const int g = r + sd * st[D] + (sa - 1) * st[A] + (sb - 1) * st[B];
-            s_vw[D][g] += avw[D][slot];
-            s_w[D][g] += aw[D][slot];
-            if (cls[D] & slot_classes(sa, sb)) s_touch[g] = 1;
+        for (int D = 0; D < 3; ++D) {
+          int sd, sa, sb;
+          if (D == 0) {
+            if (dx < 0) continue;
+            sd = dx; sa = k / 3; sb = k % 3;
+          } else {
+            if (k >= 6) continue;
+            sa = dx + 1; sd = k / 3; sb = k % 3;
           }
+          const int slot = sd * 9 + sa * 3 + sb;
+          const int A = D == 0 ? 1 : 0, B = D == 2 ? 1 : 2;
+          const int g = r + sd * st[D] + (sa - 1) * st[A] + (sb - 1) * st[B];
+          s_vw[D][g] += avw[D][slot];
+          s_w[D][g] += aw[D][slot];
+          if (cls[D] & slot_classes(sa, sb)) s_touch[g] = 1;
         }
-        __syncthreads();
       }
+      asm volatile("" ::: "memory");
+    }
+    __syncthreads();
+  }
 }
 
 // TransferParticlesToGrid (flip_simulation.cc:808-923) restated as a gather
-// in two kernels. k_p2g_own: one workgroup per leaf with particles sums its
-// own 512 cells' particles into the samples of its 10^3 region (each
-// particle is visited once); the own samples' partial sums go
-// to vx/vy/vz, ws* and vflag, the ring samples (those of neighbouring
-// leaves) to the leaf's ring block. k_p2g_combine then completes every
-// active leaf's samples from the ring blocks of its neighbours, in a fixed
-// order, and normalises.
+// in two kernels. k_p2g_own: one workgroup of 256 threads per leaf with
+// particles; thread t sums cell t, then cell t + 256 (x in 0..3, then 4..7),
+// into the samples of the leaf's 10^3 region (each particle is visited
+// once); the own samples' partial sums go to vx/vy/vz, ws* and vflag, the
+// ring samples (those of neighbouring leaves) to the leaf's ring block.
+// 256 threads let two leaves share a CU, so one leaf's commit overlaps the
+// other's sums. k_p2g_combine then completes every active leaf's samples
+// from the ring blocks of its neighbours, in a fixed order, and normalises.
 // ADJ = true: the transpose of G2P's interpolation, i.e. the un-normalised
 // sums sum_p w * value_p with the particle values taken from P.a[3..5]; no
 // weights, labels or activity are written.
+constexpr int kP2gThreads = 256;
 template <bool ADJ>
-__global__ __launch_bounds__(512) void k_p2g_own(Dom d, LeafList L,
-                                                 const uint32_t *lhas, const uint32_t *vstart,
-                                                 Particles P, float *vx, float *vy, float *vz,
-                                                 float *wx, float *wy, float *wz, uint8_t *vflag,
-                                                 float *ring, uint8_t *ring_t, int64_t rstride) {
+__global__ __launch_bounds__(kP2gThreads) void k_p2g_own(
+    Dom d, LeafList L, const uint32_t *lhas, const uint32_t *vstart, Particles P, float *vx,
+    float *vy, float *vz, float *wx, float *wy, float *wz, uint8_t *vflag, float *ring,
+    uint8_t *ring_t, int64_t rstride) {
+  constexpr int T = kP2gThreads, NC = 512 / T;
   __shared__ float s_vw[3][1000];
   __shared__ float s_w[3][1000];
   __shared__ uint8_t s_touch[1000];
@@ -521,25 +553,46 @@ __global__ __launch_bounds__(512) void k_p2g_own(Dom d, LeafList L,
     int o[3];
     leaf_origin(d, leaf, o);
     const int t = threadIdx.x;
-    for (int i = t; i < 1000; i += 512) {
+    for (int i = t; i < 1000; i += T) {
 #pragma unroll
       for (int a = 0; a < 3; ++a) { s_vw[a][i] = 0.0f; s_w[a][i] = 0.0f; }
       s_touch[i] = 0;
     }
-    const int lx = t >> 6, ly = (t >> 3) & 7, lz = t & 7;
-    const int c[3] = {o[0] + lx, o[1] + ly, o[2] + lz};
-    const int r = (lx + 1) * 100 + (ly + 1) * 10 + (lz + 1);
-    const int64_t v = (leaf << 9) | t;
-    const int64_t sv = sort_begin(d, leaf) | t;
-    const uint32_t beg = vstart[sv], end = vstart[sv + 1];
-    __syncthreads();
-    p2g_own(P, c, r, beg, end, s_vw, s_w, s_touch);
-    vx[v] = s_vw[0][r]; vy[v] = s_vw[1][r]; vz[v] = s_vw[2][r];
-    if (!ADJ) {
-      wx[v] = s_w[0][r]; wy[v] = s_w[1][r]; wz[v] = s_w[2][r];
-      vflag[v] = s_touch[r];
+    const int64_t sb = sort_begin(d, leaf);
+    // The leaf's particle run, the same for every thread.
+    LeafRuns R;
+    R.first = __builtin_amdgcn_readfirstlane(vstart[sb]);
+    const uint32_t last = __builtin_amdgcn_readfirstlane(vstart[sb + 512]);
+    const int bytes = int((last - R.first + 3) * 4);
+#pragma unroll
+    for (int a = 0; a < 6; ++a)
+      R.a[a] = __builtin_amdgcn_make_buffer_rsrc(P.a[a] + R.first, 0, bytes, kRawBufferWord3);
+    uint32_t beg[NC], end[NC];
+#pragma unroll
+    for (int h = 0; h < NC; ++h) {
+      beg[h] = vstart[sb | (t + h * T)];
+      end[h] = vstart[(sb | (t + h * T)) + 1];
     }
-    for (int i = t; i < kRing; i += 512) {
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < NC; ++h) {
+      const int cell = t + h * T, lx = cell >> 6, ly = (cell >> 3) & 7, lz = cell & 7;
+      const int c[3] = {o[0] + lx, o[1] + ly, o[2] + lz};
+      p2g_own(R, c, (lx + 1) * 100 + (ly + 1) * 10 + (lz + 1), beg[h], end[h], s_vw, s_w,
+              s_touch);
+    }
+#pragma unroll
+    for (int h = 0; h < NC; ++h) {
+      const int cell = t + h * T, lx = cell >> 6, ly = (cell >> 3) & 7, lz = cell & 7;
+      const int r = (lx + 1) * 100 + (ly + 1) * 10 + (lz + 1);
+      const int64_t v = (leaf << 9) | cell;
+      vx[v] = s_vw[0][r]; vy[v] = s_vw[1][r]; vz[v] = s_vw[2][r];
+      if (!ADJ) {
+        wx[v] = s_w[0][r]; wy[v] = s_w[1][r]; wz[v] = s_w[2][r];
+        vflag[v] = s_touch[r];
+      }
+    }
+    for (int i = t; i < kRing; i += T) {
       int x, y, z;
       ring_cell(i, x, y, z);
       const int g = x * 100 + y * 10 + z;
@@ -888,7 +941,7 @@ hipError_t launch_p2g(fsl_ctx *c, float *const wsum[3]) {
   float *ring = static_cast<float *>(c->p2g_ring);
   uint8_t *ring_t = reinterpret_cast<uint8_t *>(ring + 6 * rs);
   float *w[3] = {c->p2g_w, c->p2g_w + c->nvox, c->p2g_w + 2 * c->nvox};
-  k_p2g_own<false><<<leaf_grid(c->n_act, c->counts_on_device), 512, 0, c->stream>>>(
+  k_p2g_own<false><<<leaf_grid(c->n_act, c->counts_on_device), kP2gThreads, 0, c->stream>>>(
       c->dom, act_leaves(c), c->lhas, c->vstart, c->P, c->v[0], c->v[1], c->v[2], w[0],
       w[1], w[2], c->vflag, ring, ring_t, rs);
   k_p2g_combine<false><<<leaf_grid(c->n_act, c->counts_on_device), 512, 0, c->stream>>>(
@@ -904,7 +957,7 @@ hipError_t launch_g2p_transpose(fsl_ctx *c, float *const pval[3], float *const o
   for (int a = 0; a < 3; ++a) A.a[3 + a] = pval[a];
   const int64_t rs = c->nleaves * kRing;
   float *ring = static_cast<float *>(c->p2g_ring);
-  k_p2g_own<true><<<leaf_grid(c->n_act, c->counts_on_device), 512, 0, c->stream>>>(
+  k_p2g_own<true><<<leaf_grid(c->n_act, c->counts_on_device), kP2gThreads, 0, c->stream>>>(
       c->dom, act_leaves(c), c->lhas, c->vstart, A, out[0], out[1], out[2], nullptr,
       nullptr, nullptr, nullptr, ring, nullptr, rs);
   k_p2g_combine<true><<<leaf_grid(c->n_act, c->counts_on_device), 512, 0, c->stream>>>(
