@@ -456,8 +456,7 @@ int fsl_create(fsl_ctx **out, const int dims[3], const int tiles[3], int device,
     rc = dev_alloc(ctx, &ctx->Q.a[a], size_t(ctx->cap + fsl::kParticlePad));
   ctx->pbuf0 = ctx->P.a[0];
   if (!rc) rc = dev_alloc(ctx, &ctx->key, size_t(ctx->cap));
-  if (!rc) rc = dev_alloc(ctx, &ctx->perm, size_t(ctx->cap));
-  if (!rc) rc = dev_alloc(ctx, &ctx->skey, size_t(ctx->cap));
+  if (!rc) rc = dev_alloc(ctx, &ctx->perm, size_t(ctx->cap + fsl::kParticlePad));
   if (!rc) rc = dev_alloc(ctx, &ctx->p2g_w, 3 * nv);
   if (!rc) {
     uint8_t *ring = nullptr;
@@ -494,7 +493,7 @@ int fsl_destroy(fsl_ctx *ctx) {
                   ctx->p[0], ctx->p[1], ctx->Ap, ctx->x64, ctx->partials, ctx->pcg, ctx->lhas,
                   ctx->lact, ctx->lfluid, ctx->lscan, ctx->act_list, ctx->fluid_list, ctx->morton,
                   ctx->lrank,
-                  ctx->lflag_m, ctx->key, ctx->skey, ctx->p2g_w, ctx->p2g_ring,
+                  ctx->lflag_m, ctx->key, ctx->p2g_w, ctx->p2g_ring,
                   ctx->perm, ctx->vcount, ctx->vstart, ctx->scan_tmp, ctx->flags, ctx->dcount,
                   ctx->sstate};
   for (void *b : bufs) if (b) (void)hipFree(b);

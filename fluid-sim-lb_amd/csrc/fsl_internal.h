@@ -164,9 +164,10 @@ struct DevFlags {
 struct Particles {
   float *a[9];  // px py pz vx vy vz gx gy gz
 };
-// Particle-sized arrays that P2G reads (the particle arrays and the adjoint's
-// particle values) hold this many elements beyond the capacity: P2G loads
-// four consecutive particles at a time, up to three past a leaf's last one.
+// Particle-sized arrays that are read four elements at a time (the particle
+// arrays and the adjoint's particle values in P2G, the sort permutation in
+// the in-cell sort) hold this many elements beyond the capacity: the loads
+// reach up to three elements past a leaf's last particle.
 constexpr int64_t kParticlePad = 4;
 
 // A leaf list on the device, list[0, n). When `count` is set the length is
@@ -246,7 +247,6 @@ struct fsl_ctx {
   int64_t cap = 0, np = 0;
   fsl::Particles P{}, Q{};
   uint32_t *key = nullptr, *perm = nullptr;
-  uint32_t *skey = nullptr;  // cell of each sorted slot
   // P2G scratch: weight sums of the own cells (3 * nvox) and the ring blocks
   // that carry contributions to neighbouring leaves.
   float *p2g_w = nullptr;

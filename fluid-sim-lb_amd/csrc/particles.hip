@@ -194,14 +194,19 @@ __global__ void k_advect(Dom d, int64_t np, const uint32_t *np_dev, Particles P,
   if (r.rank == 0 && k != kDead) atomicAdd(&vcount[k], uint32_t(r.count));
 }
 
+// Buffer loads of four 32-bit values: the sort and P2G read a cell's run of
+// particles (or particle indices) four at a time through a descriptor based
+// at the leaf's first particle. Word 3 of a raw (untyped, stride 0)
+// descriptor on gfx9-family chips: data format 32 bits, no swizzle.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+constexpr int kRawBufferWord3 = 0x00020000;
+
 // Counting-sort scatter. vcount returns to zero, ready for the next step.
 // A run of equal keys takes one atomic and fills consecutive slots in lane
 // order, ascending from the cell's next free slot, so a cell whose particles
-// arrive in one run is already in its previous order. skey[slot] records the
-// slot's cell.
+// arrive in one run is already in its previous order.
 __global__ void k_scatter(int64_t np, const uint32_t *np_dev, const uint32_t *key,
-                          const uint32_t *vstart, uint32_t *vcount, uint32_t *perm,
-                          uint32_t *skey) {
+                          const uint32_t *vstart, uint32_t *vcount, uint32_t *perm) {
   if (np_dev) np = *np_dev;
   const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const uint32_t k = q < np ? key[q] : kDead;
@@ -211,34 +216,37 @@ __global__ void k_scatter(int64_t np, const uint32_t *np_dev, const uint32_t *ke
   left = __shfl(left, r.head);
   if (k == kDead) return;
   const uint32_t b = vstart[k], total = vstart[k + 1] - b;
-  const uint32_t slot = b + (total - left) + uint32_t(r.rank);
-  perm[slot] = uint32_t(q);
-  skey[slot] = k;
+  perm[b + (total - left) + uint32_t(r.rank)] = uint32_t(q);
 }
 
-// Restore the previous order inside each cell (stable sort). The thread at
-// a cell's first slot checks the cell's old indices and sorts them when runs
-// arrived out of order (a cell filled by several runs). A cell of at most
-// kCellRegs particles is loaded into registers with all loads in flight at
-// once, checked and sorted there (odd-even transposition, padding sorts
-// last) and written back only when it was out of order; larger cells take
-// the insertion sort in memory.
+// Restore the previous order inside each cell (stable sort): one workgroup
+// per leaf in sort order (leaf rank r owns sort indices r * 512 + t), one
+// thread per cell; leaves without particles leave at once. A cell's old
+// indices are sorted when runs arrived out of order (a cell filled by
+// several runs). A cell of at most kCellRegs particles is read into
+// registers four indices per load (a buffer descriptor based at the leaf's
+// first slot; perm carries kParticlePad spare elements for the loads past the
+// leaf's last slot), checked and sorted there (odd-even transposition,
+// padding sorts last) and written back only when it was out of order; larger
+// cells take the insertion sort in memory.
 constexpr uint32_t kCellRegs = 16;
-// n_dev, when set, holds the count of sorted particles (vstart's last entry).
-__global__ void k_cell_sort(uint32_t n, const uint32_t *n_dev, const uint32_t *skey,
-                            const uint32_t *vstart, uint32_t *perm) {
-  if (n_dev) n = *n_dev;
-  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  const uint32_t k = skey[s];
-  if (s > 0 && skey[s - 1] == k) return;
-  const uint32_t b = s, e = vstart[k + 1];
-  if (e - b < 2) return;
+__global__ __launch_bounds__(512) void k_cell_sort(const uint32_t *vstart, uint32_t *perm) {
+  const int64_t s0 = int64_t(blockIdx.x) << 9;
+  const uint32_t first = vstart[s0], last = vstart[s0 + 512];
+  if (last - first < 2) return;
+  const uint32_t b = vstart[s0 + threadIdx.x], e = vstart[s0 + threadIdx.x + 1];
   const uint32_t m = e - b;
+  if (m < 2) return;
   if (m <= kCellRegs) {
+    const __amdgpu_buffer_rsrc_t R = __builtin_amdgcn_make_buffer_rsrc(
+        perm + first, 0, int((last - first + 3) * 4), kRawBufferWord3);
     uint32_t v[kCellRegs];
 #pragma unroll
-    for (uint32_t i = 0; i < kCellRegs; ++i) v[i] = perm[b + (i < m ? i : m - 1)];
+    for (uint32_t j = 0; j < kCellRegs / 4; ++j) {
+      u32x4 x = {~0u, ~0u, ~0u, ~0u};
+      if (4 * j < m) x = __builtin_amdgcn_raw_buffer_load_b128(R, int(b - first + 4 * j) * 4, 0, 0);
+      v[4 * j] = x.x; v[4 * j + 1] = x.y; v[4 * j + 2] = x.z; v[4 * j + 3] = x.w;
+    }
     bool in_order = true;
 #pragma unroll
     for (uint32_t i = 1; i < kCellRegs; ++i)
@@ -442,10 +450,6 @@ struct LeafRuns {
   __amdgpu_buffer_rsrc_t a[6];  // px py pz and the three particle values
   uint32_t first;               // the leaf's first particle
 };
-typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-// Descriptor word 3 for raw (untyped, stride 0) buffer loads on gfx9-family
-// chips: data format 32 bits, no swizzle.
-constexpr int kRawBufferWord3 = 0x00020000;
 
 // One cell's particles, all three components in one pass (each particle's
 // position and velocity are loaded once), summed in particle order into the
@@ -888,11 +892,10 @@ hipError_t launch_sort_tail(fsl_ctx *c, uint32_t n_alive, bool keep_gv) {
   const int64_t nb = dev ? c->np : int64_t(n_alive);
   if (c->np > 0) {
     k_scatter<<<blocks_for(c->np, kThreads), kThreads, 0, c->stream>>>(
-        c->np, np_dev, c->key, c->vstart, c->vcount, c->perm, c->skey);
+        c->np, np_dev, c->key, c->vstart, c->vcount, c->perm);
   }
   if (nb > 0) {
-    k_cell_sort<<<blocks_for(nb, kThreads), kThreads, 0, c->stream>>>(n_alive, na_dev, c->skey,
-                                                                       c->vstart, c->perm);
+    k_cell_sort<<<unsigned(c->nleaves), 512, 0, c->stream>>>(c->vstart, c->perm);
     if (keep_gv)
       k_gather<9><<<blocks_for(nb, kThreads), kThreads, 0, c->stream>>>(n_alive, na_dev, out,
                                                                          c->perm, c->P, c->Q);
