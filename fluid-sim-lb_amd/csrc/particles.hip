@@ -741,30 +741,47 @@ __global__ __launch_bounds__(256) void k_g2p(Dom d, LeafList L,
       s_u[0][idx] = a[3]; s_u[1][idx] = a[4]; s_u[2][idx] = a[5];
     }
     __syncthreads();
-    const uint32_t b = vstart[sort_begin(d, leaf)], e = vstart[sort_begin(d, leaf) + kLeaf];
-    // Four particles per trip (q, q + 256, ...), their loads issued together
-    // (indices clamped to the leaf's last particle; clamped copies are not
-    // processed).
-    constexpr uint32_t kTrip = 4;
-    for (uint32_t q0 = b + threadIdx.x; q0 < e; q0 += kTrip * blockDim.x) {
-      float pp[kTrip][3], pv[kTrip][3];
+    const int64_t sb = sort_begin(d, leaf);
+    const uint32_t b = __builtin_amdgcn_readfirstlane(vstart[sb]),
+                   e = __builtin_amdgcn_readfirstlane(vstart[sb + kLeaf]);
+    // Each thread takes four consecutive particles per trip, read with one
+    // 16-byte load per array through descriptors based at the leaf's first
+    // particle (the arrays carry kParticlePad spare elements, so the loads
+    // past the leaf's last particle stay inside them) and written with one
+    // 16-byte store per array when all four are in the leaf and in the box.
+    // Particles outside the box (removed by DeleteOutsideParticles) keep
+    // their values, so a quad holding one is written particle by particle.
+    const int bytes = int((e - b + 3) * 4);
+    __amdgpu_buffer_rsrc_t R[9];
 #pragma unroll
-      for (uint32_t u = 0; u < kTrip; ++u) {
-        const uint32_t q = q0 + u * blockDim.x, qq = q < e ? q : e - 1;
+    for (int a = 0; a < 9; ++a)
+      R[a] = __builtin_amdgcn_make_buffer_rsrc(P.a[a] + b, 0, bytes, kRawBufferWord3);
+    for (uint32_t q0 = b + 4 * threadIdx.x; q0 < e; q0 += 4 * blockDim.x) {
+      const int off = int(q0 - b) * 4;
+      float pp[4][3], pv[4][3];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          pp[u][a] = P.a[a][qq];
-          pv[u][a] = P.a[3 + a][qq];
-        }
+      for (int a = 0; a < 3; ++a) {
+        const u32x4 x = __builtin_amdgcn_raw_buffer_load_b128(R[a], off, 0, 0);
+        const u32x4 y = __builtin_amdgcn_raw_buffer_load_b128(R[3 + a], off, 0, 0);
+        pp[0][a] = __uint_as_float(x.x); pp[1][a] = __uint_as_float(x.y);
+        pp[2][a] = __uint_as_float(x.z); pp[3][a] = __uint_as_float(x.w);
+        pv[0][a] = __uint_as_float(y.x); pv[1][a] = __uint_as_float(y.y);
+        pv[2][a] = __uint_as_float(y.z); pv[3][a] = __uint_as_float(y.w);
       }
+      float nv[4][3], ng[4][3];
+      bool all_in = q0 + 3 < e;
 #pragma unroll
-      for (uint32_t u = 0; u < kTrip; ++u) {
-        const uint32_t q = q0 + u * blockDim.x;
-        if (q >= e) break;
+      for (uint32_t u = 0; u < 4; ++u) {
         const float *pos = pp[u];
         const int c[3] = {int(floorf(pos[0])), int(floorf(pos[1])), int(floorf(pos[2]))};
-        if (!in_bbox(d, c[0], c[1], c[2])) continue;  // removed by DeleteOutsideParticles
+        const bool in = q0 + u < e && in_bbox(d, c[0], c[1], c[2]);
+        all_in = all_in && in;
         const int lc[3] = {c[0] - o[0] + 1, c[1] - o[1] + 1, c[2] - o[2] + 1};
+        if (!in) {
+#pragma unroll
+          for (int a = 0; a < 3; ++a) { nv[u][a] = 0.0f; ng[u][a] = 0.0f; }
+          continue;
+        }
         const float gv[3] = {interp<0>(pos, c, lc, s_v[0]), interp<1>(pos, c, lc, s_v[1]),
                              interp<2>(pos, c, lc, s_v[2])};
         const float dv[3] = {interp<0>(pos, c, lc, s_u[0]), interp<1>(pos, c, lc, s_u[1]),
@@ -774,8 +791,32 @@ __global__ __launch_bounds__(256) void k_g2p(Dom d, LeafList L,
           const float s1 = dv[a] + pv[u][a];
           const float s2 = s1 - gv[a];
           const float s3 = flip * s2;
-          P.a[3 + a][q] = gv[a] + s3;
-          P.a[6 + a][q] = gv[a];
+          nv[u][a] = gv[a] + s3;
+          ng[u][a] = gv[a];
+        }
+      }
+      if (all_in) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          const u32x4 x = {__float_as_uint(nv[0][a]), __float_as_uint(nv[1][a]),
+                           __float_as_uint(nv[2][a]), __float_as_uint(nv[3][a])};
+          const u32x4 y = {__float_as_uint(ng[0][a]), __float_as_uint(ng[1][a]),
+                           __float_as_uint(ng[2][a]), __float_as_uint(ng[3][a])};
+          __builtin_amdgcn_raw_buffer_store_b128(x, R[3 + a], off, 0, 0);
+          __builtin_amdgcn_raw_buffer_store_b128(y, R[6 + a], off, 0, 0);
+        }
+      } else {
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) {
+          const uint32_t q = q0 + u;
+          if (q >= e) break;
+          const int c[3] = {int(floorf(pp[u][0])), int(floorf(pp[u][1])), int(floorf(pp[u][2]))};
+          if (!in_bbox(d, c[0], c[1], c[2])) continue;
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            P.a[3 + a][q] = nv[u][a];
+            P.a[6 + a][q] = ng[u][a];
+          }
         }
       }
     }
