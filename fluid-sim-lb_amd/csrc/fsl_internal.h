@@ -220,7 +220,7 @@ struct fsl_ctx {
   // Leaves.
   uint32_t *lhas = nullptr;   // leaf holds particles
   uint32_t *lact = nullptr;   // leaf active (26-dilation of lhas)
-  uint32_t *lfluid = nullptr; // leaf holds a FLUID cell
+  uint32_t *lfluid = nullptr; // FLUID rows in the leaf (nonzero: a fluid leaf)
   uint32_t *lscan = nullptr;  // scan scratch over leaves (nleaves + 1)
   int32_t *act_list = nullptr, *fluid_list = nullptr;
   int32_t *morton = nullptr;   // leaf indices in Morton (Z-curve) order of leaf coordinates

@@ -96,8 +96,9 @@ __global__ __launch_bounds__(512) void k_gravity(Dom d, LeafList L,
 
 // ComputeDivergence (flip_simulation.cc:715-736) fused with the solver
 // labels: lab = marker | diag << 2, diag = count of FLUID or AIR neighbours
-// (CreateLaplacianMatrix, solver.cc:214-231). Also flags leaves holding FLUID
-// cells and counts rows per tile (solver.cc:80, metagrid.cc:84-87).
+// (CreateLaplacianMatrix, solver.cc:214-231). Also stores each leaf's count
+// of FLUID rows in lfluid (nonzero marks a fluid leaf); k_row_totals sums
+// them.
 __global__ __launch_bounds__(512) void k_divergence(Dom d, LeafList L,
                                                     const uint8_t *marker, const float *vx,
                                                     const float *vy, const float *vz,
@@ -142,14 +143,39 @@ __global__ __launch_bounds__(512) void k_divergence(Dom d, LeafList L,
     div[v] = dv;
     lab[v] = l;
     const unsigned cnt = unsigned(block_sum<unsigned>(fluid ? 1u : 0u, s_cnt));
-    if (threadIdx.x == 0) {
-      lfluid[v >> 9] = cnt ? 1u : 0u;
-      if (cnt) {
-        atomicAdd(&flags->n_rows, (unsigned long long)cnt);
-        atomicAdd(&flags->tile_rows[tile_of(d, c[0], c[1], c[2]) & 63], (unsigned long long)cnt);
-      }
-    }
+    if (threadIdx.x == 0) lfluid[v >> 9] = cnt;
   });
+}
+
+// Row totals (solver.cc:80) and rows per tile (metagrid.cc:84-87) from the
+// per-leaf FLUID counts: each workgroup sums a contiguous range of leaves
+// per tile in LDS, then adds its nonzero tiles to flags (zeroed before the
+// divergence) with one atomic each. A leaf lies in one tile (tiles are
+// multiples of 8 cells wide); leaves outside the domain hold no rows.
+constexpr int kRowTotalGroups = 64;
+__global__ __launch_bounds__(256) void k_row_totals(Dom d, int64_t nleaves,
+                                                   const uint32_t *lfluid, DevFlags *flags) {
+  __shared__ unsigned long long s_tile[64];
+  if (threadIdx.x < 64) s_tile[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t chunk = (nleaves + gridDim.x - 1) / gridDim.x;
+  const int64_t beg = int64_t(blockIdx.x) * chunk;
+  const int64_t end = beg + chunk < nleaves ? beg + chunk : nleaves;
+  for (int64_t l = beg + threadIdx.x; l < end; l += blockDim.x) {
+    const uint32_t cnt = lfluid[l];
+    if (!cnt) continue;
+    int o[3];
+    leaf_origin(d, l, o);
+    atomicAdd(&s_tile[tile_of(d, o[0], o[1], o[2]) & 63], (unsigned long long)cnt);
+  }
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    const unsigned long long t = s_tile[threadIdx.x];
+    if (t) {
+      atomicAdd(&flags->tile_rows[threadIdx.x], t);
+      atomicAdd(&flags->n_rows, t);
+    }
+  }
 }
 
 // UpdateVelocityFromPressure (flip_simulation.cc:738-778) over faces
@@ -399,6 +425,7 @@ hipError_t launch_divergence(fsl_ctx *c) {
   k_divergence<<<act_grid(c), 512, 0, c->stream>>>(c->dom, act_leaves(c), c->marker, c->v[0],
                                                     c->v[1], c->v[2], c->div, c->lab, c->lfluid,
                                                     c->flags);
+  k_row_totals<<<kRowTotalGroups, 256, 0, c->stream>>>(c->dom, c->nleaves, c->lfluid, c->flags);
   return hipGetLastError();
 }
 hipError_t launch_pressure_update(fsl_ctx *c) {

@@ -330,7 +330,7 @@ __global__ void k_leaf_dilate(Dom d, const uint32_t *lhas, uint32_t *lact) {
 __global__ void k_gather_flags(int64_t n, const int32_t *order, const uint32_t *flag,
                                uint32_t *fm) {
   const int64_t m = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (m < n) fm[m] = flag[order[m]];
+  if (m < n) fm[m] = flag[order[m]] ? 1u : 0u;
 }
 __global__ void k_compact(int64_t n, const int32_t *order, const uint32_t *fm,
                           const uint32_t *pos, int32_t *list, uint32_t *count_out) {
