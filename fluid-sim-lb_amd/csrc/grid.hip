@@ -213,7 +213,13 @@ __global__ __launch_bounds__(512) void k_velocity_update(Dom d, LeafList L, cons
   for_each_voxel(d, L, [&](int64_t v, const int *c) {
     if (c[0] < 0 || c[1] < 0 || c[2] < 0 || c[0] > d.n[0] || c[1] > d.n[1] || c[2] > d.n[2]) return;
     const uint8_t f = vflag[v];
-    if ((f & 1) && !(f & 2)) atomicOr(&flags->vel_update, 1u);
+    // The flag only goes from 0 to 1: one lane per wavefront sets it, and
+    // only while a read still shows it unset (same-address atomics from
+    // every voxel serialise).
+    const unsigned long long need = __ballot((f & 1) && !(f & 2));
+    if (need && (threadIdx.x & 63) == unsigned(__ffsll((long long)need) - 1) &&
+        !__hip_atomic_load(&flags->vel_update, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      atomicOr(&flags->vel_update, 1u);
     if ((f & 3) == 3) {
       ux[v] = vx[v] - ux[v];
       uy[v] = vy[v] - uy[v];

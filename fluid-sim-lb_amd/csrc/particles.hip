@@ -616,69 +616,100 @@ __global__ __launch_bounds__(kP2gThreads) void k_p2g_own(
 // when it holds no particles) plus, for samples on its boundary, the ring
 // entries of the neighbouring leaves with particles, neighbours in (dx, dy,
 // dz) order; then normalises (flip_simulation.cc:900-922) and marks FLUID
-// (flip_simulation.cc:852-857).
+// (flip_simulation.cc:852-857). 128 threads per leaf, each a run of four
+// voxels along z, so a thread's own values move as 16-byte (float) and
+// 4-byte (label) accesses.
+constexpr int kCombineThreads = 128;
 template <bool ADJ>
-__global__ __launch_bounds__(512) void k_p2g_combine(Dom d, LeafList L,
-                                                     const uint32_t *lhas,
-                                                     const uint32_t *vstart, float *vx,
-                                                     float *vy, float *vz, float *wx, float *wy,
-                                                     float *wz, uint8_t *vflag, uint8_t *marker,
-                                                     const float *ring, const uint8_t *ring_t,
-                                                     int64_t rstride, float *wsx, float *wsy,
-                                                     float *wsz) {
+__global__ __launch_bounds__(kCombineThreads) void k_p2g_combine(
+    Dom d, LeafList L, const uint32_t *lhas, const uint32_t *vstart, float *vx, float *vy,
+    float *vz, float *wx, float *wy, float *wz, uint8_t *vflag, uint8_t *marker,
+    const float *ring, const uint8_t *ring_t, int64_t rstride, float *wsx, float *wsy,
+    float *wsz) {
   for_each_leaf(L.list, L.count, L.n, [&](int64_t leaf) {
     const int t = threadIdx.x;
-    const int lx = t >> 6, ly = (t >> 3) & 7, lz = t & 7;
-    const int64_t v = (leaf << 9) | t;
-    float vw[3] = {0.0f, 0.0f, 0.0f}, w[3] = {0.0f, 0.0f, 0.0f};
-    uint8_t touch = 0;
+    const int lx = t >> 4, ly = (t >> 1) & 7, lz0 = (t & 1) * 4;
+    const int64_t v0 = (leaf << 9) | (lx << 6) | (ly << 3) | lz0;
+    float vw[3][4] = {}, w[3][4] = {};
+    uint32_t touch = 0;  // one byte per voxel
+    float *const V[3] = {vx, vy, vz};
+    float *const W[3] = {wx, wy, wz};
     if (lhas[leaf]) {
-      vw[0] = vx[v]; vw[1] = vy[v]; vw[2] = vz[v];
-      if (!ADJ) {
-        w[0] = wx[v]; w[1] = wy[v]; w[2] = wz[v];
-        touch = vflag[v];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const float4 x = *reinterpret_cast<const float4 *>(V[a] + v0);
+        vw[a][0] = x.x; vw[a][1] = x.y; vw[a][2] = x.z; vw[a][3] = x.w;
+        if (!ADJ) {
+          const float4 y = *reinterpret_cast<const float4 *>(W[a] + v0);
+          w[a][0] = y.x; w[a][1] = y.y; w[a][2] = y.z; w[a][3] = y.w;
+        }
       }
+      if (!ADJ) touch = *reinterpret_cast<const uint32_t *>(vflag + v0);
     }
     const int64_t l2 = d.L[2], l12 = int64_t(d.L[1]) * d.L[2];
     const int L[3] = {int(leaf / l12), int((leaf / l2) % d.L[1]), int(leaf % l2)};
-    const int lc[3] = {lx, ly, lz};
-    int lo[3], hi[3];
+    {
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      lo[a] = lc[a] == 0 && L[a] > 0 ? -1 : 0;
-      hi[a] = lc[a] == 7 && L[a] + 1 < d.L[a] ? 1 : 0;
-    }
-    for (int dx = lo[0]; dx <= hi[0]; ++dx)
-      for (int dy = lo[1]; dy <= hi[1]; ++dy)
-        for (int dz = lo[2]; dz <= hi[2]; ++dz) {
-          if (dx == 0 && dy == 0 && dz == 0) continue;
-          const int64_t nb = leaf + dx * l12 + dy * l2 + dz;
-          if (!lhas[nb]) continue;
-          const int64_t e = nb * kRing + ring_index(lx + 1 - 8 * dx, ly + 1 - 8 * dy, lz + 1 - 8 * dz);
+      for (int u = 0; u < 4; ++u) {
+        const int lc[3] = {lx, ly, lz0 + u};
+        int lo[3], hi[3];
 #pragma unroll
-          for (int a = 0; a < 3; ++a) vw[a] += ring[a * rstride + e];
-          if (!ADJ) {
-#pragma unroll
-            for (int a = 0; a < 3; ++a) w[a] += ring[(3 + a) * rstride + e];
-            touch |= ring_t[e];
-          }
+        for (int a = 0; a < 3; ++a) {
+          lo[a] = lc[a] == 0 && L[a] > 0 ? -1 : 0;
+          hi[a] = lc[a] == 7 && L[a] + 1 < d.L[a] ? 1 : 0;
         }
+        for (int dx = lo[0]; dx <= hi[0]; ++dx)
+          for (int dy = lo[1]; dy <= hi[1]; ++dy)
+            for (int dz = lo[2]; dz <= hi[2]; ++dz) {
+              if (dx == 0 && dy == 0 && dz == 0) continue;
+              const int64_t nb = leaf + dx * l12 + dy * l2 + dz;
+              if (!lhas[nb]) continue;
+              const int64_t e =
+                  nb * kRing + ring_index(lc[0] + 1 - 8 * dx, lc[1] + 1 - 8 * dy, lc[2] + 1 - 8 * dz);
+#pragma unroll
+              for (int a = 0; a < 3; ++a) vw[a][u] += ring[a * rstride + e];
+              if (!ADJ) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) w[a][u] += ring[(3 + a) * rstride + e];
+                touch |= uint32_t(ring_t[e]) << (8 * u);
+              }
+            }
+      }
+    }
     if (ADJ) {
-      vx[v] = vw[0]; vy[v] = vw[1]; vz[v] = vw[2];
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+        *reinterpret_cast<float4 *>(V[a] + v0) = make_float4(vw[a][0], vw[a][1], vw[a][2], vw[a][3]);
       return;
     }
-    if (wsx) { wsx[v] = w[0]; wsy[v] = w[1]; wsz[v] = w[2]; }
-    float out[3];
+    if (wsx) {
+      float *const S[3] = {wsx, wsy, wsz};
 #pragma unroll
-    for (int a = 0; a < 3; ++a) out[a] = double(w[a]) < 1e-5 ? 0.0f : vw[a] / w[a];
-    vx[v] = out[0]; vy[v] = out[1]; vz[v] = out[2];
-    vflag[v] = touch ? 1 : 0;
+      for (int a = 0; a < 3; ++a)
+        *reinterpret_cast<float4 *>(S[a] + v0) = make_float4(w[a][0], w[a][1], w[a][2], w[a][3]);
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      float out[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) out[u] = double(w[a][u]) < 1e-5 ? 0.0f : vw[a][u] / w[a][u];
+      *reinterpret_cast<float4 *>(V[a] + v0) = make_float4(out[0], out[1], out[2], out[3]);
+    }
+    uint32_t flag = 0, mark = 0;
     int o[3];
     leaf_origin(d, leaf, o);
-    uint8_t m = kAir;
-    const int64_t sv = sort_index(d, v);
-    if (in_bbox(d, o[0] + lx, o[1] + ly, o[2] + lz) && vstart[sv + 1] > vstart[sv]) m = kFluid;
-    marker[v] = m;
+    // Sort indices of the four voxels are consecutive: five range bounds.
+    const int64_t sv0 = sort_index(d, v0);
+    const uint4 s4 = *reinterpret_cast<const uint4 *>(vstart + sv0);
+    const uint32_t sb[5] = {s4.x, s4.y, s4.z, s4.w, vstart[sv0 + 4]};
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if ((touch >> (8 * u)) & 0xFF) flag |= 1u << (8 * u);
+      if (in_bbox(d, o[0] + lx, o[1] + ly, o[2] + lz0 + u) && sb[u + 1] > sb[u])
+        mark |= uint32_t(kFluid) << (8 * u);
+    }
+    *reinterpret_cast<uint32_t *>(vflag + v0) = flag;
+    *reinterpret_cast<uint32_t *>(marker + v0) = mark;
   });
 }
 
@@ -988,7 +1019,7 @@ hipError_t launch_p2g(fsl_ctx *c, float *const wsum[3]) {
   k_p2g_own<false><<<leaf_grid(c->n_act, c->counts_on_device), kP2gThreads, 0, c->stream>>>(
       c->dom, act_leaves(c), c->lhas, c->vstart, c->P, c->v[0], c->v[1], c->v[2], w[0],
       w[1], w[2], c->vflag, ring, ring_t, rs);
-  k_p2g_combine<false><<<leaf_grid(c->n_act, c->counts_on_device), 512, 0, c->stream>>>(
+  k_p2g_combine<false><<<leaf_grid(c->n_act, c->counts_on_device), kCombineThreads, 0, c->stream>>>(
       c->dom, act_leaves(c), c->lhas, c->vstart, c->v[0], c->v[1], c->v[2], w[0], w[1],
       w[2], c->vflag, c->marker, ring, ring_t, rs, wsum ? wsum[0] : nullptr,
       wsum ? wsum[1] : nullptr, wsum ? wsum[2] : nullptr);
@@ -1004,7 +1035,7 @@ hipError_t launch_g2p_transpose(fsl_ctx *c, float *const pval[3], float *const o
   k_p2g_own<true><<<leaf_grid(c->n_act, c->counts_on_device), kP2gThreads, 0, c->stream>>>(
       c->dom, act_leaves(c), c->lhas, c->vstart, A, out[0], out[1], out[2], nullptr,
       nullptr, nullptr, nullptr, ring, nullptr, rs);
-  k_p2g_combine<true><<<leaf_grid(c->n_act, c->counts_on_device), 512, 0, c->stream>>>(
+  k_p2g_combine<true><<<leaf_grid(c->n_act, c->counts_on_device), kCombineThreads, 0, c->stream>>>(
       c->dom, act_leaves(c), c->lhas, c->vstart, out[0], out[1], out[2], nullptr,
       nullptr, nullptr, nullptr, nullptr, ring, nullptr, rs, nullptr, nullptr, nullptr);
   return hipGetLastError();
