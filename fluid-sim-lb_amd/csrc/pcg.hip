@@ -169,22 +169,25 @@ struct Walk {
   __device__ bool valid() const { return mine() < end; }
   __device__ void next() { w += ng * step; }
 };
+// NT: the workgroup size when the kernel fixes it (0: read blockDim, which
+// is a memory load at the kernel's start).
+template <int NT = 0>
 __device__ inline Walk walk(int64_t nf) {
   const int64_t per = gridDim.x / kXcd;
   const int64_t chunk = (nf + kXcd - 1) / kXcd;
   const int64_t beg = int64_t(blockIdx.x % kXcd) * chunk;
-  return {beg + blockIdx.x / kXcd, beg + chunk < nf ? beg + chunk : nf, per,
-          int64_t(blockDim.x / kG)};
+  const int nt = NT ? NT : int(blockDim.x);
+  return {beg + blockIdx.x / kXcd, beg + chunk < nf ? beg + chunk : nf, per, int64_t(nt / kG)};
 }
 __device__ inline int gtid() { return threadIdx.x % kG; }
 
 // Block-reduce K values and publish this workgroup's partial sums: wave
 // sums, one barrier, then thread 0 adds the waves' sums in order.
-template <typename A, int K>
+template <typename A, int K, int NT = 0>
 __device__ inline void publish(A vals[K], A *partials) {
   __shared__ A s_red[K][16];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int nw = (blockDim.x + 63) >> 6;
+  const int nw = ((NT ? NT : int(blockDim.x)) + 63) >> 6;
 #pragma unroll
   for (int k = 0; k < K; ++k) vals[k] = wave_sum(vals[k]);
   if (lane == 0) {
@@ -253,6 +256,61 @@ __device__ inline void total_all(const A *partials, int n, double out[K]) {
   }
 }
 
+// The same totals in two steps, so that the loads of the partials are in
+// flight while the caller issues other loads: parts_issue loads partials
+// t, t + NT, ..., t + (R - 1) NT of thread t (NT threads) at once (a round
+// past n re-reads partial 0 and is discarded), parts_total adds them (and,
+// beyond R rounds, the rest in a loop) in the order of total_all, so the
+// bits are total_all's.
+template <typename A, int K, int R>
+struct PartRegs {
+  A v[R][K];
+};
+template <typename A, int K, int NT, int R>
+__device__ inline void parts_issue(const A *partials, int n, PartRegs<A, K, R> &pr) {
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int q = int(threadIdx.x) + r * NT;
+    const int qq = q < n ? q : 0;
+#pragma unroll
+    for (int j = 0; j < K; ++j) pr.v[r][j] = partials[K * qq + j];
+  }
+}
+template <typename A, int K, int NT, int R>
+__device__ inline void parts_total(const A *partials, int n, const PartRegs<A, K, R> &pr,
+                                   double out[K]) {
+  __shared__ A s_t[K][16];
+  A v[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) v[j] = A(0);
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (int(threadIdx.x) + r * NT < n) {
+#pragma unroll
+      for (int j = 0; j < K; ++j) v[j] += pr.v[r][j];
+    }
+  }
+  for (int q = int(threadIdx.x) + R * NT; q < n; q += NT) {
+#pragma unroll
+    for (int j = 0; j < K; ++j) v[j] += partials[K * q + j];
+  }
+#pragma unroll
+  for (int j = 0; j < K; ++j) v[j] = wave_sum(v[j]);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  constexpr int nw = (NT + 63) >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int j = 0; j < K; ++j) s_t[j][wave] = v[j];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    A t = A(0);
+    for (int w = 0; w < nw; ++w) t += s_t[j][w];
+    out[j] = double(t);
+  }
+}
+
 // Mode bits of single-reduction iteration k: bit 0, x is updated (odd k);
 // bit 1, x lacks alpha_{k-1} p_{k-1} (odd k, except the iteration a solve
 // resumed at); bit 2, x is folded into x64 before a scheduled refresh.
@@ -260,9 +318,12 @@ inline int sr_mode_of(int k, int resume_k, int refresh) {
   return ((k & 1) ? (k == resume_k ? 1 : 3) : 0) | (refresh > 0 && (k + 1) % refresh == 0 ? 4 : 0);
 }
 
-__device__ inline bool converged(const PcgState *st, double rr) {
+__device__ inline bool converged(int stop_mode, double thr, double rr) {
   // Relative mode uses <=, so a zero right-hand side (x0 = 0) stops at once.
-  return st->stop_mode == FSL_STOP_ABS_RR ? rr < st->thr : sqrt(rr) <= st->thr;
+  return stop_mode == FSL_STOP_ABS_RR ? rr < thr : sqrt(rr) <= thr;
+}
+__device__ inline bool converged(const PcgState *st, double rr) {
+  return converged(st->stop_mode, st->thr, rr);
 }
 
 // Loads p of a leaf and its halo into LDS, where cell values come from
@@ -705,6 +766,15 @@ __global__ __launch_bounds__(kT) void k_refresh_r(Dom d, const int32_t *flist, i
 // p are double-buffered by iteration parity, as other workgroups read the
 // halo values of the iteration's inputs.
 
+// Workgroups of k_cg per CU. Each launch begins with every workgroup reading
+// all partials of the previous launch and ends with a workgroup reduction
+// and a store, so these per-launch costs grow with the workgroup count;
+// four per CU still keep the leaf rounds fed.
+constexpr int kCgMaxPerCu = 4;
+// Rounds of partials each thread of k_cg loads at once: with 256 CUs and one
+// 256-thread workgroup per leaf group, at most kCgMaxPerCu rounds.
+constexpr int kCgPartRounds = kCgMaxPerCu;
+
 // Tile P over [-2, 10)^3 of the leaf and tile U over [-1, 9)^3, with row
 // strides padded so that the wave access patterns of the stencil and of the
 // halo faces spread over the LDS banks (x strides 151 and 137 words instead
@@ -903,11 +973,28 @@ __global__ __launch_bounds__(kG * GR) void k_cg(Dom d, const int32_t *flist, int
   __shared__ int32_t s_leaf[GR][kSched];
   // 1 / diag by label (label = FLUID | diag << 2; 0 for every non-row label).
   __shared__ V s_dinv[32];
+  const int g = threadIdx.x / kG, t = gtid();
+  // First the loads nothing else waits for: this group's leaf schedule and
+  // the partials of the previous launch, all in flight together.
+  const Walk q0 = walk<kG * GR>(nf);
+  const int64_t pos0 = q0.mine() + int64_t(t) * q0.ng * q0.step;
+  const int32_t sched0 = flist[pos0 < q0.end ? pos0 : 0];
+  const int32_t leaf0 = flist[q0.mine() < q0.end ? q0.mine() : 0];  // round 0's leaf
+  PartRegs<A, 3, kCgPartRounds> pr;
+  parts_issue<A, 3, kG * GR, kCgPartRounds>(part_in, g_parts, pr);
+  // State of iteration k - 1 (slots (k - 1) & 1 = (k + 1) & 1, which this
+  // launch does not write) and the stop test's input, read before anything
+  // is stored.
+  const double rzh_prev = st->rzh[(k + 1) & 1], alph_prev = st->alph[(k + 1) & 1];
+  const int stop_k = st->stop_k, stop_mode = st->stop_mode, max_iter = st->max_iter;
+  const double thr = st->thr;
+  // Keeps the loads above ahead of the exit test below (the compiler would
+  // otherwise move them after it, into the path that uses them).
+  asm volatile("" ::: "memory");
   // Exit only on a stop stored by an earlier launch: workgroup 0 of this
   // launch may store its own stop before other workgroups start, and those
   // must still reach the same decision and apply the pending x update.
-  if (st->stop_k < k) return;
-  const int g = threadIdx.x / kG, t = gtid();
+  if (stop_k < k) return;
   const int w = __builtin_amdgcn_readfirstlane(t >> 6), l = t & 63;
   if (threadIdx.x < 32) {
     const int lb = threadIdx.x;
@@ -946,7 +1033,7 @@ __global__ __launch_bounds__(kG * GR) void k_cg(Dom d, const int32_t *flist, int
 
   struct Pre {
     V orr[2], op[2], ox[2], hr[2][2], hp[2][2];
-    uint32_t ol, hl[2];
+    uint16_t ol, hl[2];
     Off base;
     bool valid;
   };
@@ -958,46 +1045,47 @@ __global__ __launch_bounds__(kG * GR) void k_cg(Dom d, const int32_t *flist, int
     c.valid = valid;
     c.base = Off(leaf) << 9;
     const Off v = c.base + Off(2 * t);
-    c.ol = load_labels<2>(lab, v);
+    c.ol = uint16_t(load_labels<2>(lab, v));
     load_run<2>(rin, v, c.orr);
     load_run<2>(pin, v, c.op);
     if (mode & 1) load_run<2>(x, v, c.ox);
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
       const Off hv = c.base + Off(hoff[s]);
-      c.hl[s] = load_labels<2>(lab, hv);
+      c.hl[s] = uint16_t(load_labels<2>(lab, hv));
       load_run<2>(rin, hv, c.hr[s]);
       load_run<2>(pin, hv, c.hp[s]);
     }
   };
-  const Walk q0 = walk(nf);
   Walk q = q0;
   auto refill = [&](int j0) {
     const int64_t pos = q0.mine() + int64_t(j0 + t) * q0.ng * q0.step;
     s_leaf[g][t] = flist[pos < q0.end ? pos : 0];
   };
   int j = 0;  // round of q
-  refill(0);
-  __syncthreads();
   Pre cur;
-  cur.valid = false;
-  if (q.more()) load(q.valid(), s_leaf[g][0], cur);
-  // Scalars of iteration k, reduced under the latency of the first loads.
+  // Issued also when the group has no leaf (leaf0 is then a valid list entry
+  // and the values are discarded): with no branch around these loads, the
+  // wait for the partials below does not wait for them.
+  load(q.more() && q.valid(), leaf0, cur);
+  // The schedule is read from round 1 on, after the barrier in parts_total.
+  s_leaf[g][t] = sched0;
+  // Scalars of iteration k, reduced under the latency of the first leaf's loads.
   double tot[3];
-  total_all<A, 3>(part_in, g_parts, tot);
+  parts_total<A, 3, kG * GR, kCgPartRounds>(part_in, g_parts, pr, tot);
   const A gam = A(tot[0]), del = A(tot[1]);
   const double rr = tot[2];
   A beta = A(0), pAp = del;
   if (k > 0) {
-    beta = gam / A(st->rzh[(k - 1) & 1]);
+    beta = gam / A(rzh_prev);
     const A m = beta * gam;
-    pAp = del - m / A(st->alph[(k - 1) & 1]);
+    pAp = del - m / A(alph_prev);
   }
   const A alpha_a = gam / pAp;
   int status = 0;
   if (isnan(rr)) status = FSL_ERR_NAN;
   else if (!(double(beta) <= 1e5)) status = FSL_ERR_BETA;  // solver.cc:361
-  bool stop = status != 0 || converged(st, rr) || k >= st->max_iter;
+  bool stop = status != 0 || converged(stop_mode, thr, rr) || k >= max_iter;
   if (!stop && !(double(pAp) > 0.0 && double(alpha_a) <= 1e5)) {  // solver.cc:333
     status = FSL_ERR_ALPHA;
     stop = true;
@@ -1018,7 +1106,7 @@ __global__ __launch_bounds__(kG * GR) void k_cg(Dom d, const int32_t *flist, int
     }
   }
   // alpha_{k-1}: x lacks alpha_{k-1} p_{k-1} when bit 1 of mode is set.
-  const V alpha_prev = V(st->alph[(k + 1) & 1]);
+  const V alpha_prev = V(alph_prev);
   if (stop) {
     if (status == 0 && (mode & 2)) {
       // Stopping at x_k: apply the pending alpha_{k-1} p_{k-1} (p_{k-1} is
@@ -1191,7 +1279,7 @@ __global__ __launch_bounds__(kG * GR) void k_cg(Dom d, const int32_t *flist, int
       }
     }
   }
-  publish<A, 3>(acc, part_out);
+  publish<A, 3, kG * GR>(acc, part_out);
 }
 
 // MIXED residual refresh (single-reduction path): r = b - A x64 in FP64 on
@@ -1619,11 +1707,6 @@ int resident_per_xcd(K kernel, int threads, int max_per_cu) {
   }
   return cached;
 }
-// Workgroups of k_cg per CU. Each launch begins with every workgroup reading
-// all partials of the previous launch and ends with a workgroup reduction
-// and a store, so these per-launch costs grow with the workgroup count;
-// four per CU still keep the leaf rounds fed.
-constexpr int kCgMaxPerCu = 4;
 template <typename K>
 inline int grid_cg(int64_t n, K kernel) {
   const int64_t cap = std::min(resident_per_xcd(kernel, kG * kCgGroups, kCgMaxPerCu), 256);
