@@ -353,25 +353,37 @@ int free_checkpoints(fsl_ctx *ctx) {
   return FSL_OK;
 }
 
+// `host` may also be a device array of the context's device: it is then read
+// in place.
 int upload_box(fsl_ctx *ctx, const float *host, float *const g[3], bool add) {
   AdjState &A = *ctx->adj;
   const int64_t nb = box_count(ctx->dom);
-  FSL_HIP(hipMemcpyAsync(A.box, host, size_t(3 * nb) * sizeof(float), hipMemcpyHostToDevice,
-                         ctx->stream));
-  k_box_in<<<blocks_for(nb, kThreads), kThreads, 0, ctx->stream>>>(ctx->dom, A.box, g[0], g[1],
+  const float *src = A.box;
+  if (on_ctx_device(ctx, host)) {
+    src = host;
+  } else {
+    FSL_HIP(hipMemcpyAsync(A.box, host, size_t(3 * nb) * sizeof(float), hipMemcpyHostToDevice,
+                           ctx->stream));
+  }
+  k_box_in<<<blocks_for(nb, kThreads), kThreads, 0, ctx->stream>>>(ctx->dom, src, g[0], g[1],
                                                                     g[2], add ? 1 : 0);
   FSL_HIP(hipGetLastError());
   return FSL_OK;
 }
 
+// `host` may also be a device array of the context's device: it is then
+// written in place.
 int download_box(fsl_ctx *ctx, float *const g[3], int bit, float *host) {
   AdjState &A = *ctx->adj;
   const int64_t nb = box_count(ctx->dom);
+  const bool dev = on_ctx_device(ctx, host);
   k_box_out<<<blocks_for(nb, kThreads), kThreads, 0, ctx->stream>>>(ctx->dom, g[0], g[1], g[2],
-                                                                     ctx->vflag, bit, A.box);
+                                                                     ctx->vflag, bit,
+                                                                     dev ? host : A.box);
   FSL_HIP(hipGetLastError());
-  FSL_HIP(hipMemcpyAsync(host, A.box, size_t(3 * nb) * sizeof(float), hipMemcpyDeviceToHost,
-                         ctx->stream));
+  if (!dev)
+    FSL_HIP(hipMemcpyAsync(host, A.box, size_t(3 * nb) * sizeof(float), hipMemcpyDeviceToHost,
+                           ctx->stream));
   FSL_HIP(hipStreamSynchronize(ctx->stream));
   return FSL_OK;
 }

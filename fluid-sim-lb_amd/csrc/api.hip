@@ -978,9 +978,27 @@ int fsl_compute_dt(fsl_ctx *ctx, int cfl, float frame_step, float *step, float *
   return FSL_OK;
 }
 
+namespace fsl {
+bool on_ctx_device(const fsl_ctx *c, const void *p) {
+  if (!p) return false;
+  hipPointerAttribute_t a{};
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();  // plain host memory: clear the error the query leaves
+    return false;
+  }
+  return a.type == hipMemoryTypeDevice && a.device == c->device;
+}
+}  // namespace fsl
+
 int fsl_get_vec(fsl_ctx *ctx, int which, float *out, uint8_t *active) {
   CHECK_CTX();
   if (!out || (which != FSL_GRID_VELOCITY && which != FSL_GRID_VELOCITY_UPDATE)) return FSL_ERR_ARG;
+  if (fsl::on_ctx_device(ctx, out) && (!active || fsl::on_ctx_device(ctx, active))) {
+    // Device arrays: written in place on the context's stream.
+    FSL_HIP(fsl::launch_box_vec(ctx, which, out, active, true));
+    FSL_HIP(hipStreamSynchronize(ctx->stream));
+    return FSL_OK;
+  }
   const int64_t nb = box_count(ctx);
   float *dbox = nullptr;
   uint8_t *dact = nullptr;
@@ -1000,6 +1018,13 @@ int fsl_set_vec(fsl_ctx *ctx, int which, const float *in, const uint8_t *active)
   if (!in || !active || (which != FSL_GRID_VELOCITY && which != FSL_GRID_VELOCITY_UPDATE)) return FSL_ERR_ARG;
   int rc = all_leaves(ctx);
   if (rc) return rc;
+  if (fsl::on_ctx_device(ctx, in) && fsl::on_ctx_device(ctx, active)) {
+    // Device arrays: read in place on the context's stream.
+    FSL_HIP(fsl::launch_box_vec(ctx, which, const_cast<float *>(in), const_cast<uint8_t *>(active),
+                                false));
+    FSL_HIP(hipStreamSynchronize(ctx->stream));
+    return FSL_OK;
+  }
   const int64_t nb = box_count(ctx);
   float *dbox = nullptr;
   uint8_t *dact = nullptr;

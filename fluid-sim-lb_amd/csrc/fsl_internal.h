@@ -323,6 +323,9 @@ hipError_t launch_cfl(fsl_ctx *c);
 hipError_t launch_dt(fsl_ctx *c, const DtParams &p);
 hipError_t launch_fill_u32(fsl_ctx *c, uint32_t *p, int64_t n, uint32_t value);
 hipError_t launch_box_vec(fsl_ctx *c, int which, float *out3, uint8_t *act, bool to_box);
+// True when p points into device memory of the context's device (a caller's
+// array that kernels can read or write directly); false for host memory.
+bool on_ctx_device(const fsl_ctx *c, const void *p);
 hipError_t launch_box_scalar(fsl_ctx *c, float *grid, float *box, bool to_box);
 hipError_t launch_box_u8(fsl_ctx *c, uint8_t *grid, int32_t *box, bool to_box);
 hipError_t launch_all_leaves(fsl_ctx *c);

@@ -155,8 +155,19 @@ class FslError(RuntimeError):
         self.code = code
 
 
-def _ptr(a: np.ndarray | None):
-    return None if a is None else a.ctypes.data_as(C.c_void_p)
+def _ptr(a):
+    """Address of a numpy array, or of a torch tensor's storage (device
+    tensors are read and written in place by the entry points that accept
+    device arrays)."""
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data_as(C.c_void_p)
+    return C.c_void_p(a.data_ptr())
+
+
+def _is_device_tensor(a) -> bool:
+    return not isinstance(a, np.ndarray) and getattr(a, "is_cuda", False)
 
 
 def solve_params(precision=PCG_FP32, stop_mode=STOP_ABS_RR, threshold=None, dims=None,
@@ -336,10 +347,21 @@ class Context:
         self.call("fsl_project_forward", C.byref(params), C.byref(st))
         return st.as_dict()
 
-    def project_backward(self, params: SolveParams, grad_out: np.ndarray):
-        g = np.ascontiguousarray(grad_out, dtype=np.float32)
-        assert g.shape == self.box + (3,)
-        out = np.empty_like(g)
+    def project_backward(self, params: SolveParams, grad_out):
+        """P^T grad_out. A float32 device tensor in box format is read in
+        place and the result is a device tensor (no host staging); anything
+        else goes through host memory as a numpy array."""
+        if _is_device_tensor(grad_out):
+            import torch
+
+            g = grad_out.detach().to(torch.float32).contiguous()
+            assert tuple(g.shape) == self.box + (3,)
+            out = torch.empty_like(g)
+            torch.cuda.current_stream(g.device).synchronize()
+        else:
+            g = np.ascontiguousarray(grad_out, dtype=np.float32)
+            assert g.shape == self.box + (3,)
+            out = np.empty_like(g)
         st = SolveStats()
         self.call("fsl_project_backward", C.byref(params), _ptr(g), _ptr(out), C.byref(st))
         return out, st.as_dict()
@@ -437,16 +459,36 @@ class Context:
         return st.as_dict()
 
     # -- grids (box format over [-1, n])
-    def get_vec(self, which=GRID_VELOCITY):
-        out = np.empty(self.box + (3,), np.float32)
-        act = np.empty(self.box, np.uint8)
+    def get_vec(self, which=GRID_VELOCITY, device=None):
+        """(values, activity) in box format: numpy arrays, or float32 / uint8
+        tensors on `device` (a torch CUDA device of this context), written
+        in place with no host staging."""
+        if device is not None:
+            import torch
+
+            out = torch.empty(self.box + (3,), dtype=torch.float32, device=device)
+            act = torch.empty(self.box, dtype=torch.uint8, device=device)
+            torch.cuda.current_stream(out.device).synchronize()
+        else:
+            out = np.empty(self.box + (3,), np.float32)
+            act = np.empty(self.box, np.uint8)
         self.call("fsl_get_vec", int(which), _ptr(out), _ptr(act))
         return out, act
 
     def set_vec(self, which, values, active):
-        v = np.ascontiguousarray(values, dtype=np.float32)
-        a = np.ascontiguousarray(active, dtype=np.uint8)
-        assert v.shape == self.box + (3,) and a.shape == self.box
+        """Sets a vector grid from box-format values and activity: numpy
+        arrays, or CUDA tensors of this context's device (read in place)."""
+        if _is_device_tensor(values):
+            import torch
+
+            v = values.detach().to(torch.float32).contiguous()
+            a = torch.as_tensor(active, device=v.device).to(torch.uint8).contiguous()
+            assert tuple(v.shape) == self.box + (3,) and tuple(a.shape) == self.box
+            torch.cuda.current_stream(v.device).synchronize()
+        else:
+            v = np.ascontiguousarray(values, dtype=np.float32)
+            a = np.ascontiguousarray(active, dtype=np.uint8)
+            assert v.shape == self.box + (3,) and a.shape == self.box
         self.call("fsl_set_vec", int(which), _ptr(v), _ptr(a))
 
     def get_scalar(self, which):

@@ -225,6 +225,13 @@ int fsl_set_step_graph(fsl_ctx *ctx, int enable);
 int fsl_compute_dt(fsl_ctx *ctx, int cfl, float frame_step, float *step, float *dt);
 
 /* ---- grid access in box format ---- */
+/* fsl_get_vec / fsl_set_vec also take arrays in device memory of the
+ * context's device (for example a torch tensor's storage): those are read or
+ * written in place on the context's stream, with no staging copy. The
+ * caller's writes to them must be complete (its stream synchronized) before
+ * the call; the call returns after its own work on them is complete. With
+ * fsl_get_vec, out and a non-null active must both be host or both be
+ * device arrays. */
 int fsl_get_vec(fsl_ctx *ctx, int which, float *out, uint8_t *active);
 int fsl_set_vec(fsl_ctx *ctx, int which, const float *in, const uint8_t *active);
 int fsl_get_scalar(fsl_ctx *ctx, int which, float *out);
@@ -303,7 +310,9 @@ int fsl_project_forward(fsl_ctx *ctx, const fsl_solve_params *params, fsl_solve_
 /* Backward: given g = dL/du_post (box format, 3 floats per voxel), writes
  * dL/du = P^T g (box format). One adjoint PCG solve on the same operator,
  * with the labels of the last fsl_divergence / fsl_project_forward. The
- * divergence and pressure grids are left as the forward pass wrote them. */
+ * divergence and pressure grids are left as the forward pass wrote them.
+ * grad_out and grad_in may be host arrays or device arrays of the context's
+ * device (read and written in place, as with fsl_get_vec). */
 int fsl_project_backward(fsl_ctx *ctx, const fsl_solve_params *params, const float *grad_out,
                          float *grad_in, fsl_solve_stats *stats);
 
