@@ -182,10 +182,15 @@ def pcg_compare(ctx, dims):
         rec["pcg_GBps"] = bytes_it * it / st["seconds"] / 1e9 if st["seconds"] > 0 else None
         out[name] = rec
         pressures[name] = ctx.get_scalar(fsl.GRID_PRESSURE).astype(np.float64)
-    # Un-hooked solve times (the timing hook's events slow the launches).
+    # Un-hooked solve times (the timing hook's events slow the launches). Each
+    # solve runs twice and the second is timed: a single-reduction solve
+    # enqueues as many launches as the previous solve on the context took, so
+    # the first run after a different solve (e.g. the 1e-12 one) would time
+    # launches queued past convergence.
     for name, prec, var, stop, tol, pc in runs:
         prm = fsl.solve_params(precision=prec, stop_mode=stop, threshold=tol, dims=dims,
                                max_iterations=20000, variant=var, preconditioner=pc)
+        ctx.solve(prm)
         out[name]["solve_s_unhooked"] = ctx.solve(prm)["seconds"]
     ref = pressures["fp64_tight"]
     den = np.linalg.norm(ref)
@@ -204,7 +209,8 @@ def pcg_compare(ctx, dims):
         for name, prec in (("mixed", fsl.PCG_MIXED), ("fp64", fsl.PCG_FP64)):
             prm = fsl.solve_params(precision=prec, stop_mode=fsl.STOP_REL, threshold=tol,
                                    dims=dims, max_iterations=20000)
-            st = ctx.solve(prm)
+            ctx.solve(prm)
+            st = ctx.solve(prm)  # timed with the launch count predicted right
             pr = ctx.get_scalar(fsl.GRID_PRESSURE).astype(np.float64)
             row[name] = {"iterations": st["iterations"], "solve_s": st["seconds"],
                          "true_rel_residual": st["true_rel_residual"],

@@ -352,6 +352,18 @@ int require_grid(fsl_ctx *ctx) {
 
 }  // namespace
 
+namespace fsl {
+bool on_ctx_device(const fsl_ctx *c, const void *p) {
+  if (!p) return false;
+  hipPointerAttribute_t a{};
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();  // plain host memory: clear the error the query leaves
+    return false;
+  }
+  return a.type == hipMemoryTypeDevice && a.device == c->device;
+}
+}  // namespace fsl
+
 extern "C" {
 
 int fsl_create(fsl_ctx **out, const int dims[3], const int tiles[3], int device,
@@ -977,18 +989,6 @@ int fsl_compute_dt(fsl_ctx *ctx, int cfl, float frame_step, float *step, float *
   *dt = g;
   return FSL_OK;
 }
-
-namespace fsl {
-bool on_ctx_device(const fsl_ctx *c, const void *p) {
-  if (!p) return false;
-  hipPointerAttribute_t a{};
-  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
-    (void)hipGetLastError();  // plain host memory: clear the error the query leaves
-    return false;
-  }
-  return a.type == hipMemoryTypeDevice && a.device == c->device;
-}
-}  // namespace fsl
 
 int fsl_get_vec(fsl_ctx *ctx, int which, float *out, uint8_t *active) {
   CHECK_CTX();

@@ -1136,9 +1136,8 @@ __global__ __launch_bounds__(kG * GR) void k_cg(Dom d, const int32_t *flist, int
   const V alpha = V(alpha_a), bv = V(beta);
   // p = u + beta p_old with u = dinv * r; the select drops whatever a
   // non-row cell holds.
-  auto pnew = [&](V r, V p, uint32_t lb, V &di) {
+  auto pnew = [&](V r, V p, uint32_t lb, V di) {
     const uint8_t lc = uint8_t(lb);
-    di = s_dinv[lc & 31];
     const V u = r * di;
     const V m = bv * p;
     const V pv = k == 0 ? u : u + m;
@@ -1156,9 +1155,19 @@ __global__ __launch_bounds__(kG * GR) void k_cg(Dom d, const int32_t *flist, int
     const Off base = cur.base;
     V wr[2], wp[2], wx[2], wd[2], wo[2];
     uint8_t wl[2];
+    // The 1 / diag of the own and halo cells, read from the table together.
+    V hd[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
       wl[i] = uint8_t(cur.ol >> (8 * i));
+      wd[i] = s_dinv[wl[i] & 31];
+    }
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+      for (int e = 0; e < 2; ++e) hd[s][e] = s_dinv[(cur.hl[s] >> (8 * e)) & 31];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
       wp[i] = pnew(cur.orr[i], cur.op[i], wl[i], wd[i]);
       wr[i] = cur.orr[i];
       wx[i] = cur.ox[i];
@@ -1173,8 +1182,7 @@ __global__ __launch_bounds__(kG * GR) void k_cg(Dom d, const int32_t *flist, int
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
           const uint32_t lb = cur.hl[s] >> (8 * e);
-          V di;
-          tp[hp0[s] + e] = pnew(cur.hr[s][e], cur.hp[s][e], lb, di);
+          tp[hp0[s] + e] = pnew(cur.hr[s][e], cur.hp[s][e], lb, hd[s][e]);
           if ((rmask[s] >> e) & 1) {
             tr1[rbase[s] + e] = cur.hr[s][e];
             tl1[rbase[s] + e] = uint8_t(lb);

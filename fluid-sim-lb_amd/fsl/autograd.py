@@ -11,6 +11,11 @@ A CUDA tensor on the context's device crosses the C ABI in place: the entry
 points read and write its storage on the library's stream (after torch's
 current stream has been synchronized), so nothing is staged through host
 memory. A CPU tensor is copied through host arrays.
+
+In a process that uses both torch's GPU tensors and the library, initialize
+torch's GPU runtime (torch.cuda.init()) before the library's first call:
+torch's lazy initialization has failed ("No HIP GPUs are available") in a
+process where the library had already run rollouts.
 """
 from __future__ import annotations
 

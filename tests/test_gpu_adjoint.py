@@ -303,37 +303,63 @@ def test_autograd_projection_matches_c_abi():
     ctx.close()
 
 
+# Run in a fresh process that initializes torch's GPU runtime before the
+# library's: torch's lazy CUDA initialization has failed ("No HIP GPUs are
+# available") in a process where the library had already run rollouts.
+_DEVICE_AUTOGRAD = r"""
+import sys
+import numpy as np
+import torch
+torch.cuda.init()
+sys.path[:0] = [{root!r}, {pkg!r}, {tests!r}]
+import fsl
+from fsl import scenes
+from fsl.autograd import project
+from test_gpu_adjoint import forward_labels, projection_ctx, tight
+
+n = (32, 32, 32)
+marker = forward_labels(n, scenes.sphere_drop_mask(n, 8, (16, 23, 16), 5), 8)
+ctx = projection_ctx(n, marker)
+rng = np.random.default_rng(11)
+u_h = rng.standard_normal(ctx.box + (3,)).astype(np.float32)
+w_h = rng.standard_normal(ctx.box + (3,)).astype(np.float32)
+dev = torch.device("cuda:0")
+u = torch.tensor(u_h, device=dev, requires_grad=True)
+w = torch.tensor(w_h, device=dev)
+up = project(ctx, u, tight())
+assert up.is_cuda
+up_dev = up.detach().cpu().numpy()
+(up * w).sum().backward()
+assert u.grad.is_cuda
+g_dev = u.grad.cpu().numpy()
+# The same projection through host arrays.
+ctx.set_vec(fsl.GRID_VELOCITY, u_h, np.ones(ctx.box, np.uint8))
+ctx.project_forward(tight())
+up_host, _ = ctx.get_vec(fsl.GRID_VELOCITY)
+g_host, _ = ctx.project_backward(tight(), w_h)
+assert np.array_equal(up_dev, up_host), "forward values differ"
+assert np.array_equal(g_dev, g_host), "gradients differ"
+ctx.close()
+print("device autograd ok")
+"""
+
+
 def test_autograd_projection_device_tensors_in_place():
     """CUDA tensors cross the C ABI in place (no host staging): the forward
     values and the gradient equal the host-array path's bit for bit, and stay
     on the device."""
-    import torch
+    import os
+    import subprocess
+    import sys
 
-    from fsl.autograd import project
-
-    n = (32, 32, 32)
-    marker = forward_labels(n, scenes.sphere_drop_mask(n, 8, (16, 23, 16), 5), 8)
-    ctx = projection_ctx(n, marker)
-    rng = np.random.default_rng(11)
-    u_h = rng.standard_normal(ctx.box + (3,)).astype(np.float32)
-    w_h = rng.standard_normal(ctx.box + (3,)).astype(np.float32)
-    dev = torch.device("cuda:0")
-    u = torch.tensor(u_h, device=dev, requires_grad=True)
-    w = torch.tensor(w_h, device=dev)
-    up = project(ctx, u, tight())
-    assert up.is_cuda
-    up_dev = up.detach().cpu().numpy()
-    (up * w).sum().backward()
-    assert u.grad.is_cuda
-    g_dev = u.grad.cpu().numpy()
-    # The same projection through host arrays.
-    ctx.set_vec(fsl.GRID_VELOCITY, u_h, np.ones(ctx.box, np.uint8))
-    ctx.project_forward(tight())
-    up_host, _ = ctx.get_vec(fsl.GRID_VELOCITY)
-    g_host, _ = ctx.project_backward(tight(), w_h)
-    assert np.array_equal(up_dev, up_host)
-    assert np.array_equal(g_dev, g_host)
-    ctx.close()
+    here = os.path.dirname(os.path.abspath(__file__))
+    root = os.path.dirname(here)
+    script = _DEVICE_AUTOGRAD.format(root=root, pkg=os.path.join(root, "fluid-sim-lb_amd"),
+                                     tests=here)
+    r = subprocess.run([sys.executable, "-c", script], capture_output=True, text=True,
+                       timeout=240)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0 and "device autograd ok" in r.stdout
 
 
 def test_projection_with_ic_matches_jacobi():
