@@ -937,6 +937,18 @@ __device__ inline uint32_t load_labels(const uint8_t *lab, Off i) {
   else if constexpr (N == 2) return *reinterpret_cast<const uint16_t *>(&elem(lab, i));
   else return elem(lab, i);
 }
+// The same pair as one agent-scope relaxed atomic store, which the hardware
+// writes through the L2 (sc1): k_cg stores r, p and x this way, so a launch
+// ends with none of their lines dirty in the L2 (at 256^3 0.8 us per launch
+// faster than write-back stores; at 512^3 1% slower).
+template <typename T, typename Off>
+__device__ inline void store_pair_wt(T *p, Off i, T a, T b) {
+  static_assert(sizeof(T) == 4, "float pairs");
+  union { T f[2]; unsigned long long u; } v;
+  v.f[0] = a; v.f[1] = b;
+  __hip_atomic_store(reinterpret_cast<unsigned long long *>(&elem(p, i)), v.u, __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
 template <typename T, typename Off>
 __device__ inline void store_pair(T *p, Off i, T a, T b) {
   typedef T VT __attribute__((ext_vector_type(2)));
@@ -1247,10 +1259,16 @@ __global__ __launch_bounds__(kG * GR) void k_cg(Dom d, const int32_t *flist, int
         *xp = acc64;
         store_pair(x, v, V(0), V(0));
       } else if (mode & 1) {
-        store_pair(x, v, xn[0], xn[1]);
+        if constexpr (sizeof(V) == 4) store_pair_wt(x, v, xn[0], xn[1]);
+        else store_pair(x, v, xn[0], xn[1]);
       }
-      store_pair(rout, v, rn[0], rn[1]);
-      store_pair(pout, v, wp[0], wp[1]);
+      if constexpr (sizeof(V) == 4) {
+        store_pair_wt(rout, v, rn[0], rn[1]);
+        store_pair_wt(pout, v, wp[0], wp[1]);
+      } else {
+        store_pair(rout, v, rn[0], rn[1]);
+        store_pair(pout, v, wp[0], wp[1]);
+      }
       if (m1ok) {
         const uint8_t lb = tl1[m1];
         const V di = s_dinv[lb & 31];
