@@ -282,6 +282,54 @@ def test_rollout_gradient_sphere_drop_128():
     ctx.close()
 
 
+def test_projection_gradient_sphere_drop_128_fp64_fd():
+    """The 128^3 sphere drop's labels (one step's stages): the gradient of
+    L(u) = sum over the 16^3 target box of |P u|^2 through the GPU adjoint
+    (2 P^T (mask P u), FP64 solves), against central finite differences with
+    eps = 1e-3 of the FP64 oracle's own projection of u on the same labels.
+    L is quadratic in u, so the central difference has no truncation error:
+    the check measures the GPU gradient against the FP64 reference."""
+    dims, mk, seed = scenes.WORKLOADS["sphere_drop_128"]
+    marker = forward_labels(dims, mk(dims), seed)
+    ctx = projection_ctx(dims, marker)
+    rng = np.random.default_rng(12)
+    u = (0.5 * rng.standard_normal(ctx.box + (3,))).astype(np.float32)
+    tmask = target_box(ctx.box, (56, 56, 56), (72, 72, 72))
+    pu = project(ctx, u)
+    g, st = ctx.project_backward(tight(), 2.0 * pu * tmask)
+    ctx.close()
+    assert st["true_rel_residual"] <= 1e-9
+    g = g.astype(np.float64)
+    o = Oracle(dims, fp64=True)
+    o.set_params(9.8, (0.0, 9.8, 0.0), 0.95)
+    o.set_box_boundary()
+    o.set_marker(marker)
+    act = np.ones(o.box, np.uint8)
+    zero = np.zeros(o.box)
+
+    def L(uu):
+        o.set_vec(0, uu, act)
+        o.divergence()
+        o.set_scalar(1, zero)
+        rc, _, _ = o.solve(precision=1, stop_mode=1, threshold=1e-14, max_iter=20000)
+        assert rc == 0
+        o.pressure_update()
+        return loss_of(o.get_vec(0)[0], tmask)
+
+    u64 = u.astype(np.float64)
+    assert abs(L(u64) - loss_of(pu, tmask)) <= 1e-5 * loss_of(pu, tmask)
+    gnorm = float(np.linalg.norm(g))
+    eps = 1e-3
+    dirs = [g / gnorm] + [rng.standard_normal(u.shape) for _ in range(2)]
+    for t, e in enumerate(dirs):
+        fd = (L(u64 + eps * e) - L(u64 - eps * e)) / (2 * eps)
+        ad = float(np.sum(g * e))
+        scale = abs(fd) if t == 0 else gnorm
+        print(f"128^3 projection dL/du . e{t}: adjoint {ad:.8e} FP64 FD {fd:.8e} "
+              f"(scale {scale:.3e})")
+        assert abs(ad - fd) <= 1e-4 * scale
+
+
 def test_autograd_projection_matches_c_abi():
     import torch
 
