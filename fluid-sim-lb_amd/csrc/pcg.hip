@@ -784,17 +784,72 @@ constexpr int kCgMaxPerCu = 4;
 // 256-thread workgroup per leaf group, at most kCgMaxPerCu rounds.
 constexpr int kCgPartRounds = kCgMaxPerCu;
 
-// Tile P over [-2, 10)^3 of the leaf and tile U over [-1, 9)^3, with row
-// strides padded so that the wave access patterns of the stencil and of the
-// halo faces spread over the LDS banks (x strides 151 and 137 words instead
-// of 144 and 100: about a third fewer bank-conflict cycles for 32 or 64
-// banks).
-constexpr int kPX = 151, kPY = 12, kPSize = 11 * kPX + 11 * kPY + 12;
-constexpr int kUX = 137, kUY = 12, kUSize = 9 * kUX + 9 * kUY + 10;
+// Tile P over [-2, 10)^3 of the leaf and tile U over [-1, 9)^3. Every stride
+// is even and a cell at even z sits at an even index, so the z-pairs a
+// thread owns (and the halo pairs, which start at even z) are aligned: each
+// moves as one 8-byte (FP64: 16-byte) LDS access. The x strides (174 and 142
+// words, against the 144 and 120 of unpadded rows) spread the pair accesses
+// of a wavefront over the banks.
+constexpr int kPX = 174, kPY = 12, kPSize = 11 * kPX + 11 * kPY + 12;
+constexpr int kUX = 142, kUY = 12, kUSize = 9 * kUX + 9 * kUY + 12;
+static_assert(kPX % 2 == 0 && kPY % 2 == 0 && kUX % 2 == 0 && kUY % 2 == 0 && kPSize % 2 == 0 &&
+                  kUSize % 2 == 0,
+              "aligned z-pairs");
 __device__ inline int t12(int x, int y, int z) { return (x + 2) * kPX + (y + 2) * kPY + (z + 2); }
 __device__ inline int t12_of_local(int c) { return t12(c >> 6, (c >> 3) & 7, c & 7); }
-__device__ inline int u10(int x, int y, int z) { return (x + 1) * kUX + (y + 1) * kUY + (z + 1); }
+__device__ inline int u10(int x, int y, int z) { return (x + 1) * kUX + (y + 1) * kUY + (z + 2); }
 __device__ inline int u10_of_local(int c) { return u10(c >> 6, (c >> 3) & 7, c & 7); }
+
+// Elements i and i + 1 of an LDS tile (i even, so the pair is aligned) as
+// one access.
+template <typename V>
+__device__ inline void lds_get2(const V *p, int i, V &a, V &b) {
+  typedef V V2 __attribute__((ext_vector_type(2)));
+  const V2 v = *reinterpret_cast<const V2 *>(p + i);
+  a = v[0];
+  b = v[1];
+}
+template <typename V>
+__device__ inline void lds_put2(V *p, int i, V a, V b) {
+  typedef V V2 __attribute__((ext_vector_type(2)));
+  V2 v;
+  v[0] = a;
+  v[1] = b;
+  *reinterpret_cast<V2 *>(p + i) = v;
+}
+
+// A p on a thread's own z-pair, cells c and c + 1 of the P tile (c even),
+// with the additions of stencil_s in its order: the x and y neighbours as
+// pair reads, the z neighbours outside the pair as single reads, and the
+// pair itself from pc (the values the tile holds there).
+template <typename V, bool ORD>
+__device__ inline void stencil_pair(const V *sp, int c, const V pc[2], const int diag[2],
+                                    V out[2]) {
+#pragma clang fp contract(off)
+  V xm[2], xp[2], ym[2], yp[2];
+  lds_get2(sp, c - kPX, xm[0], xm[1]);
+  lds_get2(sp, c + kPX, xp[0], xp[1]);
+  lds_get2(sp, c - kPY, ym[0], ym[1]);
+  lds_get2(sp, c + kPY, yp[0], yp[1]);
+  const V zm[2] = {sp[c - 1], pc[0]}, zp[2] = {pc[1], sp[c + 2]};
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    if (ORD) {
+      V s = V(0);
+      s = s + (-xm[i]);
+      s = s + (-ym[i]);
+      s = s + (-zm[i]);
+      { const V m = V(diag[i]) * pc[i]; s = s + m; }
+      s = s + (-zp[i]);
+      s = s + (-yp[i]);
+      s = s + (-xp[i]);
+      out[i] = s;
+    } else {
+      const V nb = ((xm[i] + xp[i]) + (ym[i] + yp[i])) + (zm[i] + zp[i]);
+      out[i] = fmav(V(diag[i]), pc[i], -nb);
+    }
+  }
+}
 
 // Halo cell m of the fused pass, relative to the leaf origin. m in [0, 384):
 // face layers at distance 1 (face m/64: x-, x+, y-, y+, z-, z+); m in
@@ -975,17 +1030,19 @@ __device__ inline void store_pair(T *p, Off i, T a, T b) {
 // p_k on the leaf and its halo in the P tile and hands the r and labels of
 // the upper distance-1 cells to a balanced mapping (48 per wave) through
 // LDS; phase 2 applies A to p_k there and on the own cells and forms
-// u_{k+1}; phase 3 accumulates gamma, delta (edge form) and r.r.
+// u_{k+1}; phase 3 accumulates gamma, delta (edge form) and r.r. At most
+// 128 VGPRs, so that four workgroups fit on a CU (kCgMaxPerCu) in FP64 too.
 template <typename V, typename A, bool ORD, int GR, bool OFF32>
-__global__ __launch_bounds__(kG * GR) void k_cg(Dom d, const int32_t *flist, int64_t nf,
+__global__ __launch_bounds__(kG * GR) __attribute__((amdgpu_waves_per_eu(kCgMaxPerCu * GR)))
+void k_cg(Dom d, const int32_t *flist, int64_t nf,
                                                 const uint8_t *lab, const V *rin, V *rout,
                                                 const V *pin, V *pout, V *x, double *x64,
                                                 const A *part_in, A *part_out, int g_parts,
                                                 int k, int mode, PcgState *st) {
 #pragma clang fp contract(off)
   using Off = typename std::conditional<OFF32, uint32_t, int64_t>::type;
-  __shared__ V s_p[GR][kPSize];
-  __shared__ V s_u[GR][kUSize];
+  __shared__ __align__(16) V s_p[GR][kPSize];
+  __shared__ __align__(16) V s_u[GR][kUSize];
   __shared__ V s_r1[GR][192];        // r of the upper distance-1 cells
   __shared__ uint8_t s_l1[GR][192];  // their labels
   // Leaf schedule: entry j is the list entry of round j of this group
@@ -1200,20 +1257,21 @@ __global__ __launch_bounds__(kG * GR) void k_cg(Dom d, const int32_t *flist, int
       wo[i] = cur.op[i];
     }
     if (valid) {
-      tp[op12] = wp[0];
-      tp[op12 + 1] = wp[1];
+      lds_put2(tp, op12, wp[0], wp[1]);
 #pragma unroll
       for (int s = 0; s < 2; ++s) {
         if (!hon[s]) continue;
+        V hv[2];
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
           const uint32_t lb = cur.hl[s] >> (8 * e);
-          tp[hp0[s] + e] = pnew(cur.hr[s][e], cur.hp[s][e], lb, hd[s][e]);
+          hv[e] = pnew(cur.hr[s][e], cur.hp[s][e], lb, hd[s][e]);
           if ((rmask[s] >> e) & 1) {
             tr1[rbase[s] + e] = cur.hr[s][e];
             tl1[rbase[s] + e] = uint8_t(lb);
           }
         }
+        lds_put2(tp, hp0[s], hv[0], hv[1]);
       }
     }
     q.next();
@@ -1232,14 +1290,18 @@ __global__ __launch_bounds__(kG * GR) void k_cg(Dom d, const int32_t *flist, int
     // u_{k+1} there; x, r and p of the own pair are written.
     V rn[2], un[2];
     if (valid) {
-      V xn[2];
+      V xn[2], aps[2];
+      {
+        const int dg[2] = {wl[0] >> 2, wl[1] >> 2};
+        stencil_pair<V, ORD>(tp, op12, wp, dg, aps);
+      }
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
         rn[i] = V(0);
         un[i] = V(0);
         xn[i] = wx[i];
         if (is_fluid(wl[i])) {
-          const V ap = stencil_s<V, ORD, kPX, kPY>(tp, op12 + i, wl[i] >> 2);
+          const V ap = aps[i];
           // x: alpha_{k-1} p_{k-1} (when pending) then alpha_k p_k, the two
           // axpys of consecutive iterations in order.
           if (ORD) {
@@ -1258,8 +1320,8 @@ __global__ __launch_bounds__(kG * GR) void k_cg(Dom d, const int32_t *flist, int
           }
           un[i] = rn[i] * wd[i];
         }
-        tu[ou10 + i] = un[i];
       }
+      lds_put2(tu, ou10, un[0], un[1]);
       const Off v = base + Off(2 * t);
       if (mode & 4) {
         // Fold before a residual refresh: x64 += x_{k+1}, x = 0 (x and x64
@@ -1306,12 +1368,17 @@ __global__ __launch_bounds__(kG * GR) void k_cg(Dom d, const int32_t *flist, int
     // symmetric; u is 0 off the rows). FP64 accumulation, one rounding per
     // term.
     if (valid) {
+      // The +x and +y neighbours as pair reads; the +z neighbour of cell 2t
+      // is cell 2t+1 (un[1], the value the tile holds there).
+      V ux[2], uy[2];
+      lds_get2(tu, ou10 + kUX, ux[0], ux[1]);
+      lds_get2(tu, ou10 + kUY, uy[0], uy[1]);
+      const V uz[2] = {un[1], tu[ou10 + 2]};
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
         if (!is_fluid(wl[i])) continue;
-        const int c = ou10 + i;
         const A uc = A(un[i]);
-        const A sn = (A(tu[c + kUX]) + A(tu[c + kUY])) + A(tu[c + 1]);
+        const A sn = (A(ux[i]) + A(uy[i])) + A(uz[i]);
         const A tt = fmav(A(wl[i] >> 2), uc, A(-2) * sn);
         acc[0] = fmav(A(rn[i]), uc, acc[0]);
         acc[1] = fmav(uc, tt, acc[1]);
