@@ -618,7 +618,10 @@ __global__ __launch_bounds__(kP2gThreads) void k_p2g_own(
 // dz) order; then normalises (flip_simulation.cc:900-922) and marks FLUID
 // (flip_simulation.cc:852-857). 128 threads per leaf, each a run of four
 // voxels along z, so a thread's own values move as 16-byte (float) and
-// 4-byte (label) accesses.
+// 4-byte (label) accesses. The neighbours holding particles are found once
+// per leaf (one ballot per wavefront), and each voxel's ring entries (up to
+// seven neighbours) are loaded together before they are added in (dx, dy,
+// dz) order: one round trip per voxel instead of one per neighbour.
 constexpr int kCombineThreads = 128;
 template <bool ADJ>
 __global__ __launch_bounds__(kCombineThreads) void k_p2g_combine(
@@ -648,32 +651,65 @@ __global__ __launch_bounds__(kCombineThreads) void k_p2g_combine(
     }
     const int64_t l2 = d.L[2], l12 = int64_t(d.L[1]) * d.L[2];
     const int L[3] = {int(leaf / l12), int((leaf / l2) % d.L[1]), int(leaf % l2)};
+    // Neighbour leaves holding particles: bit (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1)
+    // (lanes 0-26 of each wavefront test one neighbour each).
+    uint32_t nbm;
     {
+      const int j = int(threadIdx.x & 63);
+      bool has = false;
+      if (j < 27 && j != 13) {
+        const int dx = j / 9 - 1, dy = (j / 3) % 3 - 1, dz = j % 3 - 1;
+        if (L[0] + dx >= 0 && L[0] + dx < d.L[0] && L[1] + dy >= 0 && L[1] + dy < d.L[1] &&
+            L[2] + dz >= 0 && L[2] + dz < d.L[2])
+          has = lhas[leaf + dx * l12 + dy * l2 + dz] != 0;
+      }
+      nbm = uint32_t(__ballot(has));
+    }
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int lc[3] = {lx, ly, lz0 + u};
-        int lo[3], hi[3];
+    for (int u = 0; u < 4; ++u) {
+      const int lc[3] = {lx, ly, lz0 + u};
+      // Per axis the neighbour offsets this voxel reaches, in increasing order:
+      // option 0 is -1 on a lower face, else 0; option 1 (faces only) is 0 or +1.
+      int o0[3];
+      bool face[3];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          lo[a] = lc[a] == 0 && L[a] > 0 ? -1 : 0;
-          hi[a] = lc[a] == 7 && L[a] + 1 < d.L[a] ? 1 : 0;
+      for (int a = 0; a < 3; ++a) {
+        face[a] = lc[a] == 0 || lc[a] == 7;
+        o0[a] = lc[a] == 0 ? -1 : 0;
+      }
+      // The loads of all (up to 7) neighbours first, then the sums in (dx, dy, dz) order.
+      float rv[8][6];
+      uint32_t rt[8];
+      bool ok[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int ix = q >> 2, iy = (q >> 1) & 1, iz = q & 1;
+        const int dx = o0[0] + ix, dy = o0[1] + iy, dz = o0[2] + iz;
+        ok[q] = (ix == 0 || face[0]) && (iy == 0 || face[1]) && (iz == 0 || face[2]) &&
+                (((nbm >> ((dx + 1) * 9 + (dy + 1) * 3 + (dz + 1))) & 1u) != 0);
+        if (ok[q]) {
+          const int64_t nb = leaf + dx * l12 + dy * l2 + dz;
+          const int64_t e =
+              nb * kRing + ring_index(lc[0] + 1 - 8 * dx, lc[1] + 1 - 8 * dy, lc[2] + 1 - 8 * dz);
+#pragma unroll
+          for (int a = 0; a < 3; ++a) rv[q][a] = ring[a * rstride + e];
+          if (!ADJ) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) rv[q][3 + a] = ring[(3 + a) * rstride + e];
+            rt[q] = ring_t[e];
+          }
         }
-        for (int dx = lo[0]; dx <= hi[0]; ++dx)
-          for (int dy = lo[1]; dy <= hi[1]; ++dy)
-            for (int dz = lo[2]; dz <= hi[2]; ++dz) {
-              if (dx == 0 && dy == 0 && dz == 0) continue;
-              const int64_t nb = leaf + dx * l12 + dy * l2 + dz;
-              if (!lhas[nb]) continue;
-              const int64_t e =
-                  nb * kRing + ring_index(lc[0] + 1 - 8 * dx, lc[1] + 1 - 8 * dy, lc[2] + 1 - 8 * dz);
+      }
 #pragma unroll
-              for (int a = 0; a < 3; ++a) vw[a][u] += ring[a * rstride + e];
-              if (!ADJ) {
+      for (int q = 0; q < 8; ++q) {
+        if (!ok[q]) continue;
 #pragma unroll
-                for (int a = 0; a < 3; ++a) w[a][u] += ring[(3 + a) * rstride + e];
-                touch |= uint32_t(ring_t[e]) << (8 * u);
-              }
-            }
+        for (int a = 0; a < 3; ++a) vw[a][u] += rv[q][a];
+        if (!ADJ) {
+#pragma unroll
+          for (int a = 0; a < 3; ++a) w[a][u] += rv[q][3 + a];
+          touch |= rt[q] << (8 * u);
+        }
       }
     }
     if (ADJ) {

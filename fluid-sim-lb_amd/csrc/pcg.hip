@@ -28,9 +28,10 @@
 //   FP64  - double everywhere.
 //   MIXED - float vectors and stencil, double dot products (exact products
 //           accumulated by FMA in each thread, then DPP trees and fixed-order
-//           workgroup and cross-workgroup sums in double: for these sums of
-//           positive terms the rounding stays below about 1e-14 relative; the
-//           two-pass path's totals are Neumaier-compensated), and every `refresh`
+//           workgroup and cross-workgroup sums in double: r.z and r.r are
+//           sums of positive terms, for which the rounding stays below about
+//           1e-14 relative; the two-pass path's totals are
+//           Neumaier-compensated), and every `refresh`
 //           iterations the float iterate is folded into a double solution and
 //           the true residual b - A x is recomputed in double (a reliable
 //           update; CG continues with the same direction).
