@@ -1410,8 +1410,8 @@ __global__ __launch_bounds__(kG) void k_refresh_cg(Dom d, const int32_t *flist, 
                                                    double *part_odd, PcgState *st, int force) {
 #pragma clang fp contract(off)
   using Off = typename std::conditional<OFF32, uint32_t, int64_t>::type;
-  __shared__ double s_p[kPSize];
-  __shared__ float s_u[kUSize];
+  __shared__ __align__(16) double s_p[kPSize];
+  __shared__ __align__(16) float s_u[kUSize];
   __shared__ float s_b1[192];
   __shared__ uint8_t s_l1[192];
   __shared__ float s_dinv[32];
@@ -1459,56 +1459,76 @@ __global__ __launch_bounds__(kG) void k_refresh_cg(Dom d, const int32_t *flist, 
   }
   const int op12 = t12_of_local(2 * t), ou10 = u10_of_local(2 * t);
   double acc[3] = {0.0, 0.0, 0.0};
-  for (Walk q = walk(nf); q.more(); q.next()) {
-    // Every round has a valid leaf: one group per workgroup.
-    const Off base = Off(flist[q.mine()]) << 9;
-    const Off v = base + Off(2 * t);
-    const uint32_t ol = load_labels<2>(lab, v);
-    double ox[2];
-    float ob[2];
-    load_run<2>(x64, v, ox);
-    load_run<2>(div, v, ob);
-    double hx[2][2];
-    float hb[2][2];
-    uint32_t hl[2];
+  // One leaf's loads, issued a round ahead of its arithmetic.
+  struct RPre {
+    double ox[2], hx[2][2];
+    float ob[2], hb[2][2];
+    uint32_t ol, hl[2];
+    Off base;
+  };
+  auto rload = [&](int32_t leaf, RPre &c) {
+    c.base = Off(leaf) << 9;
+    const Off v = c.base + Off(2 * t);
+    c.ol = load_labels<2>(lab, v);
+    load_run<2>(x64, v, c.ox);
+    load_run<2>(div, v, c.ob);
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
-      const Off hv = base + Off(hoff[s]);
-      hl[s] = load_labels<2>(lab, hv);
-      load_run<2>(x64, hv, hx[s]);
-      load_run<2>(div, hv, hb[s]);
+      const Off hv = c.base + Off(hoff[s]);
+      c.hl[s] = load_labels<2>(lab, hv);
+      load_run<2>(x64, hv, c.hx[s]);
+      load_run<2>(div, hv, c.hb[s]);
     }
-    // Phase 1: x64 on the 1160 cells (0 off the rows) into the tile.
+  };
+  Walk q = walk<kG>(nf);
+  RPre cur;
+  if (q.more()) rload(flist[q.mine()], cur);
+  while (q.more()) {
+    // Every round has a valid leaf: one group per workgroup.
+    const Off v = cur.base + Off(2 * t);
+    // Phase 1: x64 on the 1160 cells (0 off the rows) into the tile, as pairs.
     uint8_t wl[2];
+    float ob[2];
+    double px[2];
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
-      wl[i] = uint8_t(ol >> (8 * i));
-      s_p[op12 + i] = is_fluid(wl[i]) ? ox[i] : 0.0;
+      wl[i] = uint8_t(cur.ol >> (8 * i));
+      ob[i] = cur.ob[i];
+      px[i] = is_fluid(wl[i]) ? cur.ox[i] : 0.0;
     }
+    lds_put2(s_p, op12, px[0], px[1]);
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
       if (!hon[s]) continue;
+      double hv[2];
 #pragma unroll
       for (int e = 0; e < 2; ++e) {
-        const uint8_t lb = uint8_t(hl[s] >> (8 * e));
-        s_p[hp0[s] + e] = is_fluid(lb) ? hx[s][e] : 0.0;
+        const uint8_t lb = uint8_t(cur.hl[s] >> (8 * e));
+        hv[e] = is_fluid(lb) ? cur.hx[s][e] : 0.0;
         if ((rmask[s] >> e) & 1) {
-          s_b1[rbase[s] + e] = hb[s][e];
+          s_b1[rbase[s] + e] = cur.hb[s][e];
           s_l1[rbase[s] + e] = lb;
         }
       }
+      lds_put2(s_p, hp0[s], hv[0], hv[1]);
     }
+    q.next();
+    if (q.more()) rload(flist[q.mine()], cur);
     __syncthreads();
     // Phase 2: r = b - A x64 (FP64) on the own pair and the upper distance-1 cells.
     float rn[2], un[2];
+    {
+      double aps[2];
+      const int dg[2] = {wl[0] >> 2, wl[1] >> 2};
+      stencil_pair<double, false>(s_p, op12, px, dg, aps);
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      rn[i] = 0.0f;
-      if (is_fluid(wl[i]))
-        rn[i] = float(double(ob[i]) - stencil_s<double, false, kPX, kPY>(s_p, op12 + i, wl[i] >> 2));
-      un[i] = rn[i] * s_dinv[wl[i] & 31];
-      s_u[ou10 + i] = un[i];
+      for (int i = 0; i < 2; ++i) {
+        rn[i] = 0.0f;
+        if (is_fluid(wl[i])) rn[i] = float(double(ob[i]) - aps[i]);
+        un[i] = rn[i] * s_dinv[wl[i] & 31];
+      }
     }
+    lds_put2(s_u, ou10, un[0], un[1]);
     store_pair(rout, v, rn[0], rn[1]);
     if (m1ok) {
       const uint8_t lb = s_l1[m1];
@@ -1519,19 +1539,25 @@ __global__ __launch_bounds__(kG) void k_refresh_cg(Dom d, const int32_t *flist, 
       s_u[r1u] = uh;
     }
     __syncthreads();
-    // Phase 3: gamma, delta (edge form, as in k_cg), r.r in FP64.
+    // Phase 3: gamma, delta (edge form, as in k_cg), r.r in FP64. The next
+    // round's phase 1 writes only the P tile, which this phase does not
+    // read, and its barrier comes before the next phase 2 writes the U tile.
+    {
+      float ux[2], uy[2];
+      lds_get2(s_u, ou10 + kUX, ux[0], ux[1]);
+      lds_get2(s_u, ou10 + kUY, uy[0], uy[1]);
+      const float uz[2] = {un[1], s_u[ou10 + 2]};
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      if (!is_fluid(wl[i])) continue;
-      const int c = ou10 + i;
-      const double uc = double(un[i]);
-      const double sn = (double(s_u[c + kUX]) + double(s_u[c + kUY])) + double(s_u[c + 1]);
-      const double tt = fmav(double(wl[i] >> 2), uc, -2.0 * sn);
-      acc[0] = fmav(double(rn[i]), uc, acc[0]);
-      acc[1] = fmav(uc, tt, acc[1]);
-      acc[2] = fmav(double(rn[i]), double(rn[i]), acc[2]);
+      for (int i = 0; i < 2; ++i) {
+        if (!is_fluid(wl[i])) continue;
+        const double uc = double(un[i]);
+        const double sn = (double(ux[i]) + double(uy[i])) + double(uz[i]);
+        const double tt = fmav(double(wl[i] >> 2), uc, -2.0 * sn);
+        acc[0] = fmav(double(rn[i]), uc, acc[0]);
+        acc[1] = fmav(uc, tt, acc[1]);
+        acc[2] = fmav(double(rn[i]), double(rn[i]), acc[2]);
+      }
     }
-    __syncthreads();
   }
   publish<double, 3>(acc, part_out);
 }
