@@ -291,11 +291,25 @@ __global__ void k_gather(uint32_t n, const uint32_t *n_dev, uint32_t *count_out,
     n = *n_dev;
     if (blockIdx.x == 0 && threadIdx.x == 0) *count_out = n;
   }
-  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  // Four consecutive destinations per thread: one 16-byte index load and one
+  // 16-byte store per array (every array is its own allocation, so element
+  // 4k is 16-byte aligned); the tail goes element by element.
+  const uint32_t s = (blockIdx.x * blockDim.x + threadIdx.x) * 4u;
   if (s >= n) return;
-  const uint32_t q = perm[s];
+  if (s + 3u < n) {
+    const uint4 q = *reinterpret_cast<const uint4 *>(perm + s);
 #pragma unroll
-  for (int a = 0; a < NA; ++a) out.a[a][s] = in.a[a][q];
+    for (int a = 0; a < NA; ++a) {
+      const float4 v = make_float4(in.a[a][q.x], in.a[a][q.y], in.a[a][q.z], in.a[a][q.w]);
+      *reinterpret_cast<float4 *>(out.a[a] + s) = v;
+    }
+    return;
+  }
+  for (uint32_t i = s; i < n; ++i) {
+    const uint32_t q = perm[i];
+#pragma unroll
+    for (int a = 0; a < NA; ++a) out.a[a][i] = in.a[a][q];
+  }
 }
 
 // ------------------------------------------------------------- leaf lists
@@ -1005,10 +1019,10 @@ hipError_t launch_sort_tail(fsl_ctx *c, uint32_t n_alive, bool keep_gv) {
   if (nb > 0) {
     k_cell_sort<<<unsigned(c->nleaves), 512, 0, c->stream>>>(c->vstart, c->perm);
     if (keep_gv)
-      k_gather<9><<<blocks_for(nb, kThreads), kThreads, 0, c->stream>>>(n_alive, na_dev, out,
+      k_gather<9><<<blocks_for((nb + 3) / 4, kThreads), kThreads, 0, c->stream>>>(n_alive, na_dev, out,
                                                                          c->perm, c->P, c->Q);
     else
-      k_gather<6><<<blocks_for(nb, kThreads), kThreads, 0, c->stream>>>(n_alive, na_dev, out,
+      k_gather<6><<<blocks_for((nb + 3) / 4, kThreads), kThreads, 0, c->stream>>>(n_alive, na_dev, out,
                                                                          c->perm, c->P, c->Q);
   }
   Particles t = c->P;
