@@ -208,15 +208,32 @@ constexpr int kRawBufferWord3 = 0x00020000;
 __global__ void k_scatter(int64_t np, const uint32_t *np_dev, const uint32_t *key,
                           const uint32_t *vstart, uint32_t *vcount, uint32_t *perm) {
   if (np_dev) np = *np_dev;
-  const int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  const uint32_t k = q < np ? key[q] : kDead;
-  const KeyRun r = key_run(k);
-  uint32_t left = 0;  // slots of the cell still free before this run
-  if (r.rank == 0 && k != kDead) left = atomicSub(&vcount[k], uint32_t(r.count));
-  left = __shfl(left, r.head);
-  if (k == kDead) return;
-  const uint32_t b = vstart[k], total = vstart[k + 1] - b;
-  perm[b + (total - left) + uint32_t(r.rank)] = uint32_t(q);
+  // Four independent windows of particles per thread (q, q + T, q + 2T,
+  // q + 3T for T threads), so four atomics are in flight per wavefront.
+  const int64_t nt = int64_t(gridDim.x) * blockDim.x;
+  const int64_t q0 = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  uint32_t k[4], left[4];
+  KeyRun r[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int64_t q = q0 + j * nt;
+    k[j] = q < np ? key[q] : kDead;
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) r[j] = key_run(k[j]);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    left[j] = 0;  // slots of the cell still free before this run
+    if (r[j].rank == 0 && k[j] != kDead) left[j] = atomicSub(&vcount[k[j]], uint32_t(r[j].count));
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) left[j] = __shfl(left[j], r[j].head);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    if (k[j] == kDead) continue;
+    const uint32_t b = vstart[k[j]], total = vstart[k[j] + 1] - b;
+    perm[b + (total - left[j]) + uint32_t(r[j].rank)] = uint32_t(q0 + j * nt);
+  }
 }
 
 // Restore the previous order inside each cell (stable sort): one workgroup
@@ -1013,7 +1030,7 @@ hipError_t launch_sort_tail(fsl_ctx *c, uint32_t n_alive, bool keep_gv) {
   uint32_t *out = dev ? c->dcount + kCountParticles : nullptr;
   const int64_t nb = dev ? c->np : int64_t(n_alive);
   if (c->np > 0) {
-    k_scatter<<<blocks_for(c->np, kThreads), kThreads, 0, c->stream>>>(
+    k_scatter<<<blocks_for((c->np + 3) / 4, kThreads), kThreads, 0, c->stream>>>(
         c->np, np_dev, c->key, c->vstart, c->vcount, c->perm);
   }
   if (nb > 0) {
